@@ -1,0 +1,57 @@
+/* -----------------------------------------------------------------------------
+ * massaction_mi355.h -- the mass-action workload on the MI355X: device copy of
+ * the problem, right-hand side and block Jacobian as HIP kernels wrapped as
+ * CVODE callbacks (CVRhsFn / CVLsJacFn, include/cvode/cvode.h:93,
+ * include/cvode/cvode_ls.h:54), and a driver that integrates all cells as one
+ * coupled system through the N_Vector / block-LU drop-ins -- the structure of
+ * the reference's batched example (examples/cvode/magma/
+ * cv_bruss_batched_magma.cpp: one vector holding every cell, a block-diagonal
+ * Jacobian, BDF + Newton + batched direct solve).
+ * ---------------------------------------------------------------------------*/
+#ifndef SUNAMD_MASSACTION_MI355_H
+#define SUNAMD_MASSACTION_MI355_H
+
+#include "massaction.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct SUNAMD_MassActionDev_ SUNAMD_MassActionDev;
+
+/* copy the problem to HBM; kernels run on `stream` (a hipStream_t) */
+SUNAMD_MassActionDev* SUNAMD_MassActionToDevice(const SUNAMD_MassAction* p, void* stream);
+void SUNAMD_MassActionDevFree(SUNAMD_MassActionDev* d);
+
+/* CVODE callbacks: y, ydot are N_VNew_Mi355 vectors, J a SUNMatrix_Mi355BlockDense */
+int SUNAMD_MassActionRhsDev(sunrealtype t, N_Vector y, N_Vector ydot, void* user_data);
+int SUNAMD_MassActionJacDev(sunrealtype t, N_Vector y, N_Vector fy, SUNMatrix J,
+                            void* user_data, N_Vector tmp1, N_Vector tmp2, N_Vector tmp3);
+
+/* raw device-array evaluation (ncells*m vectors, ncells*m*m blocks) */
+int SUNAMD_MassActionRhsDevArrays(SUNAMD_MassActionDev* d, const double* dy, double* df);
+int SUNAMD_MassActionJacDevArrays(SUNAMD_MassActionDev* d, const double* dy, double* dJ);
+
+typedef struct
+{
+  long int nst, nfe, nsetups, netf, nni, nnf, ncfn, nje;
+  int qlast, qcur;
+  double hinused, hlast, hcur, tcur;
+  double wall_s;     /* wall time of the CVode call (device idle at both ends) */
+  long int launches; /* N_Vector kernel launches during the call */
+  int flag;          /* CVode return flag */
+} SUNAMD_SolveStats;
+
+/* integrate every cell of p from t = 0 to tout as one coupled system:
+   N_VNew_Mi355 + SUNMatrix_Mi355BlockDense + SUNLinSol_Mi355BlockLU
+   (lu_precision SUNAMD_LU_FP64 / SUNAMD_LU_FP32), scalar tolerances.
+   yout (ncells*m, host) receives y(tout). */
+int SUNAMD_MassActionSolveCoupled_Mi355(const SUNAMD_MassAction* p, double rtol,
+                                        double atol, double tout, int lu_precision,
+                                        double* yout, SUNAMD_SolveStats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

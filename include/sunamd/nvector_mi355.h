@@ -1,0 +1,60 @@
+/* -----------------------------------------------------------------------------
+ * nvector_mi355.h -- N_Vector whose data lives in MI355X (gfx950) HBM.
+ *
+ * Fills the reference N_Vector ops table (include/sundials/sundials_nvector.h:
+ * 101-195) so CVODE / ARKODE / KINSOL use it unchanged.  It replaces the
+ * reference HIP vector's constructors (src/nvector/hip/nvector_hip.hip.cpp:
+ * N_VNew_Hip :220, N_VMake_Hip :370, N_VCopyToDevice_Hip / N_VCopyFromDevice_Hip
+ * :500-560) with the same roles:
+ *   - every standard op, the fused LinearCombination / ScaleAddMulti /
+ *     DotProdMulti and the vector-array ops are device kernels;
+ *   - reductions finish on the device and hand one value back through a
+ *     pinned host slot (one stream synchronisation per reduction, no
+ *     per-call allocation or pointer-array copies);
+ *   - N_VGetArrayPointer returns a host mirror (allocated on first use),
+ *     N_VGetDeviceArrayPointer the HBM array; the mirror is synchronised only
+ *     by N_VCopyToDevice_Mi355 / N_VCopyFromDevice_Mi355.
+ * N_VGetVectorID returns SUNDIALS_NVEC_CUSTOM.
+ * Elementwise ops round exactly like the reference serial vector (same
+ * formula per element, no fused multiply-add); reductions sum in a fixed
+ * tree order, so they are deterministic but not bit-equal to a sequential sum.
+ * ---------------------------------------------------------------------------*/
+#ifndef SUNAMD_NVECTOR_MI355_H
+#define SUNAMD_NVECTOR_MI355_H
+
+#include "sundials_core.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* new vector of the given length (device data allocated, host mirror lazy) */
+N_Vector N_VNew_Mi355(sunindextype length, SUNContext sunctx);
+/* vector with no data */
+N_Vector N_VNewEmpty_Mi355(sunindextype length, SUNContext sunctx);
+/* wrap caller-owned device (and optionally host) arrays */
+N_Vector N_VMake_Mi355(sunindextype length, sunrealtype* h_vdata,
+                       sunrealtype* d_vdata, SUNContext sunctx);
+
+/* host <-> device copies of the whole vector (synchronous) */
+SUNErrCode N_VCopyToDevice_Mi355(N_Vector v);
+SUNErrCode N_VCopyFromDevice_Mi355(N_Vector v);
+/* copy raw host data in / out without a host mirror (synchronous) */
+SUNErrCode N_VCopyFromHostArray_Mi355(N_Vector v, const sunrealtype* src);
+SUNErrCode N_VCopyToHostArray_Mi355(N_Vector v, sunrealtype* dst);
+
+/* the HIP stream (hipStream_t) all ops of this vector and its clones run on */
+void* N_VGetStream_Mi355(N_Vector v);
+/* wait for all work queued on the vector's stream */
+SUNErrCode N_VSynchronize_Mi355(N_Vector v);
+/* enable (default) or disable the fused and vector-array ops */
+SUNErrCode N_VEnableFusedOps_Mi355(N_Vector v, sunbooleantype tf);
+
+/* number of HIP kernel launches issued by this vector family (for profiling) */
+long int N_VGetNumLaunches_Mi355(N_Vector v);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

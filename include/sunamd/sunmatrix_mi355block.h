@@ -1,0 +1,57 @@
+/* -----------------------------------------------------------------------------
+ * Block-diagonal SUNMatrix in MI355X HBM and its batched LU linear solver.
+ *
+ * Matrix: nblocks dense m x m blocks, block-contiguous, column-major inside a
+ * block (block k, column j, row i at k*m*m + j*m + i) -- the layout of the
+ * reference's batched precedent SUNMatrix_MagmaDenseBlock
+ * (include/sunmatrix/sunmatrix_magmadense.h:41-69,
+ *  src/sunmatrix/magmadense/sunmatrix_magmadense.cpp:84-110).
+ * Ops fill the reference SUNMatrix table (include/sundials/sundials_matrix.h:
+ * 89-105) with the dense matrix's per-element arithmetic
+ * (src/sunmatrix/dense/sunmatrix_dense.c:229-334).  The vector of a block
+ * system is the concatenation of the per-block vectors.
+ *
+ * Solver (SUNLINEARSOLVER_DIRECT): replaces SUNLinSol_MagmaDense
+ * (src/sunlinsol/magmadense/sunlinsol_magmadense.cpp:314-440, which calls
+ * magma_dgetrf/getrs_batched) with hand-written HIP batched LU.  In FP64 mode
+ * the factors, pivots and solutions are bit-for-bit those of the reference
+ * dense GETRF / GETRS (src/sundials/sundials_dense.c:85-186) applied block by
+ * block.  In FP32 mode the blocks are factored in single precision and the
+ * solve runs in single precision on FP64 right-hand sides (the Newton
+ * iteration around it stays FP64).  A zero pivot in any block makes setup
+ * return SUNLS_LUFACT_FAIL, with SUNLinSolLastFlag = (global column index of
+ * the first zero pivot) + 1, like the dense solver.
+ * ---------------------------------------------------------------------------*/
+#ifndef SUNAMD_SUNMATRIX_MI355BLOCK_H
+#define SUNAMD_SUNMATRIX_MI355BLOCK_H
+
+#include "sundials_core.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SUNAMD_LU_FP64 0
+#define SUNAMD_LU_FP32 1
+
+/* stream: the hipStream_t the owning vectors run on (N_VGetStream_Mi355) */
+SUNMatrix SUNMatrix_Mi355BlockDense(sunindextype nblocks, sunindextype m, void* stream,
+                                    SUNContext sunctx);
+sunindextype SUNMatrix_Mi355BlockDense_NumBlocks(SUNMatrix A);
+sunindextype SUNMatrix_Mi355BlockDense_BlockSize(SUNMatrix A);
+sunrealtype* SUNMatrix_Mi355BlockDense_DeviceData(SUNMatrix A);
+/* host mirror (allocated on first use) and synchronous copies */
+sunrealtype* SUNMatrix_Mi355BlockDense_HostData(SUNMatrix A);
+SUNErrCode SUNMatrix_Mi355BlockDense_CopyToDevice(SUNMatrix A);
+SUNErrCode SUNMatrix_Mi355BlockDense_CopyFromDevice(SUNMatrix A);
+
+SUNLinearSolver SUNLinSol_Mi355BlockLU(N_Vector y, SUNMatrix A, int precision,
+                                       SUNContext sunctx);
+/* device pivot array (int32, nblocks*m, pivot row of step k of block b at b*m+k) */
+int* SUNLinSol_Mi355BlockLU_DevicePivots(SUNLinearSolver S);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

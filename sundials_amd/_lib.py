@@ -1,0 +1,209 @@
+"""ctypes bindings for the in-tree C-ABI libraries.
+
+libsunamd_core.so   generic layer, Newton, BDF integrator, problem generator
+libsundials_amd.so  HIP N_Vector, block-diagonal matrix + batched LU, kernels
+
+The HIP library is the product path: `hip()` raises if it has not been built,
+there is no fallback implementation.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(_HERE, "lib")
+
+c_real = C.c_double
+c_index = C.c_int64
+c_void_p = C.c_void_p
+
+_core = None
+_hip = None
+
+
+class MassAction(C.Structure):
+    """Mirror of SUNAMD_MassAction (include/sunamd/massaction.h)."""
+    _fields_ = [
+        ("m", C.c_int),
+        ("R", C.c_int),
+        ("ncells", C.c_int64),
+        ("seed", C.c_uint64),
+        ("sigma", C.c_double),
+        ("idx", C.POINTER(C.c_int)),
+        ("kf0", C.POINTER(C.c_double)),
+        ("kb0", C.POINTER(C.c_double)),
+        ("kf", C.POINTER(C.c_double)),
+        ("kb", C.POINTER(C.c_double)),
+        ("y0", C.POINTER(C.c_double)),
+    ]
+
+
+class SolveStats(C.Structure):
+    """Mirror of SUNAMD_SolveStats (include/sunamd/massaction_mi355.h)."""
+    _fields_ = [
+        ("nst", C.c_long), ("nfe", C.c_long), ("nsetups", C.c_long), ("netf", C.c_long),
+        ("nni", C.c_long), ("nnf", C.c_long), ("ncfn", C.c_long), ("nje", C.c_long),
+        ("qlast", C.c_int), ("qcur", C.c_int),
+        ("hinused", C.c_double), ("hlast", C.c_double), ("hcur", C.c_double),
+        ("tcur", C.c_double), ("wall_s", C.c_double), ("launches", C.c_long),
+        ("flag", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _sig(lib, name, restype, argtypes):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = argtypes
+    return f
+
+
+def core():
+    """Load libsunamd_core.so (plain C)."""
+    global _core
+    if _core is not None:
+        return _core
+    path = os.path.join(LIBDIR, "libsunamd_core.so")
+    if not os.path.exists(path):
+        raise RuntimeError("libsunamd_core.so not built: run __graft_entry__.build()")
+    lib = C.CDLL(path, mode=C.RTLD_GLOBAL)
+    P = C.POINTER
+    _sig(lib, "SUNContext_Create", C.c_int, [C.c_int, P(c_void_p)])
+    _sig(lib, "SUNContext_Free", C.c_int, [P(c_void_p)])
+    _sig(lib, "SUNContext_GetLastError", C.c_int, [c_void_p])
+    for n in ("N_VDestroy",):
+        _sig(lib, n, None, [c_void_p])
+    _sig(lib, "N_VClone", c_void_p, [c_void_p])
+    _sig(lib, "N_VGetLength", c_index, [c_void_p])
+    _sig(lib, "N_VGetArrayPointer", P(c_real), [c_void_p])
+    _sig(lib, "N_VGetDeviceArrayPointer", c_void_p, [c_void_p])
+    _sig(lib, "N_VLinearSum", None, [c_real, c_void_p, c_real, c_void_p, c_void_p])
+    _sig(lib, "N_VConst", None, [c_real, c_void_p])
+    for n in ("N_VProd", "N_VDiv"):
+        _sig(lib, n, None, [c_void_p, c_void_p, c_void_p])
+    _sig(lib, "N_VScale", None, [c_real, c_void_p, c_void_p])
+    for n in ("N_VAbs", "N_VInv"):
+        _sig(lib, n, None, [c_void_p, c_void_p])
+    _sig(lib, "N_VAddConst", None, [c_void_p, c_real, c_void_p])
+    _sig(lib, "N_VDotProd", c_real, [c_void_p, c_void_p])
+    _sig(lib, "N_VMaxNorm", c_real, [c_void_p])
+    _sig(lib, "N_VWrmsNorm", c_real, [c_void_p, c_void_p])
+    _sig(lib, "N_VWrmsNormMask", c_real, [c_void_p, c_void_p, c_void_p])
+    _sig(lib, "N_VMin", c_real, [c_void_p])
+    _sig(lib, "N_VWL2Norm", c_real, [c_void_p, c_void_p])
+    _sig(lib, "N_VL1Norm", c_real, [c_void_p])
+    _sig(lib, "N_VCompare", None, [c_real, c_void_p, c_void_p])
+    _sig(lib, "N_VInvTest", C.c_int, [c_void_p, c_void_p])
+    _sig(lib, "N_VConstrMask", C.c_int, [c_void_p, c_void_p, c_void_p])
+    _sig(lib, "N_VMinQuotient", c_real, [c_void_p, c_void_p])
+    _sig(lib, "N_VLinearCombination", C.c_int, [C.c_int, P(c_real), P(c_void_p), c_void_p])
+    _sig(lib, "N_VScaleAddMulti", C.c_int,
+         [C.c_int, P(c_real), c_void_p, P(c_void_p), P(c_void_p)])
+    _sig(lib, "N_VDotProdMulti", C.c_int, [C.c_int, c_void_p, P(c_void_p), P(c_real)])
+    _sig(lib, "N_VScaleVectorArray", C.c_int,
+         [C.c_int, P(c_real), P(c_void_p), P(c_void_p)])
+    _sig(lib, "N_VWrmsNormVectorArray", C.c_int,
+         [C.c_int, P(c_void_p), P(c_void_p), P(c_real)])
+    _sig(lib, "SUNMatDestroy", None, [c_void_p])
+    _sig(lib, "SUNMatZero", C.c_int, [c_void_p])
+    _sig(lib, "SUNMatCopy", C.c_int, [c_void_p, c_void_p])
+    _sig(lib, "SUNMatScaleAddI", C.c_int, [c_real, c_void_p])
+    _sig(lib, "SUNMatScaleAdd", C.c_int, [c_real, c_void_p, c_void_p])
+    _sig(lib, "SUNMatMatvec", C.c_int, [c_void_p, c_void_p, c_void_p])
+    _sig(lib, "SUNLinSolSetup", C.c_int, [c_void_p, c_void_p])
+    _sig(lib, "SUNLinSolSolve", C.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_real])
+    _sig(lib, "SUNLinSolLastFlag", c_index, [c_void_p])
+    _sig(lib, "SUNLinSolFree", C.c_int, [c_void_p])
+    _sig(lib, "SUNLinSolInitialize", C.c_int, [c_void_p])
+    _sig(lib, "SUNAMD_Uniform", c_real, [C.c_uint64, C.c_uint64, C.c_uint64])
+    _sig(lib, "SUNAMD_MassActionCreate", P(MassAction),
+         [C.c_int, C.c_int, C.c_int64, C.c_uint64, C.c_double])
+    _sig(lib, "SUNAMD_MassActionCreateRange", P(MassAction),
+         [C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_uint64, C.c_double])
+    _sig(lib, "SUNAMD_MassActionTile", P(MassAction), [P(MassAction), C.c_int64])
+    _sig(lib, "SUNAMD_MassActionFree", None, [P(MassAction)])
+    _sig(lib, "SUNAMD_MassActionRhsHost", None,
+         [P(MassAction), P(c_real), P(c_real), C.c_int64, C.c_int64])
+    _sig(lib, "SUNAMD_MassActionJacHost", None,
+         [P(MassAction), P(c_real), P(c_real), C.c_int64, C.c_int64])
+    _core = lib
+    return lib
+
+
+def hip():
+    """Load libsundials_amd.so (the HIP product path); raises if missing."""
+    global _hip
+    if _hip is not None:
+        return _hip
+    core()
+    path = os.path.join(LIBDIR, "libsundials_amd.so")
+    if not os.path.exists(path):
+        raise RuntimeError("libsundials_amd.so not built: run __graft_entry__.build()")
+    lib = C.CDLL(path, mode=C.RTLD_GLOBAL)
+    P = C.POINTER
+    _sig(lib, "N_VNew_Mi355", c_void_p, [c_index, c_void_p])
+    _sig(lib, "N_VNewEmpty_Mi355", c_void_p, [c_index, c_void_p])
+    _sig(lib, "N_VMake_Mi355", c_void_p, [c_index, c_void_p, c_void_p, c_void_p])
+    for n in ("N_VCopyToDevice_Mi355", "N_VCopyFromDevice_Mi355", "N_VSynchronize_Mi355"):
+        _sig(lib, n, C.c_int, [c_void_p])
+    _sig(lib, "N_VCopyFromHostArray_Mi355", C.c_int, [c_void_p, P(c_real)])
+    _sig(lib, "N_VCopyToHostArray_Mi355", C.c_int, [c_void_p, P(c_real)])
+    _sig(lib, "N_VGetStream_Mi355", c_void_p, [c_void_p])
+    _sig(lib, "N_VEnableFusedOps_Mi355", C.c_int, [c_void_p, C.c_int])
+    _sig(lib, "N_VGetNumLaunches_Mi355", C.c_long, [c_void_p])
+    _sig(lib, "SUNMatrix_Mi355BlockDense", c_void_p, [c_index, c_index, c_void_p, c_void_p])
+    _sig(lib, "SUNMatrix_Mi355BlockDense_DeviceData", c_void_p, [c_void_p])
+    _sig(lib, "SUNMatrix_Mi355BlockDense_HostData", P(c_real), [c_void_p])
+    _sig(lib, "SUNMatrix_Mi355BlockDense_CopyToDevice", C.c_int, [c_void_p])
+    _sig(lib, "SUNMatrix_Mi355BlockDense_CopyFromDevice", C.c_int, [c_void_p])
+    _sig(lib, "SUNLinSol_Mi355BlockLU", c_void_p, [c_void_p, c_void_p, C.c_int, c_void_p])
+    _sig(lib, "SUNLinSol_Mi355BlockLU_DevicePivots", c_void_p, [c_void_p])
+    _sig(lib, "SUNAMD_MassActionToDevice", c_void_p, [P(MassAction), c_void_p])
+    _sig(lib, "SUNAMD_MassActionDevFree", None, [c_void_p])
+    _sig(lib, "SUNAMD_MassActionRhsDevArrays", C.c_int, [c_void_p, c_void_p, c_void_p])
+    _sig(lib, "SUNAMD_MassActionJacDevArrays", C.c_int, [c_void_p, c_void_p, c_void_p])
+    _sig(lib, "SUNAMD_MassActionSolveCoupled_Mi355", C.c_int,
+         [P(MassAction), c_real, c_real, c_real, C.c_int, P(c_real), P(SolveStats)])
+    _hip = lib
+    return lib
+
+
+def load_oracle():
+    """Load the CPU oracle (test infrastructure only)."""
+    core()
+    root = os.path.dirname(_HERE)
+    path = os.path.join(root, "oracle", "_build", "liboracle.so")
+    if not os.path.exists(path):
+        raise RuntimeError("oracle not built: run __graft_entry__.build()")
+    lib = C.CDLL(path, mode=C.RTLD_GLOBAL)
+    P = C.POINTER
+    _sig(lib, "N_VNew_Serial", c_void_p, [c_index, c_void_p])
+    _sig(lib, "N_VMake_Serial", c_void_p, [c_index, P(c_real), c_void_p])
+    _sig(lib, "N_VEnableFusedOps_Serial", C.c_int, [c_void_p, C.c_int])
+    _sig(lib, "SUNDenseMatrix", c_void_p, [c_index, c_index, c_void_p])
+    _sig(lib, "SUNDenseMatrix_Data", P(c_real), [c_void_p])
+    _sig(lib, "SUNLinSol_Dense", c_void_p, [c_void_p, c_void_p, c_void_p])
+    _sig(lib, "SUNBlockDenseMatrix_Oracle", c_void_p, [c_index, c_index, c_void_p])
+    _sig(lib, "SUNBlockDenseMatrix_Oracle_Data", P(c_real), [c_void_p])
+    _sig(lib, "SUNLinSol_BlockDense_Oracle", c_void_p, [c_void_p, c_void_p, c_void_p])
+    _sig(lib, "oracle_block_getrf", c_index, [P(c_real), c_index, P(c_index)])
+    _sig(lib, "oracle_block_getrs", None, [P(c_real), c_index, P(c_index), P(c_real)])
+    _sig(lib, "oracle_roberts_run", C.c_int, [C.c_int, P(c_real)])
+    _sig(lib, "oracle_massaction_solve_coupled", C.c_int,
+         [P(MassAction), c_real, c_real, c_real, P(c_real), P(c_real)])
+    _sig(lib, "oracle_massaction_solve_per_cell", C.c_int,
+         [P(MassAction), c_real, c_real, c_real, C.c_int, P(c_real), P(c_real), P(c_real)])
+    return lib
+
+
+def new_context():
+    ctx = c_void_p()
+    rc = core().SUNContext_Create(0, C.byref(ctx))
+    if rc != 0:
+        raise RuntimeError("SUNContext_Create failed: %d" % rc)
+    return ctx
+
+
+def free_context(ctx):
+    core().SUNContext_Free(C.byref(ctx))

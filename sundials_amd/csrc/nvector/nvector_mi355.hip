@@ -1,0 +1,1293 @@
+// -----------------------------------------------------------------------------
+// N_Vector for MI355X (gfx950): every op is a hand-written HIP kernel.
+//
+// Ops-table semantics follow the reference (include/sundials/sundials_nvector.h
+// :101-195; per-op formulas from src/nvector/serial/nvector_serial.c:387-1592).
+// Design, replacing the reference HIP vector (src/nvector/hip/nvector_hip.hip.cpp,
+// VectorKernels.hip.hpp, VectorArrayKernels.hip.hpp):
+//  * streaming ops move 128-bit (2 x FP64) per lane per access, non-temporal
+//    for vectors larger than the last-level cache, grid sized to the device;
+//  * LinearSum keeps the serial vector's rounding: its nine special cases
+//    collapse to three per-element formulas, a*x+b*y, a*(x+y), a*(x-y)
+//    (nvector_serial.c:387-480 / :1600-1775), with no fused multiply-add;
+//  * reductions: per-lane sequential accumulation, 64-wide wavefront
+//    shuffle tree, LDS across wavefronts, then the last workgroup to finish
+//    (ticket counter) folds the per-workgroup partials in a fixed order and
+//    writes the scalar(s) straight into pinned host memory -- one kernel and
+//    one stream synchronisation per reduction, deterministic results;
+//  * DotProdMulti reads x once for all nvec dot products (the serial vector
+//    re-reads x per product, nvector_serial.c:1019-1024) and spreads the work
+//    over the whole device (the reference uses one workgroup per product);
+//  * LinearCombination / ScaleAddMulti write each output once; vector
+//    pointers and coefficients travel as kernel arguments (no per-call
+//    hipMalloc / hipMemcpy of pointer arrays).
+// -----------------------------------------------------------------------------
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "sunamd/nvector_mi355.h"
+
+#define NV_MAX_MULTI 32      // vectors per fused call handled in one kernel
+#define NV_BLOCK 256         // threads per workgroup
+#define NV_MAX_RED_BLOCKS 2048
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+// ----------------------------------------------------------------------------
+// execution context shared by a vector and all of its clones
+// ----------------------------------------------------------------------------
+struct Mi355Exec
+{
+  int refcount;
+  int device;
+  int num_cus;
+  hipStream_t stream;
+  double* d_partials;     // NV_MAX_RED_BLOCKS * (NV_MAX_MULTI + 1)
+  unsigned int* d_ticket; // last-workgroup ticket (kept at zero between calls)
+  double* h_result;       // pinned, mapped: NV_MAX_MULTI + 1 scalars
+  long int launches;
+};
+
+struct Mi355Content
+{
+  sunindextype length;
+  sunbooleantype own_host;
+  sunbooleantype own_dev;
+  sunrealtype* host;
+  sunrealtype* dev;
+  Mi355Exec* exec;
+};
+
+#define CONTENT(v) ((Mi355Content*)((v)->content))
+#define NVLEN(v)   (CONTENT(v)->length)
+#define DDATA(v)   (CONTENT(v)->dev)
+#define EXEC(v)    (CONTENT(v)->exec)
+
+static void hip_fail(SUNContext ctx, hipError_t e, const char* what, int line)
+{
+  fprintf(stderr, "[sundials_amd nvector_mi355] %s failed (line %d): %s\n", what, line,
+          hipGetErrorString(e));
+  fflush(stderr);
+  SUNAMD_SetLastError(ctx, SUN_ERR_EXT_FAIL);
+}
+
+#define HIPCHK(ctx, call)                                             \
+  do {                                                                \
+    hipError_t e_ = (call);                                           \
+    if (e_ != hipSuccess) { hip_fail((ctx), e_, #call, __LINE__); }   \
+  } while (0)
+
+#define LAUNCHCHK(v)                                                  \
+  do {                                                                \
+    EXEC(v)->launches++;                                              \
+    hipError_t e_ = hipGetLastError();                                \
+    if (e_ != hipSuccess) { hip_fail((v)->sunctx, e_, "kernel launch", __LINE__); } \
+  } while (0)
+
+static Mi355Exec* exec_create(SUNContext ctx)
+{
+  Mi355Exec* ex = (Mi355Exec*)calloc(1, sizeof(Mi355Exec));
+  if (!ex) { return NULL; }
+  ex->refcount = 1;
+  HIPCHK(ctx, hipGetDevice(&ex->device));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ex->device) == hipSuccess)
+  {
+    ex->num_cus = prop.multiProcessorCount;
+  }
+  if (ex->num_cus <= 0) { ex->num_cus = 256; }
+  hipError_t e = hipStreamCreateWithFlags(&ex->stream, hipStreamNonBlocking);
+  if (e != hipSuccess)
+  {
+    hip_fail(ctx, e, "hipStreamCreate", __LINE__);
+    free(ex);
+    return NULL;
+  }
+  e = hipMalloc((void**)&ex->d_partials,
+                sizeof(double) * NV_MAX_RED_BLOCKS * (NV_MAX_MULTI + 1));
+  if (e == hipSuccess) { e = hipMalloc((void**)&ex->d_ticket, sizeof(unsigned int)); }
+  if (e == hipSuccess) { e = hipMemset(ex->d_ticket, 0, sizeof(unsigned int)); }
+  if (e == hipSuccess)
+  {
+    e = hipHostMalloc((void**)&ex->h_result, sizeof(double) * (NV_MAX_MULTI + 1),
+                      hipHostMallocMapped);
+  }
+  if (e != hipSuccess)
+  {
+    hip_fail(ctx, e, "exec allocation", __LINE__);
+    if (ex->d_partials) { hipFree(ex->d_partials); }
+    if (ex->d_ticket) { hipFree(ex->d_ticket); }
+    hipStreamDestroy(ex->stream);
+    free(ex);
+    return NULL;
+  }
+  return ex;
+}
+
+static void exec_release(Mi355Exec* ex)
+{
+  if (!ex) { return; }
+  if (--ex->refcount > 0) { return; }
+  hipStreamSynchronize(ex->stream);
+  hipFree(ex->d_partials);
+  hipFree(ex->d_ticket);
+  hipHostFree(ex->h_result);
+  hipStreamDestroy(ex->stream);
+  free(ex);
+}
+
+// ----------------------------------------------------------------------------
+// launch geometry
+// ----------------------------------------------------------------------------
+// streaming: one 128-bit element pair per lane per iteration, grid capped at
+// 16 workgroups per CU and grid-strided beyond that
+static inline int stream_grid(const Mi355Exec* ex, sunindextype npairs)
+{
+  sunindextype g   = (npairs + NV_BLOCK - 1) / NV_BLOCK;
+  sunindextype cap = (sunindextype)ex->num_cus * 16;
+  if (g > cap) { g = cap; }
+  if (g < 1) { g = 1; }
+  return (int)g;
+}
+
+static inline int red_grid(const Mi355Exec* ex, sunindextype npairs)
+{
+  sunindextype g   = (npairs + 4 * NV_BLOCK - 1) / (4 * NV_BLOCK);
+  sunindextype cap = (sunindextype)ex->num_cus * 8;
+  if (cap > NV_MAX_RED_BLOCKS) { cap = NV_MAX_RED_BLOCKS; }
+  if (g > cap) { g = cap; }
+  if (g < 1) { g = 1; }
+  return (int)g;
+}
+
+// ----------------------------------------------------------------------------
+// device helpers
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ dbl2 ld2(const double* __restrict__ p, long i)
+{
+  return __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(p) + i);
+}
+__device__ __forceinline__ void st2(double* __restrict__ p, long i, dbl2 v)
+{
+  __builtin_nontemporal_store(v, reinterpret_cast<dbl2*>(p) + i);
+}
+
+enum RedOp
+{
+  RSUM = 0,
+  RMAX = 1,
+  RMIN = 2
+};
+
+template <int OP>
+__device__ __forceinline__ double rcomb(double a, double b)
+{
+  if (OP == RSUM) { return a + b; }
+  if (OP == RMAX) { return (b > a) ? b : a; }
+  return (b < a) ? b : a;
+}
+
+template <int OP>
+__device__ __forceinline__ double wave_reduce(double v)
+{
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) { v = rcomb<OP>(v, __shfl_xor(v, off, 64)); }
+  return v;
+}
+
+// reduce NV per-lane values over the workgroup; result valid in thread 0
+template <int OP, int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double* lds /* NV*4 */)
+{
+  const int lane = threadIdx.x & 63;
+  const int wid  = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+  {
+    double r = wave_reduce<OP>(v[k]);
+    if (lane == 0) { lds[k * (NV_BLOCK / 64) + wid] = r; }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+    {
+      double r = lds[k * (NV_BLOCK / 64)];
+      for (int w = 1; w < NV_BLOCK / 64; w++) { r = rcomb<OP>(r, lds[k * (NV_BLOCK / 64) + w]); }
+      v[k] = r;
+    }
+  }
+}
+
+// Each workgroup publishes NV partials; the last workgroup (ticket) folds
+// them in workgroup order and writes the results to `out` (pinned host).
+template <int OP, int NV>
+__device__ __forceinline__ void grid_finish(double (&v)[NV], double* partials,
+                                            unsigned int* ticket, double* out,
+                                            double identity)
+{
+  __shared__ double lds[NV * (NV_BLOCK / 64)];
+  __shared__ bool last;
+  block_reduce<OP, NV>(v, lds);
+  if (threadIdx.x == 0)
+  {
+#pragma unroll
+    for (int k = 0; k < NV; k++) { partials[k * gridDim.x + blockIdx.x] = v[k]; }
+    __threadfence();
+    unsigned int t = atomicAdd(ticket, 1u);
+    last           = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) { return; }
+  __threadfence();
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+  {
+    acc[k] = identity;
+    for (unsigned int b = threadIdx.x; b < gridDim.x; b += NV_BLOCK)
+    {
+      acc[k] = rcomb<OP>(acc[k], __builtin_nontemporal_load(&partials[k * gridDim.x + b]));
+    }
+  }
+  __syncthreads();
+  block_reduce<OP, NV>(acc, lds);
+  if (threadIdx.x == 0)
+  {
+#pragma unroll
+    for (int k = 0; k < NV; k++) { out[k] = acc[k]; }
+    *ticket = 0u;
+    __threadfence_system();
+  }
+}
+
+// ----------------------------------------------------------------------------
+// streaming kernels: pairs [0, n/2) vectorised, odd tail element by thread 0
+// ----------------------------------------------------------------------------
+#define GRID_LOOP(np) \
+  for (long i = (long)blockIdx.x * NV_BLOCK + threadIdx.x; i < (np); i += (long)gridDim.x * NV_BLOCK)
+#define TAIL(n) if (((n) & 1) && blockIdx.x == 0 && threadIdx.x == 0)
+
+enum LSMode
+{
+  LS_GENERIC  = 0, // a*x + b*y
+  LS_SCALESUM = 1, // a*(x+y)
+  LS_SCALEDIF = 2  // a*(x-y)
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(NV_BLOCK) k_linearsum(double a, const double* __restrict__ x,
+                                                        double b, const double* __restrict__ y,
+                                                        double* __restrict__ z, long n)
+{
+  const long np = n >> 1;
+  GRID_LOOP(np)
+  {
+    dbl2 xv = ld2(x, i), yv = ld2(y, i), r;
+    if (MODE == LS_GENERIC) { r = (a * xv) + (b * yv); }
+    else if (MODE == LS_SCALESUM) { r = a * (xv + yv); }
+    else { r = a * (xv - yv); }
+    st2(z, i, r);
+  }
+  TAIL(n)
+  {
+    double xv = x[n - 1], yv = y[n - 1];
+    if (MODE == LS_GENERIC) { z[n - 1] = (a * xv) + (b * yv); }
+    else if (MODE == LS_SCALESUM) { z[n - 1] = a * (xv + yv); }
+    else { z[n - 1] = a * (xv - yv); }
+  }
+}
+
+__global__ void __launch_bounds__(NV_BLOCK) k_const(double c, double* __restrict__ z, long n)
+{
+  const long np = n >> 1;
+  dbl2 cv       = {c, c};
+  GRID_LOOP(np) { st2(z, i, cv); }
+  TAIL(n) { z[n - 1] = c; }
+}
+
+// unary / binary elementwise family
+enum EwOp
+{
+  EW_PROD,
+  EW_DIV,
+  EW_SCALE,
+  EW_ABS,
+  EW_INV,
+  EW_ADDCONST,
+  EW_COMPARE
+};
+
+template <int OP>
+__device__ __forceinline__ double ew(double x, double y, double c)
+{
+  if (OP == EW_PROD) { return x * y; }
+  if (OP == EW_DIV) { return x / y; }
+  if (OP == EW_SCALE) { return c * x; }
+  if (OP == EW_ABS) { return fabs(x); }
+  if (OP == EW_INV) { return 1.0 / x; }
+  if (OP == EW_ADDCONST) { return x + c; }
+  return (fabs(x) >= c) ? 1.0 : 0.0;
+}
+
+template <int OP>
+__global__ void __launch_bounds__(NV_BLOCK) k_ew(const double* __restrict__ x,
+                                                 const double* __restrict__ y, double c,
+                                                 double* __restrict__ z, long n)
+{
+  const bool binary = (OP == EW_PROD || OP == EW_DIV);
+  const long np     = n >> 1;
+  GRID_LOOP(np)
+  {
+    dbl2 xv = ld2(x, i);
+    dbl2 yv = binary ? ld2(y, i) : xv;
+    dbl2 r;
+    r.x = ew<OP>(xv.x, yv.x, c);
+    r.y = ew<OP>(xv.y, yv.y, c);
+    st2(z, i, r);
+  }
+  TAIL(n) { z[n - 1] = ew<OP>(x[n - 1], binary ? y[n - 1] : 0.0, c); }
+}
+
+// ----------------------------------------------------------------------------
+// reduction kernels
+// ----------------------------------------------------------------------------
+enum RedKind
+{
+  RK_DOT,      // sum x*y
+  RK_WSQR,     // sum (x*w)^2
+  RK_WSQRMASK, // sum (x*w)^2 where id > 0
+  RK_MAXABS,   // max |x|
+  RK_MIN,      // min x
+  RK_L1        // sum |x|
+};
+
+template <int K>
+struct RedTraits;
+template <>
+struct RedTraits<RK_DOT>
+{
+  static const int op = RSUM;
+};
+template <>
+struct RedTraits<RK_WSQR>
+{
+  static const int op = RSUM;
+};
+template <>
+struct RedTraits<RK_WSQRMASK>
+{
+  static const int op = RSUM;
+};
+template <>
+struct RedTraits<RK_MAXABS>
+{
+  static const int op = RMAX;
+};
+template <>
+struct RedTraits<RK_MIN>
+{
+  static const int op = RMIN;
+};
+template <>
+struct RedTraits<RK_L1>
+{
+  static const int op = RSUM;
+};
+
+template <int K>
+__device__ __forceinline__ double red_elem(double x, double y, double id)
+{
+  if (K == RK_DOT) { return x * y; }
+  if (K == RK_WSQR)
+  {
+    double p = x * y;
+    return p * p;
+  }
+  if (K == RK_WSQRMASK)
+  {
+    double p = x * y;
+    return (id > 0.0) ? p * p : 0.0;
+  }
+  if (K == RK_MAXABS) { return fabs(x); }
+  if (K == RK_MIN) { return x; }
+  return fabs(x);
+}
+
+template <int K>
+__global__ void __launch_bounds__(NV_BLOCK) k_reduce(const double* __restrict__ x,
+                                                     const double* __restrict__ y,
+                                                     const double* __restrict__ id, long n,
+                                                     double identity, double* partials,
+                                                     unsigned int* ticket, double* out)
+{
+  const int OP     = RedTraits<K>::op;
+  const bool two   = (K == RK_DOT || K == RK_WSQR || K == RK_WSQRMASK);
+  const bool three = (K == RK_WSQRMASK);
+  double acc[1]    = {identity};
+  const long np    = n >> 1;
+  GRID_LOOP(np)
+  {
+    dbl2 xv = ld2(x, i);
+    dbl2 yv = two ? ld2(y, i) : xv;
+    dbl2 iv = three ? ld2(id, i) : xv;
+    acc[0]  = rcomb<OP>(acc[0], red_elem<K>(xv.x, yv.x, iv.x));
+    acc[0]  = rcomb<OP>(acc[0], red_elem<K>(xv.y, yv.y, iv.y));
+  }
+  TAIL(n)
+  {
+    acc[0] = rcomb<OP>(acc[0], red_elem<K>(x[n - 1], two ? y[n - 1] : 0.0,
+                                          three ? id[n - 1] : 0.0));
+  }
+  grid_finish<OP, 1>(acc, partials, ticket, out, identity);
+}
+
+// fused d_k = x . Y_k, k < NV (x read once)
+struct PtrPack
+{
+  const double* p[NV_MAX_MULTI];
+};
+struct WPtrPack
+{
+  double* p[NV_MAX_MULTI];
+};
+struct CoefPack
+{
+  double c[NV_MAX_MULTI];
+};
+
+template <int NV>
+__global__ void __launch_bounds__(NV_BLOCK) k_dotmulti(const double* __restrict__ x, PtrPack Y,
+                                                       long n, double* partials,
+                                                       unsigned int* ticket, double* out)
+{
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) { acc[k] = 0.0; }
+  const long np = n >> 1;
+  GRID_LOOP(np)
+  {
+    dbl2 xv = ld2(x, i);
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+    {
+      dbl2 yv = ld2(Y.p[k], i);
+      acc[k] += xv.x * yv.x;
+      acc[k] += xv.y * yv.y;
+    }
+  }
+  TAIL(n)
+  {
+#pragma unroll
+    for (int k = 0; k < NV; k++) { acc[k] += x[n - 1] * Y.p[k][n - 1]; }
+  }
+  grid_finish<RSUM, NV>(acc, partials, ticket, out, 0.0);
+}
+
+// z = c0*X0; z += c_k*X_k (k = 1..nvec-1): the rounding of the generic-layer
+// loop (sundials_nvector.c:549-572), one pass, z written once
+template <int NV>
+__global__ void __launch_bounds__(NV_BLOCK) k_lincomb(CoefPack c, PtrPack X, double* z, long n)
+{
+  const long np = n >> 1;
+  GRID_LOOP(np)
+  {
+    dbl2 acc = c.c[0] * ld2(X.p[0], i);
+#pragma unroll
+    for (int k = 1; k < NV; k++) { acc = acc + c.c[k] * ld2(X.p[k], i); }
+    st2(z, i, acc);
+  }
+  TAIL(n)
+  {
+    double acc = c.c[0] * X.p[0][n - 1];
+#pragma unroll
+    for (int k = 1; k < NV; k++) { acc = acc + c.c[k] * X.p[k][n - 1]; }
+    z[n - 1] = acc;
+  }
+}
+
+// Z_k = a_k*x + Y_k (k < NV); x read once
+template <int NV>
+__global__ void __launch_bounds__(NV_BLOCK) k_scaleaddmulti(CoefPack a, const double* __restrict__ x,
+                                                            PtrPack Y, WPtrPack Z, long n)
+{
+  const long np = n >> 1;
+  GRID_LOOP(np)
+  {
+    dbl2 xv = ld2(x, i);
+#pragma unroll
+    for (int k = 0; k < NV; k++) { st2(Z.p[k], i, (a.c[k] * xv) + ld2(Y.p[k], i)); }
+  }
+  TAIL(n)
+  {
+#pragma unroll
+    for (int k = 0; k < NV; k++) { Z.p[k][n - 1] = (a.c[k] * x[n - 1]) + Y.p[k][n - 1]; }
+  }
+}
+
+// Z_k = c_k * X_k (k < NV)
+template <int NV>
+__global__ void __launch_bounds__(NV_BLOCK) k_scalearray(CoefPack c, PtrPack X, WPtrPack Z, long n)
+{
+  const long np = n >> 1;
+  GRID_LOOP(np)
+  {
+#pragma unroll
+    for (int k = 0; k < NV; k++) { st2(Z.p[k], i, c.c[k] * ld2(X.p[k], i)); }
+  }
+  TAIL(n)
+  {
+#pragma unroll
+    for (int k = 0; k < NV; k++) { Z.p[k][n - 1] = c.c[k] * X.p[k][n - 1]; }
+  }
+}
+
+// flag/min reductions for InvTest, ConstrMask, MinQuotient
+__global__ void __launch_bounds__(NV_BLOCK) k_invtest(const double* __restrict__ x,
+                                                      double* __restrict__ z, long n,
+                                                      double* partials, unsigned int* ticket,
+                                                      double* out)
+{
+  double acc[1] = {0.0}; // max of "zero found" flags
+  for (long i = (long)blockIdx.x * NV_BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * NV_BLOCK)
+  {
+    double xv = x[i];
+    if (xv == 0.0) { acc[0] = 1.0; }
+    else { z[i] = 1.0 / xv; }
+  }
+  grid_finish<RMAX, 1>(acc, partials, ticket, out, 0.0);
+}
+
+__global__ void __launch_bounds__(NV_BLOCK) k_constrmask(const double* __restrict__ c,
+                                                         const double* __restrict__ x,
+                                                         double* __restrict__ m, long n,
+                                                         double* partials, unsigned int* ticket,
+                                                         double* out)
+{
+  double acc[1] = {0.0};
+  for (long i = (long)blockIdx.x * NV_BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * NV_BLOCK)
+  {
+    double ci = c[i], xi = x[i];
+    double mi = 0.0;
+    if (ci != 0.0)
+    {
+      bool test = (fabs(ci) > 1.5 && xi * ci <= 0.0) || (fabs(ci) > 0.5 && xi * ci < 0.0);
+      if (test) { mi = acc[0] = 1.0; }
+    }
+    m[i] = mi;
+  }
+  grid_finish<RMAX, 1>(acc, partials, ticket, out, 0.0);
+}
+
+__global__ void __launch_bounds__(NV_BLOCK) k_minquotient(const double* __restrict__ num,
+                                                          const double* __restrict__ den, long n,
+                                                          double* partials, unsigned int* ticket,
+                                                          double* out)
+{
+  double acc[1] = {DBL_MAX};
+  for (long i = (long)blockIdx.x * NV_BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * NV_BLOCK)
+  {
+    double d = den[i];
+    if (d != 0.0)
+    {
+      double q = num[i] / d;
+      acc[0]   = (q < acc[0]) ? q : acc[0];
+    }
+  }
+  grid_finish<RMIN, 1>(acc, partials, ticket, out, DBL_MAX);
+}
+
+// ----------------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------------
+static N_Vector mi_clone_empty(N_Vector w);
+static N_Vector mi_clone(N_Vector w);
+static void mi_destroy(N_Vector v);
+
+static N_Vector_ID mi_getid(N_Vector v)
+{
+  (void)v;
+  return SUNDIALS_NVEC_CUSTOM;
+}
+static sunindextype mi_getlength(N_Vector v) { return NVLEN(v); }
+static void mi_space(N_Vector v, sunindextype* lrw, sunindextype* liw)
+{
+  *lrw = NVLEN(v);
+  *liw = 2;
+}
+
+static sunrealtype* mi_host_ptr(N_Vector v)
+{
+  Mi355Content* c = CONTENT(v);
+  if (c->host == NULL && c->length > 0)
+  {
+    c->host     = (sunrealtype*)calloc((size_t)c->length, sizeof(sunrealtype));
+    c->own_host = SUNTRUE;
+  }
+  return c->host;
+}
+static sunrealtype* mi_dev_ptr(N_Vector v) { return DDATA(v); }
+static void mi_set_ptr(sunrealtype* d, N_Vector v)
+{
+  Mi355Content* c = CONTENT(v);
+  if (c->own_host && c->host) { free(c->host); }
+  c->host     = d;
+  c->own_host = SUNFALSE;
+}
+
+static inline bool aligned16(const void* p) { return (((uintptr_t)p) & 15u) == 0; }
+
+static void mi_linearsum(sunrealtype a, N_Vector x, sunrealtype b, N_Vector y, N_Vector z)
+{
+  long n = (long)NVLEN(z);
+  if (n == 0) { return; }
+  Mi355Exec* ex = EXEC(z);
+  int g         = stream_grid(ex, n >> 1);
+  if (a == b)
+  {
+    k_linearsum<LS_SCALESUM><<<g, NV_BLOCK, 0, ex->stream>>>(a, DDATA(x), b, DDATA(y), DDATA(z), n);
+  }
+  else if (a == -b)
+  {
+    k_linearsum<LS_SCALEDIF><<<g, NV_BLOCK, 0, ex->stream>>>(a, DDATA(x), b, DDATA(y), DDATA(z), n);
+  }
+  else
+  {
+    k_linearsum<LS_GENERIC><<<g, NV_BLOCK, 0, ex->stream>>>(a, DDATA(x), b, DDATA(y), DDATA(z), n);
+  }
+  LAUNCHCHK(z);
+}
+
+static void mi_const(sunrealtype c, N_Vector z)
+{
+  long n = (long)NVLEN(z);
+  if (n == 0) { return; }
+  Mi355Exec* ex = EXEC(z);
+  k_const<<<stream_grid(ex, n >> 1), NV_BLOCK, 0, ex->stream>>>(c, DDATA(z), n);
+  LAUNCHCHK(z);
+}
+
+template <int OP>
+static void mi_ew(N_Vector x, N_Vector y, double c, N_Vector z)
+{
+  long n = (long)NVLEN(z);
+  if (n == 0) { return; }
+  Mi355Exec* ex = EXEC(z);
+  k_ew<OP><<<stream_grid(ex, n >> 1), NV_BLOCK, 0, ex->stream>>>(
+    DDATA(x), y ? DDATA(y) : DDATA(x), c, DDATA(z), n);
+  LAUNCHCHK(z);
+}
+
+static void mi_prod(N_Vector x, N_Vector y, N_Vector z) { mi_ew<EW_PROD>(x, y, 0.0, z); }
+static void mi_div(N_Vector x, N_Vector y, N_Vector z) { mi_ew<EW_DIV>(x, y, 0.0, z); }
+static void mi_scale(sunrealtype c, N_Vector x, N_Vector z)
+{
+  // c == 1 (copy) and c == -1 (negate) give the same bits as c*x
+  mi_ew<EW_SCALE>(x, NULL, c, z);
+}
+static void mi_abs(N_Vector x, N_Vector z) { mi_ew<EW_ABS>(x, NULL, 0.0, z); }
+static void mi_inv(N_Vector x, N_Vector z) { mi_ew<EW_INV>(x, NULL, 0.0, z); }
+static void mi_addconst(N_Vector x, sunrealtype b, N_Vector z)
+{
+  mi_ew<EW_ADDCONST>(x, NULL, b, z);
+}
+static void mi_compare(sunrealtype c, N_Vector x, N_Vector z)
+{
+  mi_ew<EW_COMPARE>(x, NULL, c, z);
+}
+
+// run one reduction kernel, wait for it, return the host-visible result
+template <int K>
+static double mi_reduce(N_Vector x, N_Vector y, N_Vector id, double identity)
+{
+  long n        = (long)NVLEN(x);
+  Mi355Exec* ex = EXEC(x);
+  if (n == 0) { return identity; }
+  int g = red_grid(ex, n >> 1);
+  k_reduce<K><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), y ? DDATA(y) : DDATA(x),
+                                              id ? DDATA(id) : DDATA(x), n, identity,
+                                              ex->d_partials, ex->d_ticket, ex->h_result);
+  LAUNCHCHK(x);
+  HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
+  return ex->h_result[0];
+}
+
+static sunrealtype mi_dotprod(N_Vector x, N_Vector y) { return mi_reduce<RK_DOT>(x, y, NULL, 0.0); }
+static sunrealtype mi_maxnorm(N_Vector x) { return mi_reduce<RK_MAXABS>(x, NULL, NULL, 0.0); }
+static sunrealtype mi_wsqrsum(N_Vector x, N_Vector w) { return mi_reduce<RK_WSQR>(x, w, NULL, 0.0); }
+static sunrealtype mi_wsqrsummask(N_Vector x, N_Vector w, N_Vector id)
+{
+  return mi_reduce<RK_WSQRMASK>(x, w, id, 0.0);
+}
+static inline double sqrt0(double v) { return (v <= 0.0) ? 0.0 : sqrt(v); }
+static sunrealtype mi_wrmsnorm(N_Vector x, N_Vector w)
+{
+  return sqrt0(mi_wsqrsum(x, w) / (double)NVLEN(x));
+}
+static sunrealtype mi_wrmsnormmask(N_Vector x, N_Vector w, N_Vector id)
+{
+  return sqrt0(mi_wsqrsummask(x, w, id) / (double)NVLEN(x));
+}
+static sunrealtype mi_min(N_Vector x) { return mi_reduce<RK_MIN>(x, NULL, NULL, DBL_MAX); }
+static sunrealtype mi_wl2norm(N_Vector x, N_Vector w) { return sqrt0(mi_wsqrsum(x, w)); }
+static sunrealtype mi_l1norm(N_Vector x) { return mi_reduce<RK_L1>(x, NULL, NULL, 0.0); }
+
+static sunbooleantype mi_invtest(N_Vector x, N_Vector z)
+{
+  long n        = (long)NVLEN(x);
+  Mi355Exec* ex = EXEC(x);
+  if (n == 0) { return SUNTRUE; }
+  k_invtest<<<red_grid(ex, n >> 1), NV_BLOCK, 0, ex->stream>>>(
+    DDATA(x), DDATA(z), n, ex->d_partials, ex->d_ticket, ex->h_result);
+  LAUNCHCHK(x);
+  HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
+  return (ex->h_result[0] > 0.0) ? SUNFALSE : SUNTRUE;
+}
+
+static sunbooleantype mi_constrmask(N_Vector c, N_Vector x, N_Vector m)
+{
+  long n        = (long)NVLEN(x);
+  Mi355Exec* ex = EXEC(x);
+  if (n == 0) { return SUNTRUE; }
+  k_constrmask<<<red_grid(ex, n >> 1), NV_BLOCK, 0, ex->stream>>>(
+    DDATA(c), DDATA(x), DDATA(m), n, ex->d_partials, ex->d_ticket, ex->h_result);
+  LAUNCHCHK(x);
+  HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
+  return (ex->h_result[0] > 0.0) ? SUNFALSE : SUNTRUE;
+}
+
+static sunrealtype mi_minquotient(N_Vector num, N_Vector den)
+{
+  long n        = (long)NVLEN(num);
+  Mi355Exec* ex = EXEC(num);
+  if (n == 0) { return SUN_BIG_REAL; }
+  k_minquotient<<<red_grid(ex, n >> 1), NV_BLOCK, 0, ex->stream>>>(
+    DDATA(num), DDATA(den), n, ex->d_partials, ex->d_ticket, ex->h_result);
+  LAUNCHCHK(num);
+  HIPCHK(num->sunctx, hipStreamSynchronize(ex->stream));
+  return ex->h_result[0];
+}
+
+// ---- fused ops --------------------------------------------------------------
+#define NV_CASES(MACRO) \
+  MACRO(1) MACRO(2) MACRO(3) MACRO(4) MACRO(5) MACRO(6) MACRO(7) MACRO(8)
+
+static SUNErrCode mi_dotprodmulti(int nvec, N_Vector x, N_Vector* Y, sunrealtype* d)
+{
+  if (nvec < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
+  long n        = (long)NVLEN(x);
+  Mi355Exec* ex = EXEC(x);
+  if (n == 0)
+  {
+    for (int k = 0; k < nvec; k++) { d[k] = 0.0; }
+    return SUN_SUCCESS;
+  }
+  // chunks of up to 8 vectors keep the per-lane accumulators in registers;
+  // results land in the pinned slots, read back every NV_MAX_MULTI products
+  int group = 0;
+  for (int base = 0; base < nvec; base += 8)
+  {
+    int nv = nvec - base < 8 ? nvec - base : 8;
+    PtrPack P;
+    for (int k = 0; k < nv; k++) { P.p[k] = DDATA(Y[base + k]); }
+    int g        = red_grid(ex, n >> 1);
+    double* slot = ex->h_result + (base - group);
+    switch (nv)
+    {
+#define DM_CASE(NVV)                                                                       \
+  case NVV:                                                                                \
+    k_dotmulti<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), P, n, ex->d_partials,        \
+                                                    ex->d_ticket, slot);                   \
+    break;
+      NV_CASES(DM_CASE)
+#undef DM_CASE
+    }
+    LAUNCHCHK(x);
+    int filled = base + nv - group;
+    if (filled == NV_MAX_MULTI || base + nv == nvec)
+    {
+      HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
+      for (int k = 0; k < filled; k++) { d[group + k] = ex->h_result[k]; }
+      group += filled;
+    }
+  }
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode mi_lincomb(int nvec, sunrealtype* c, N_Vector* X, N_Vector z)
+{
+  if (nvec < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
+  long n        = (long)NVLEN(z);
+  Mi355Exec* ex = EXEC(z);
+  if (n == 0) { return SUN_SUCCESS; }
+  int g = stream_grid(ex, n >> 1);
+  // first chunk writes z, later chunks accumulate into z (same rounding
+  // order as the sequential loop)
+  int done = 0;
+  while (done < nvec)
+  {
+    CoefPack C;
+    PtrPack P;
+    int nv;
+    if (done == 0)
+    {
+      nv = nvec < 8 ? nvec : 8;
+      for (int k = 0; k < nv; k++)
+      {
+        C.c[k] = c[k];
+        P.p[k] = DDATA(X[k]);
+      }
+    }
+    else
+    {
+      nv     = (nvec - done) + 1 < 8 ? (nvec - done) + 1 : 8;
+      C.c[0] = 1.0;
+      P.p[0] = DDATA(z);
+      for (int k = 1; k < nv; k++)
+      {
+        C.c[k] = c[done + k - 1];
+        P.p[k] = DDATA(X[done + k - 1]);
+      }
+    }
+    switch (nv)
+    {
+#define LC_CASE(NVV)                                                                    \
+  case NVV: k_lincomb<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(C, P, DDATA(z), n); break;
+      NV_CASES(LC_CASE)
+#undef LC_CASE
+    }
+    LAUNCHCHK(z);
+    done += (done == 0) ? nv : nv - 1;
+  }
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode mi_scaleaddmulti(int nvec, sunrealtype* a, N_Vector x, N_Vector* Y,
+                                   N_Vector* Z)
+{
+  if (nvec < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
+  long n        = (long)NVLEN(x);
+  Mi355Exec* ex = EXEC(x);
+  if (n == 0) { return SUN_SUCCESS; }
+  int g = stream_grid(ex, n >> 1);
+  for (int base = 0; base < nvec; base += 8)
+  {
+    int nv = nvec - base < 8 ? nvec - base : 8;
+    CoefPack C;
+    PtrPack P;
+    WPtrPack W;
+    for (int k = 0; k < nv; k++)
+    {
+      C.c[k] = a[base + k];
+      P.p[k] = DDATA(Y[base + k]);
+      W.p[k] = DDATA(Z[base + k]);
+    }
+    switch (nv)
+    {
+#define SAM_CASE(NVV)                                                                        \
+  case NVV: k_scaleaddmulti<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(C, DDATA(x), P, W, n); break;
+      NV_CASES(SAM_CASE)
+#undef SAM_CASE
+    }
+    LAUNCHCHK(x);
+  }
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode mi_scalevectorarray(int nvec, sunrealtype* c, N_Vector* X, N_Vector* Z)
+{
+  if (nvec < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
+  long n        = (long)NVLEN(Z[0]);
+  Mi355Exec* ex = EXEC(Z[0]);
+  if (n == 0) { return SUN_SUCCESS; }
+  int g = stream_grid(ex, n >> 1);
+  for (int base = 0; base < nvec; base += 8)
+  {
+    int nv = nvec - base < 8 ? nvec - base : 8;
+    CoefPack C;
+    PtrPack P;
+    WPtrPack W;
+    for (int k = 0; k < nv; k++)
+    {
+      C.c[k] = c[base + k];
+      P.p[k] = DDATA(X[base + k]);
+      W.p[k] = DDATA(Z[base + k]);
+    }
+    switch (nv)
+    {
+#define SVA_CASE(NVV)                                                                  \
+  case NVV: k_scalearray<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(C, P, W, n); break;
+      NV_CASES(SVA_CASE)
+#undef SVA_CASE
+    }
+    LAUNCHCHK(Z[0]);
+  }
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode mi_linearsumvectorarray(int nvec, sunrealtype a, N_Vector* X, sunrealtype b,
+                                          N_Vector* Y, N_Vector* Z)
+{
+  if (nvec < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
+  for (int i = 0; i < nvec; i++) { mi_linearsum(a, X[i], b, Y[i], Z[i]); }
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode mi_constvectorarray(int nvec, sunrealtype c, N_Vector* Z)
+{
+  if (nvec < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
+  for (int i = 0; i < nvec; i++) { mi_const(c, Z[i]); }
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode mi_wrmsnormvectorarray(int nvec, N_Vector* X, N_Vector* W, sunrealtype* nrm)
+{
+  if (nvec < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
+  for (int i = 0; i < nvec; i++) { nrm[i] = mi_wrmsnorm(X[i], W[i]); }
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode mi_wrmsnormmaskvectorarray(int nvec, N_Vector* X, N_Vector* W, N_Vector id,
+                                             sunrealtype* nrm)
+{
+  if (nvec < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
+  for (int i = 0; i < nvec; i++) { nrm[i] = mi_wrmsnormmask(X[i], W[i], id); }
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode mi_scaleaddmultivectorarray(int nvec, int nsum, sunrealtype* a, N_Vector* X,
+                                              N_Vector** Y, N_Vector** Z)
+{
+  if (nvec < 1 || nsum < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
+  N_Vector YY[NV_MAX_MULTI], ZZ[NV_MAX_MULTI];
+  if (nsum > NV_MAX_MULTI) { return SUN_ERR_ARG_OUTOFRANGE; }
+  for (int i = 0; i < nvec; i++)
+  {
+    for (int j = 0; j < nsum; j++)
+    {
+      YY[j] = Y[j][i];
+      ZZ[j] = Z[j][i];
+    }
+    SUNErrCode e = mi_scaleaddmulti(nsum, a, X[i], YY, ZZ);
+    if (e) { return e; }
+  }
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode mi_lincombvectorarray(int nvec, int nsum, sunrealtype* c, N_Vector** X,
+                                        N_Vector* Z)
+{
+  if (nvec < 1 || nsum < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
+  N_Vector Yv[64];
+  if (nsum > 64) { return SUN_ERR_ARG_OUTOFRANGE; }
+  for (int i = 0; i < nvec; i++)
+  {
+    for (int j = 0; j < nsum; j++) { Yv[j] = X[j][i]; }
+    SUNErrCode e = mi_lincomb(nsum, c, Yv, Z[i]);
+    if (e) { return e; }
+  }
+  return SUN_SUCCESS;
+}
+
+// ---- XBraid-style buffers (host buffers) ------------------------------------
+static SUNErrCode mi_bufsize(N_Vector x, sunindextype* size)
+{
+  *size = NVLEN(x) * (sunindextype)sizeof(sunrealtype);
+  return SUN_SUCCESS;
+}
+static SUNErrCode mi_bufpack(N_Vector x, void* buf)
+{
+  Mi355Exec* ex = EXEC(x);
+  hipError_t e  = hipMemcpyAsync(buf, DDATA(x), (size_t)NVLEN(x) * sizeof(double),
+                                 hipMemcpyDeviceToHost, ex->stream);
+  if (e == hipSuccess) { e = hipStreamSynchronize(ex->stream); }
+  return (e == hipSuccess) ? SUN_SUCCESS : SUN_ERR_EXT_FAIL;
+}
+static SUNErrCode mi_bufunpack(N_Vector x, void* buf)
+{
+  Mi355Exec* ex = EXEC(x);
+  hipError_t e  = hipMemcpyAsync(DDATA(x), buf, (size_t)NVLEN(x) * sizeof(double),
+                                 hipMemcpyHostToDevice, ex->stream);
+  if (e == hipSuccess) { e = hipStreamSynchronize(ex->stream); }
+  return (e == hipSuccess) ? SUN_SUCCESS : SUN_ERR_EXT_FAIL;
+}
+
+static void mi_printfile(N_Vector x, FILE* f)
+{
+  sunrealtype* h = mi_host_ptr(x);
+  N_VCopyFromDevice_Mi355(x);
+  for (sunindextype i = 0; i < NVLEN(x); i++) { fprintf(f, "%.16e\n", h[i]); }
+  fprintf(f, "\n");
+}
+static void mi_print(N_Vector x) { mi_printfile(x, stdout); }
+
+static void set_ops(N_Vector v)
+{
+  N_Vector_Ops o               = v->ops;
+  o->nvgetvectorid             = mi_getid;
+  o->nvclone                   = mi_clone;
+  o->nvcloneempty              = mi_clone_empty;
+  o->nvdestroy                 = mi_destroy;
+  o->nvspace                   = mi_space;
+  o->nvgetarraypointer         = mi_host_ptr;
+  o->nvgetdevicearraypointer   = mi_dev_ptr;
+  o->nvsetarraypointer         = mi_set_ptr;
+  o->nvgetlength               = mi_getlength;
+  o->nvgetlocallength          = mi_getlength;
+  o->nvlinearsum               = mi_linearsum;
+  o->nvconst                   = mi_const;
+  o->nvprod                    = mi_prod;
+  o->nvdiv                     = mi_div;
+  o->nvscale                   = mi_scale;
+  o->nvabs                     = mi_abs;
+  o->nvinv                     = mi_inv;
+  o->nvaddconst                = mi_addconst;
+  o->nvdotprod                 = mi_dotprod;
+  o->nvmaxnorm                 = mi_maxnorm;
+  o->nvwrmsnormmask            = mi_wrmsnormmask;
+  o->nvwrmsnorm                = mi_wrmsnorm;
+  o->nvmin                     = mi_min;
+  o->nvwl2norm                 = mi_wl2norm;
+  o->nvl1norm                  = mi_l1norm;
+  o->nvcompare                 = mi_compare;
+  o->nvinvtest                 = mi_invtest;
+  o->nvconstrmask              = mi_constrmask;
+  o->nvminquotient             = mi_minquotient;
+  o->nvdotprodlocal            = mi_dotprod;
+  o->nvmaxnormlocal            = mi_maxnorm;
+  o->nvminlocal                = mi_min;
+  o->nvl1normlocal             = mi_l1norm;
+  o->nvinvtestlocal            = mi_invtest;
+  o->nvconstrmasklocal         = mi_constrmask;
+  o->nvminquotientlocal        = mi_minquotient;
+  o->nvwsqrsumlocal            = mi_wsqrsum;
+  o->nvwsqrsummasklocal        = mi_wsqrsummask;
+  o->nvdotprodmultilocal       = mi_dotprodmulti;
+  o->nvbufsize                 = mi_bufsize;
+  o->nvbufpack                 = mi_bufpack;
+  o->nvbufunpack               = mi_bufunpack;
+  o->nvprint                   = mi_print;
+  o->nvprintfile               = mi_printfile;
+}
+
+static void set_fused(N_Vector v, sunbooleantype tf)
+{
+  N_Vector_Ops o                    = v->ops;
+  o->nvlinearcombination            = tf ? mi_lincomb : NULL;
+  o->nvscaleaddmulti                = tf ? mi_scaleaddmulti : NULL;
+  o->nvdotprodmulti                 = tf ? mi_dotprodmulti : NULL;
+  o->nvlinearsumvectorarray         = tf ? mi_linearsumvectorarray : NULL;
+  o->nvscalevectorarray             = tf ? mi_scalevectorarray : NULL;
+  o->nvconstvectorarray             = tf ? mi_constvectorarray : NULL;
+  o->nvwrmsnormvectorarray          = tf ? mi_wrmsnormvectorarray : NULL;
+  o->nvwrmsnormmaskvectorarray      = tf ? mi_wrmsnormmaskvectorarray : NULL;
+  o->nvscaleaddmultivectorarray     = tf ? mi_scaleaddmultivectorarray : NULL;
+  o->nvlinearcombinationvectorarray = tf ? mi_lincombvectorarray : NULL;
+}
+
+static N_Vector new_empty(sunindextype length, SUNContext sunctx, Mi355Exec* ex)
+{
+  N_Vector v = N_VNewEmpty(sunctx);
+  if (!v) { return NULL; }
+  set_ops(v);
+  set_fused(v, SUNTRUE);
+  Mi355Content* c = (Mi355Content*)calloc(1, sizeof(Mi355Content));
+  if (!c)
+  {
+    N_VFreeEmpty(v);
+    return NULL;
+  }
+  c->length = length;
+  if (ex)
+  {
+    ex->refcount++;
+    c->exec = ex;
+  }
+  else
+  {
+    c->exec = exec_create(sunctx);
+    if (!c->exec)
+    {
+      free(c);
+      N_VFreeEmpty(v);
+      return NULL;
+    }
+  }
+  v->content = c;
+  return v;
+}
+
+static int alloc_dev(N_Vector v)
+{
+  Mi355Content* c = CONTENT(v);
+  if (c->length <= 0) { return 1; }
+  // round up to a whole 128-bit pair so vector loads never straddle the end
+  size_t bytes = (size_t)((c->length + 1) & ~(sunindextype)1) * sizeof(double);
+  hipError_t e = hipMalloc((void**)&c->dev, bytes);
+  if (e != hipSuccess)
+  {
+    hip_fail(v->sunctx, e, "hipMalloc(vector)", __LINE__);
+    c->dev = NULL;
+    return 0;
+  }
+  c->own_dev = SUNTRUE;
+  return 1;
+}
+
+extern "C" N_Vector N_VNewEmpty_Mi355(sunindextype length, SUNContext sunctx)
+{
+  if (length < 0 || !sunctx) { return NULL; }
+  return new_empty(length, sunctx, NULL);
+}
+
+extern "C" N_Vector N_VNew_Mi355(sunindextype length, SUNContext sunctx)
+{
+  N_Vector v = N_VNewEmpty_Mi355(length, sunctx);
+  if (!v) { return NULL; }
+  if (!alloc_dev(v))
+  {
+    mi_destroy(v);
+    return NULL;
+  }
+  return v;
+}
+
+extern "C" N_Vector N_VMake_Mi355(sunindextype length, sunrealtype* h_vdata,
+                                  sunrealtype* d_vdata, SUNContext sunctx)
+{
+  if (!d_vdata || !aligned16(d_vdata)) { return NULL; }
+  N_Vector v = N_VNewEmpty_Mi355(length, sunctx);
+  if (!v) { return NULL; }
+  CONTENT(v)->dev  = d_vdata;
+  CONTENT(v)->host = h_vdata;
+  return v;
+}
+
+static N_Vector mi_clone_empty(N_Vector w)
+{
+  N_Vector v = new_empty(NVLEN(w), w->sunctx, EXEC(w));
+  if (!v) { return NULL; }
+  N_VCopyOps(w, v);
+  return v;
+}
+
+static N_Vector mi_clone(N_Vector w)
+{
+  N_Vector v = mi_clone_empty(w);
+  if (!v) { return NULL; }
+  if (!alloc_dev(v))
+  {
+    mi_destroy(v);
+    return NULL;
+  }
+  return v;
+}
+
+static void mi_destroy(N_Vector v)
+{
+  if (!v) { return; }
+  Mi355Content* c = CONTENT(v);
+  if (c)
+  {
+    if (c->own_dev && c->dev)
+    {
+      hipStreamSynchronize(c->exec->stream);
+      hipFree(c->dev);
+    }
+    if (c->own_host && c->host) { free(c->host); }
+    exec_release(c->exec);
+    free(c);
+    v->content = NULL;
+  }
+  N_VFreeEmpty(v);
+}
+
+extern "C" SUNErrCode N_VCopyToDevice_Mi355(N_Vector v)
+{
+  Mi355Content* c = CONTENT(v);
+  if (c->length == 0) { return SUN_SUCCESS; }
+  if (!c->host) { return SUN_ERR_ARG_CORRUPT; }
+  hipError_t e = hipMemcpyAsync(c->dev, c->host, (size_t)c->length * sizeof(double),
+                                hipMemcpyHostToDevice, c->exec->stream);
+  if (e == hipSuccess) { e = hipStreamSynchronize(c->exec->stream); }
+  if (e != hipSuccess)
+  {
+    hip_fail(v->sunctx, e, "N_VCopyToDevice", __LINE__);
+    return SUN_ERR_EXT_FAIL;
+  }
+  return SUN_SUCCESS;
+}
+
+extern "C" SUNErrCode N_VCopyFromDevice_Mi355(N_Vector v)
+{
+  Mi355Content* c = CONTENT(v);
+  if (c->length == 0) { return SUN_SUCCESS; }
+  sunrealtype* h = mi_host_ptr(v);
+  if (!h) { return SUN_ERR_MALLOC_FAIL; }
+  hipError_t e = hipMemcpyAsync(h, c->dev, (size_t)c->length * sizeof(double),
+                                hipMemcpyDeviceToHost, c->exec->stream);
+  if (e == hipSuccess) { e = hipStreamSynchronize(c->exec->stream); }
+  if (e != hipSuccess)
+  {
+    hip_fail(v->sunctx, e, "N_VCopyFromDevice", __LINE__);
+    return SUN_ERR_EXT_FAIL;
+  }
+  return SUN_SUCCESS;
+}
+
+extern "C" SUNErrCode N_VCopyFromHostArray_Mi355(N_Vector v, const sunrealtype* src)
+{
+  Mi355Content* c = CONTENT(v);
+  if (c->length == 0) { return SUN_SUCCESS; }
+  hipError_t e = hipMemcpyAsync(c->dev, src, (size_t)c->length * sizeof(double),
+                                hipMemcpyHostToDevice, c->exec->stream);
+  if (e == hipSuccess) { e = hipStreamSynchronize(c->exec->stream); }
+  if (e != hipSuccess)
+  {
+    hip_fail(v->sunctx, e, "N_VCopyFromHostArray", __LINE__);
+    return SUN_ERR_EXT_FAIL;
+  }
+  return SUN_SUCCESS;
+}
+
+extern "C" SUNErrCode N_VCopyToHostArray_Mi355(N_Vector v, sunrealtype* dst)
+{
+  Mi355Content* c = CONTENT(v);
+  if (c->length == 0) { return SUN_SUCCESS; }
+  hipError_t e = hipMemcpyAsync(dst, c->dev, (size_t)c->length * sizeof(double),
+                                hipMemcpyDeviceToHost, c->exec->stream);
+  if (e == hipSuccess) { e = hipStreamSynchronize(c->exec->stream); }
+  if (e != hipSuccess)
+  {
+    hip_fail(v->sunctx, e, "N_VCopyToHostArray", __LINE__);
+    return SUN_ERR_EXT_FAIL;
+  }
+  return SUN_SUCCESS;
+}
+
+extern "C" void* N_VGetStream_Mi355(N_Vector v) { return (void*)EXEC(v)->stream; }
+
+extern "C" SUNErrCode N_VSynchronize_Mi355(N_Vector v)
+{
+  hipError_t e = hipStreamSynchronize(EXEC(v)->stream);
+  if (e != hipSuccess)
+  {
+    hip_fail(v->sunctx, e, "hipStreamSynchronize", __LINE__);
+    return SUN_ERR_EXT_FAIL;
+  }
+  return SUN_SUCCESS;
+}
+
+extern "C" SUNErrCode N_VEnableFusedOps_Mi355(N_Vector v, sunbooleantype tf)
+{
+  if (!v || !v->ops) { return SUN_ERR_ARG_CORRUPT; }
+  set_fused(v, tf);
+  return SUN_SUCCESS;
+}
+
+extern "C" long int N_VGetNumLaunches_Mi355(N_Vector v) { return EXEC(v)->launches; }

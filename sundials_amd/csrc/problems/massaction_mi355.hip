@@ -1,0 +1,318 @@
+// -----------------------------------------------------------------------------
+// Mass-action right-hand side and block Jacobian on the MI355X, and the
+// coupled-system driver (every cell in one N_Vector, block-diagonal Jacobian,
+// BDF + modified Newton + batched block LU).
+//
+// RHS: one lane per cell; the workgroup stages its cells' species through LDS
+// with coalesced loads (row stride m+1 to spread LDS banks), accumulates f in
+// LDS in the CPU's reaction order, and writes f back coalesced.  Rate
+// constants are stored reaction-major (r*ncells + cell) so each reaction's
+// rates are one coalesced load across the workgroup.
+// Jacobian: one group of pow2ceil(m) lanes per cell, lane = row; the block is
+// assembled in LDS (each lane only touches its own row, in the CPU's
+// reaction / target / column order) and streamed out column by column.
+// -----------------------------------------------------------------------------
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "sunamd/cvode_sunamd.h"
+#include "sunamd/massaction_mi355.h"
+#include "sunamd/nvector_mi355.h"
+#include "sunamd/sunmatrix_mi355block.h"
+
+struct SUNAMD_MassActionDev_
+{
+  int m, R;
+  long ncells;
+  int* idx;   // 4R
+  double* kf; // R*ncells, reaction-major
+  double* kb;
+  hipStream_t stream;
+};
+
+static int hipfail(hipError_t e, const char* what)
+{
+  fprintf(stderr, "[sundials_amd massaction] %s failed: %s\n", what, hipGetErrorString(e));
+  fflush(stderr);
+  return -1;
+}
+
+#define RHS_CPB 64
+
+__global__ void __launch_bounds__(RHS_CPB) k_ma_rhs(const double* __restrict__ y,
+                                                     double* __restrict__ f,
+                                                     const int* __restrict__ idx,
+                                                     const double* __restrict__ kf,
+                                                     const double* __restrict__ kb, int m,
+                                                     int R, long ncells)
+{
+  extern __shared__ double sm[];
+  const int ld     = m + 1;
+  double* ys       = sm;
+  double* fs       = sm + RHS_CPB * ld;
+  const long cell0 = (long)blockIdx.x * RHS_CPB;
+  const long rem   = ncells - cell0;
+  const int ncb    = rem < RHS_CPB ? (int)rem : RHS_CPB;
+  const double* yb = y + cell0 * m;
+  for (int e = threadIdx.x; e < ncb * m; e += RHS_CPB)
+  {
+    int c            = e / m;
+    ys[c * ld + e - c * m] = yb[e];
+  }
+  __syncthreads();
+  const int c = threadIdx.x;
+  if (c < ncb)
+  {
+    const double* yc = ys + c * ld;
+    double* fc       = fs + c * ld;
+    for (int s = 0; s < m; s++) { fc[s] = 0.0; }
+    const long cell = cell0 + c;
+    for (int r = 0; r < R; r++)
+    {
+      const int a = idx[4 * r], b = idx[4 * r + 1], cc = idx[4 * r + 2], d = idx[4 * r + 3];
+      double wf = (kf[(long)r * ncells + cell] * yc[a]) * yc[b];
+      double wb = (kb[(long)r * ncells + cell] * yc[cc]) * yc[d];
+      double w  = wf - wb;
+      fc[a]     = fc[a] - w;
+      fc[b]     = fc[b] - w;
+      fc[cc]    = fc[cc] + w;
+      fc[d]     = fc[d] + w;
+    }
+  }
+  __syncthreads();
+  double* fb = f + cell0 * m;
+  for (int e = threadIdx.x; e < ncb * m; e += RHS_CPB)
+  {
+    int cc = e / m;
+    fb[e]  = fs[cc * ld + e - cc * m];
+  }
+}
+
+__global__ void __launch_bounds__(64) k_ma_jac(const double* __restrict__ y,
+                                               double* __restrict__ J,
+                                               const int* __restrict__ idx,
+                                               const double* __restrict__ kf,
+                                               const double* __restrict__ kb, int m, int G,
+                                               int R, long ncells)
+{
+  extern __shared__ double sj[];
+  const int lane   = threadIdx.x;
+  const int grp    = lane / G;
+  const int i      = lane % G;
+  const long cell  = (long)blockIdx.x * (64 / G) + grp;
+  const bool live  = cell < ncells;
+  const bool valid = live && i < m;
+  double* Jl       = sj + (long)grp * m * m;
+  if (live)
+  {
+    for (int e = i; e < m * m; e += G) { Jl[e] = 0.0; }
+  }
+  __syncthreads();
+  if (valid)
+  {
+    const double* yc = y + cell * m;
+    for (int r = 0; r < R; r++)
+    {
+      int ix[4] = {idx[4 * r], idx[4 * r + 1], idx[4 * r + 2], idx[4 * r + 3]};
+      if (ix[0] != i && ix[1] != i && ix[2] != i && ix[3] != i) { continue; }
+      double kfv   = kf[(long)r * ncells + cell];
+      double kbv   = kb[(long)r * ncells + cell];
+      double dw[4] = {kfv * yc[ix[1]], kfv * yc[ix[0]], -(kbv * yc[ix[3]]), -(kbv * yc[ix[2]])};
+      for (int t = 0; t < 4; t++)
+      {
+        if (ix[t] != i) { continue; }
+        for (int q = 0; q < 4; q++)
+        {
+          int e = ix[q] * m + i;
+          Jl[e] = (t < 2) ? (Jl[e] - dw[q]) : (Jl[e] + dw[q]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (live)
+  {
+    double* Jc = J + cell * (long)m * m;
+    for (int e = i; e < m * m; e += G) { Jc[e] = Jl[e]; }
+  }
+}
+
+static int pow2ceil_h(int m)
+{
+  int g = 1;
+  while (g < m) { g <<= 1; }
+  return g;
+}
+
+extern "C" SUNAMD_MassActionDev* SUNAMD_MassActionToDevice(const SUNAMD_MassAction* p,
+                                                           void* stream)
+{
+  if (!p || p->m > 64) { return NULL; }
+  SUNAMD_MassActionDev* d = (SUNAMD_MassActionDev*)calloc(1, sizeof(SUNAMD_MassActionDev));
+  if (!d) { return NULL; }
+  d->m      = p->m;
+  d->R      = p->R;
+  d->ncells = (long)p->ncells;
+  d->stream = (hipStream_t)stream;
+  size_t nr = (size_t)p->R * (size_t)p->ncells;
+  double* t = (double*)malloc(sizeof(double) * nr);
+  hipError_t e = hipMalloc((void**)&d->idx, sizeof(int) * 4 * (size_t)p->R);
+  if (e == hipSuccess) { e = hipMalloc((void**)&d->kf, sizeof(double) * nr); }
+  if (e == hipSuccess) { e = hipMalloc((void**)&d->kb, sizeof(double) * nr); }
+  if (e == hipSuccess && t)
+  {
+    e = hipMemcpy(d->idx, p->idx, sizeof(int) * 4 * (size_t)p->R, hipMemcpyHostToDevice);
+    for (int pass = 0; pass < 2 && e == hipSuccess; pass++)
+    {
+      const double* src = pass ? p->kb : p->kf;
+      for (long c = 0; c < d->ncells; c++)
+      {
+        for (int r = 0; r < p->R; r++) { t[(size_t)r * d->ncells + c] = src[c * p->R + r]; }
+      }
+      e = hipMemcpy(pass ? d->kb : d->kf, t, sizeof(double) * nr, hipMemcpyHostToDevice);
+    }
+  }
+  free(t);
+  if (e != hipSuccess || !t)
+  {
+    if (e != hipSuccess) { hipfail(e, "problem upload"); }
+    SUNAMD_MassActionDevFree(d);
+    return NULL;
+  }
+  return d;
+}
+
+extern "C" void SUNAMD_MassActionDevFree(SUNAMD_MassActionDev* d)
+{
+  if (!d) { return; }
+  if (d->idx) { (void)hipFree(d->idx); }
+  if (d->kf) { (void)hipFree(d->kf); }
+  if (d->kb) { (void)hipFree(d->kb); }
+  free(d);
+}
+
+extern "C" int SUNAMD_MassActionRhsDevArrays(SUNAMD_MassActionDev* d, const double* dy,
+                                             double* df)
+{
+  long g      = (d->ncells + RHS_CPB - 1) / RHS_CPB;
+  size_t shm  = sizeof(double) * 2 * RHS_CPB * (size_t)(d->m + 1);
+  k_ma_rhs<<<(unsigned)g, RHS_CPB, shm, d->stream>>>(dy, df, d->idx, d->kf, d->kb, d->m, d->R,
+                                                     d->ncells);
+  hipError_t e = hipGetLastError();
+  return (e == hipSuccess) ? 0 : hipfail(e, "rhs launch");
+}
+
+extern "C" int SUNAMD_MassActionJacDevArrays(SUNAMD_MassActionDev* d, const double* dy,
+                                             double* dJ)
+{
+  int G      = pow2ceil_h(d->m);
+  int cpw    = 64 / G;
+  long g     = (d->ncells + cpw - 1) / cpw;
+  size_t shm = sizeof(double) * (size_t)cpw * d->m * d->m;
+  k_ma_jac<<<(unsigned)g, 64, shm, d->stream>>>(dy, dJ, d->idx, d->kf, d->kb, d->m, G, d->R,
+                                                d->ncells);
+  hipError_t e = hipGetLastError();
+  return (e == hipSuccess) ? 0 : hipfail(e, "jac launch");
+}
+
+extern "C" int SUNAMD_MassActionRhsDev(sunrealtype t, N_Vector y, N_Vector ydot,
+                                       void* user_data)
+{
+  (void)t;
+  SUNAMD_MassActionDev* d = (SUNAMD_MassActionDev*)user_data;
+  return SUNAMD_MassActionRhsDevArrays(d, N_VGetDeviceArrayPointer(y),
+                                       N_VGetDeviceArrayPointer(ydot));
+}
+
+extern "C" int SUNAMD_MassActionJacDev(sunrealtype t, N_Vector y, N_Vector fy, SUNMatrix J,
+                                       void* user_data, N_Vector tmp1, N_Vector tmp2,
+                                       N_Vector tmp3)
+{
+  (void)t;
+  (void)fy;
+  (void)tmp1;
+  (void)tmp2;
+  (void)tmp3;
+  SUNAMD_MassActionDev* d = (SUNAMD_MassActionDev*)user_data;
+  return SUNAMD_MassActionJacDevArrays(d, N_VGetDeviceArrayPointer(y),
+                                       SUNMatrix_Mi355BlockDense_DeviceData(J));
+}
+
+extern "C" int SUNAMD_MassActionSolveCoupled_Mi355(const SUNAMD_MassAction* p, double rtol,
+                                                   double atol, double tout,
+                                                   int lu_precision, double* yout,
+                                                   SUNAMD_SolveStats* st)
+{
+  SUNContext ctx = NULL;
+  if (SUNContext_Create(SUN_COMM_NULL, &ctx)) { return -1; }
+  const sunindextype N = (sunindextype)p->ncells * p->m;
+  int rc               = -1;
+  N_Vector y           = N_VNew_Mi355(N, ctx);
+  SUNAMD_MassActionDev* dev = NULL;
+  SUNMatrix A          = NULL;
+  SUNLinearSolver LS   = NULL;
+  void* cv             = NULL;
+  sunrealtype t        = 0.0;
+  if (!y) { goto done; }
+  if (N_VCopyFromHostArray_Mi355(y, p->y0)) { goto done; }
+  dev = SUNAMD_MassActionToDevice(p, N_VGetStream_Mi355(y));
+  if (!dev) { goto done; }
+  A = SUNMatrix_Mi355BlockDense(p->ncells, p->m, N_VGetStream_Mi355(y), ctx);
+  if (!A) { goto done; }
+  LS = SUNLinSol_Mi355BlockLU(y, A, lu_precision, ctx);
+  if (!LS) { goto done; }
+  cv = CVodeCreate(CV_BDF, ctx);
+  if (!cv) { goto done; }
+  if (CVodeInit(cv, SUNAMD_MassActionRhsDev, 0.0, y)) { goto done; }
+  if (CVodeSStolerances(cv, rtol, atol)) { goto done; }
+  if (CVodeSetUserData(cv, dev)) { goto done; }
+  if (CVodeSetLinearSolver(cv, LS, A)) { goto done; }
+  if (CVodeSetJacFn(cv, SUNAMD_MassActionJacDev)) { goto done; }
+  if (CVodeSetMaxNumSteps(cv, 1000000)) { goto done; }
+  {
+    if (hipDeviceSynchronize() != hipSuccess) { goto done; }
+    long l0 = N_VGetNumLaunches_Mi355(y);
+    auto t0 = std::chrono::steady_clock::now();
+    int flag = CVode(cv, tout, y, &t, CV_NORMAL);
+    if (hipDeviceSynchronize() != hipSuccess) { goto done; }
+    auto t1 = std::chrono::steady_clock::now();
+    if (st)
+    {
+      long counts[8];
+      int orders[2];
+      double steps[4];
+      CVodeGetAllStats(cv, counts, orders, steps);
+      st->nst      = counts[0];
+      st->nfe      = counts[1];
+      st->nsetups  = counts[2];
+      st->netf     = counts[3];
+      st->nni      = counts[4];
+      st->nnf      = counts[5];
+      st->ncfn     = counts[6];
+      st->nje      = counts[7];
+      st->qlast    = orders[0];
+      st->qcur     = orders[1];
+      st->hinused  = steps[0];
+      st->hlast    = steps[1];
+      st->hcur     = steps[2];
+      st->tcur     = steps[3];
+      st->wall_s   = std::chrono::duration<double>(t1 - t0).count();
+      st->launches = N_VGetNumLaunches_Mi355(y) - l0;
+      st->flag     = flag;
+    }
+    if (yout && N_VCopyToHostArray_Mi355(y, yout)) { goto done; }
+    rc = (flag < 0) ? flag : 0;
+  }
+done:
+  if (cv) { CVodeFree(&cv); }
+  if (LS) { SUNLinSolFree(LS); }
+  if (A) { SUNMatDestroy(A); }
+  if (dev) { SUNAMD_MassActionDevFree(dev); }
+  if (y) { N_VDestroy(y); }
+  SUNContext_Free(&ctx);
+  return rc;
+}

@@ -1,0 +1,784 @@
+// -----------------------------------------------------------------------------
+// Block-diagonal dense SUNMatrix + batched LU solver for MI355X (gfx950).
+//
+// Reference behaviour restated:
+//   matrix ops        src/sunmatrix/dense/sunmatrix_dense.c:229-334
+//   LU factor/solve   src/sundials/sundials_dense.c:85-186 (GETRF / GETRS)
+//   solver wrapper    src/sunlinsol/dense/sunlinsol_dense.c:128-183
+//   batched precedent src/sunlinsol/magmadense/sunlinsol_magmadense.cpp:314-440
+//
+// Kernel design (one m x m block per group of G = pow2ceil(m) lanes, 64/G
+// blocks per 64-wide wavefront, one wavefront per workgroup):
+//   * lane r holds row r of its block in registers (m values); every column
+//     load/store is one coalesced 8*m-byte segment per block;
+//   * partial pivoting without moving data: each lane carries the physical
+//     row position it currently occupies; the pivot search is a butterfly
+//     arg-max over the group keyed on (|a_rk|, -position), which reproduces
+//     the reference's "first maximum, strict >" choice in the reference's
+//     row order, and a "swap" just exchanges two position tags;
+//   * the pivot row is broadcast through LDS; each remaining row scales its
+//     multiplier with the reciprocal of the pivot and subtracts
+//     pivot_row[j] * multiplier column by column, skipping zero pivot-row
+//     entries -- the reference's exact per-element operation sequence, so in
+//     FP64 the factors are bit-identical;
+//   * rows are written back to their final physical position, so the stored
+//     factors and the pivot vector are those of the reference;
+//   * GETRS gathers b through the pivot sequence, then runs the unit-lower
+//     and upper sweeps with shuffle broadcasts of the solved entry.
+// The FP32 variant factors in single precision into a separate 4*m^2-byte
+// factor array and solves in FP64 arithmetic from those factors.
+// -----------------------------------------------------------------------------
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "sunamd/nvector_mi355.h"
+#include "sunamd/sunmatrix_mi355block.h"
+
+#define EW_BLOCK 256
+
+static void hip_fail(SUNContext ctx, hipError_t e, const char* what, int line)
+{
+  fprintf(stderr, "[sundials_amd blockdense] %s failed (line %d): %s\n", what, line,
+          hipGetErrorString(e));
+  fflush(stderr);
+  SUNAMD_SetLastError(ctx, SUN_ERR_EXT_FAIL);
+}
+#define HIPCHK(ctx, call)                                           \
+  do {                                                              \
+    hipError_t e_ = (call);                                         \
+    if (e_ != hipSuccess) { hip_fail((ctx), e_, #call, __LINE__); } \
+  } while (0)
+#define LAUNCHCHK(ctx) HIPCHK(ctx, hipGetLastError())
+
+// ----------------------------------------------------------------------------
+// matrix content
+// ----------------------------------------------------------------------------
+struct BlockMat
+{
+  sunindextype nblocks, m;
+  double* dev;
+  double* host;
+  hipStream_t stream;
+};
+#define BM(A) ((BlockMat*)((A)->content))
+
+// ----------------------------------------------------------------------------
+// elementwise matrix kernels
+// ----------------------------------------------------------------------------
+__global__ void k_fill(double* __restrict__ a, long n, double v)
+{
+  for (long i = (long)blockIdx.x * EW_BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * EW_BLOCK)
+  {
+    a[i] = v;
+  }
+}
+
+// A = c*A + I (per element: a *= c; a += 1 on the diagonal)
+__global__ void k_scaleaddi(double* __restrict__ a, long n, long m, double c)
+{
+  const long mm = m * m;
+  for (long i = (long)blockIdx.x * EW_BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * EW_BLOCK)
+  {
+    long e   = i % mm;
+    double v = a[i] * c;
+    if (e % (m + 1) == 0) { v = v + 1.0; }
+    a[i] = v;
+  }
+}
+
+// A = c*A + B
+__global__ void k_scaleadd(double* __restrict__ a, const double* __restrict__ b, long n, double c)
+{
+  for (long i = (long)blockIdx.x * EW_BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * EW_BLOCK)
+  {
+    a[i] = c * a[i] + b[i];
+  }
+}
+
+// y = A x per block; one thread per row, column order as the reference
+__global__ void k_matvec(const double* __restrict__ a, const double* __restrict__ x,
+                         double* __restrict__ y, long nblocks, long m)
+{
+  const long nrows = nblocks * m;
+  for (long g = (long)blockIdx.x * EW_BLOCK + threadIdx.x; g < nrows; g += (long)gridDim.x * EW_BLOCK)
+  {
+    long blk          = g / m;
+    long i            = g - blk * m;
+    const double* ab  = a + blk * m * m;
+    const double* xb  = x + blk * m;
+    double acc        = 0.0;
+    for (long j = 0; j < m; j++) { acc = acc + ab[j * m + i] * xb[j]; }
+    y[g] = acc;
+  }
+}
+
+static int ew_grid(long n)
+{
+  long g = (n + EW_BLOCK - 1) / EW_BLOCK;
+  if (g > 65536) { g = 65536; }
+  if (g < 1) { g = 1; }
+  return (int)g;
+}
+
+// ----------------------------------------------------------------------------
+// batched LU kernels
+// ----------------------------------------------------------------------------
+__host__ __device__ constexpr int pow2ceil(int m)
+{
+  return m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : m <= 8 ? 8 : m <= 16 ? 16 : m <= 32 ? 32 : 64;
+}
+
+template <typename T>
+__device__ __forceinline__ T to_t(double v)
+{
+  return (T)v;
+}
+
+// Factor nblocks m x m blocks in place (T = double) or into an FP32 factor
+// array (T = float).  info receives min over failing blocks of
+// (block*m + k + 1) for the first zero pivot k.
+template <int M, typename T>
+__global__ void __launch_bounds__(64) k_getrf(const double* A, T* F /* may alias A */,
+                                              int* __restrict__ piv, long nblocks,
+                                              unsigned long long* info)
+{
+  constexpr int G   = pow2ceil(M);
+  constexpr int BPW = 64 / G;
+  __shared__ T prow[BPW][M];
+
+  const int lane   = threadIdx.x;
+  const int grp    = lane / G;
+  const int r      = lane % G;
+  const long blk   = (long)blockIdx.x * BPW + grp;
+  const bool valid = (blk < nblocks) && (r < M);
+
+  T a[M];
+  const double* Ab = A + blk * (long)M * M;
+#pragma unroll
+  for (int j = 0; j < M; j++) { a[j] = valid ? to_t<T>(Ab[j * M + r]) : (T)0; }
+
+  int pos  = r; // physical row currently held by this lane
+  int myp  = 0; // pivot row chosen at step k == r
+  int fail = 0;
+
+#pragma unroll
+  for (int k = 0; k < M; k++)
+  {
+    // arg-max of |a_rk| over rows at positions >= k; ties -> lowest position
+    T v    = (valid && pos >= k) ? (T)fabs((double)a[k]) : (T)-1;
+    int bp = valid ? pos : INT_MAX;
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1)
+    {
+      T ov   = __shfl_xor(v, off, 64);
+      int op = __shfl_xor(bp, off, 64);
+      if (ov > v || (ov == v && op < bp))
+      {
+        v  = ov;
+        bp = op;
+      }
+    }
+    if (r == k) { myp = bp; }
+    if (v == (T)0 && fail == 0) { fail = k + 1; }
+
+    // exchange the position tags of rows k and bp
+    if (pos == k) { pos = bp; }
+    else if (pos == bp) { pos = k; }
+
+    // the pivot row (now at position k) publishes columns k..M-1
+    if (valid && pos == k)
+    {
+#pragma unroll
+      for (int j = k; j < M; j++) { prow[grp][j] = a[j]; }
+    }
+    __syncthreads();
+    if (valid && pos > k && fail == 0)
+    {
+      const T mult = (T)1 / prow[grp][k];
+      const T l    = a[k] * mult;
+      a[k]         = l;
+#pragma unroll
+      for (int j = k + 1; j < M; j++)
+      {
+        const T pj = prow[grp][j];
+        if (pj != (T)0) { a[j] = a[j] - pj * l; }
+      }
+    }
+    __syncthreads();
+  }
+
+  if (valid)
+  {
+    T* Fb = F + blk * (long)M * M;
+#pragma unroll
+    for (int j = 0; j < M; j++) { Fb[j * M + pos] = a[j]; }
+    piv[blk * M + r] = myp;
+    if (r == 0 && fail != 0)
+    {
+      atomicMin(info, (unsigned long long)(blk * M + fail));
+    }
+  }
+}
+
+// x = LU^{-1} P b per block (x may alias b)
+template <int M, typename T>
+__global__ void __launch_bounds__(64) k_getrs(const T* __restrict__ F, const int* __restrict__ piv,
+                                              const double* b, double* x, long nblocks)
+{
+  constexpr int G   = pow2ceil(M);
+  constexpr int BPW = 64 / G;
+  const int lane    = threadIdx.x;
+  const int grp     = lane / G;
+  const int r       = lane % G;
+  const long blk    = (long)blockIdx.x * BPW + grp;
+  const bool valid  = (blk < nblocks) && (r < M);
+  const int base    = grp * G;
+
+  double lu[M];
+  const T* Fb = F + blk * (long)M * M;
+#pragma unroll
+  for (int j = 0; j < M; j++) { lu[j] = valid ? (double)Fb[j * M + r] : 0.0; }
+
+  // source position of b after the sequential swaps b[k] <-> b[p[k]]
+  int idx = r;
+  if (valid)
+  {
+    const int* p = piv + blk * M;
+#pragma unroll
+    for (int k = M - 1; k >= 0; k--)
+    {
+      int pk = p[k];
+      if (idx == k) { idx = pk; }
+      else if (idx == pk) { idx = k; }
+    }
+  }
+  double bi = valid ? b[blk * M + idx] : 0.0;
+
+  // L y = P b (unit lower)
+#pragma unroll
+  for (int k = 0; k < M - 1; k++)
+  {
+    double bk = __shfl(bi, base + k, 64);
+    if (r > k) { bi = bi - lu[k] * bk; }
+  }
+  // U x = y
+#pragma unroll
+  for (int k = M - 1; k > 0; k--)
+  {
+    if (r == k) { bi = bi / lu[k]; }
+    double bk = __shfl(bi, base + k, 64);
+    if (r < k) { bi = bi - lu[k] * bk; }
+  }
+  if (r == 0) { bi = bi / lu[0]; }
+  if (valid) { x[blk * M + r] = bi; }
+}
+
+// generic-size fallback (any m <= 64): block staged in LDS, same arithmetic
+template <typename T>
+__global__ void __launch_bounds__(64) k_getrf_lds(const double* A, T* F /* may alias A */,
+                                                  int* __restrict__ piv, long nblocks, int M,
+                                                  unsigned long long* info)
+{
+  extern __shared__ unsigned char smem_raw[];
+  T* s         = reinterpret_cast<T*>(smem_raw); // M*M, column-major, physical rows
+  const long blk = blockIdx.x;
+  const int r    = threadIdx.x;
+  if (blk >= nblocks) { return; }
+  const double* Ab = A + blk * (long)M * M;
+  for (int e = r; e < M * M; e += 64) { s[e] = to_t<T>(Ab[e]); }
+  __syncthreads();
+  __shared__ int sp[64];
+  __shared__ int sfail;
+  if (r == 0)
+  {
+    sfail = 0;
+    for (int k = 0; k < M; k++)
+    {
+      int l = k;
+      for (int i = k + 1; i < M; i++)
+      {
+        if (fabs((double)s[k * M + i]) > fabs((double)s[k * M + l])) { l = i; }
+      }
+      sp[k] = l;
+      if (s[k * M + l] == (T)0)
+      {
+        sfail = k + 1;
+        break;
+      }
+      if (l != k)
+      {
+        for (int j = 0; j < M; j++)
+        {
+          T t          = s[j * M + l];
+          s[j * M + l] = s[j * M + k];
+          s[j * M + k] = t;
+        }
+      }
+      T mult = (T)1 / s[k * M + k];
+      for (int i = k + 1; i < M; i++) { s[k * M + i] *= mult; }
+      for (int j = k + 1; j < M; j++)
+      {
+        T akj = s[j * M + k];
+        if (akj != (T)0)
+        {
+          for (int i = k + 1; i < M; i++) { s[j * M + i] = s[j * M + i] - akj * s[k * M + i]; }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  T* Fb = F + blk * (long)M * M;
+  for (int e = r; e < M * M; e += 64) { Fb[e] = s[e]; }
+  if (r < M) { piv[blk * M + r] = sp[r]; }
+  if (r == 0 && sfail) { atomicMin(info, (unsigned long long)(blk * M + sfail)); }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_getrs_lds(const T* __restrict__ F, const int* __restrict__ piv,
+                                                  const double* b, double* x, long nblocks, int M)
+{
+  __shared__ double sb[64];
+  const long blk = blockIdx.x;
+  const int r    = threadIdx.x;
+  if (blk >= nblocks) { return; }
+  if (r < M) { sb[r] = b[blk * M + r]; }
+  __syncthreads();
+  if (r == 0)
+  {
+    const T* Fb  = F + blk * (long)M * M;
+    const int* p = piv + blk * M;
+    for (int k = 0; k < M; k++)
+    {
+      int pk = p[k];
+      if (pk != k)
+      {
+        double t = sb[k];
+        sb[k]    = sb[pk];
+        sb[pk]   = t;
+      }
+    }
+    for (int k = 0; k < M - 1; k++)
+    {
+      for (int i = k + 1; i < M; i++) { sb[i] = sb[i] - (double)Fb[k * M + i] * sb[k]; }
+    }
+    for (int k = M - 1; k > 0; k--)
+    {
+      sb[k] = sb[k] / (double)Fb[k * M + k];
+      for (int i = 0; i < k; i++) { sb[i] = sb[i] - (double)Fb[k * M + i] * sb[k]; }
+    }
+    sb[0] = sb[0] / (double)Fb[0];
+  }
+  __syncthreads();
+  if (r < M) { x[blk * M + r] = sb[r]; }
+}
+
+__global__ void k_info_reset(unsigned long long* info) { *info = ULLONG_MAX; }
+
+// ----------------------------------------------------------------------------
+// launch dispatch over block sizes with register-resident kernels
+// ----------------------------------------------------------------------------
+#define LU_SIZES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(12) X(16) X(24) X(32) X(48) X(64)
+
+template <typename T>
+static bool launch_getrf(int m, const double* A, T* F, int* piv, long nblocks,
+                         unsigned long long* info, hipStream_t s)
+{
+  switch (m)
+  {
+#define GETRF_CASE(MM)                                                                     \
+  case MM:                                                                                 \
+  {                                                                                        \
+    constexpr int BPW = 64 / pow2ceil(MM);                                                 \
+    long g            = (nblocks + BPW - 1) / BPW;                                         \
+    k_getrf<MM, T><<<(unsigned)g, 64, 0, s>>>(A, F, piv, nblocks, info);                   \
+    return true;                                                                           \
+  }
+    LU_SIZES(GETRF_CASE)
+#undef GETRF_CASE
+  default: break;
+  }
+  if (m > 64) { return false; }
+  k_getrf_lds<T><<<(unsigned)nblocks, 64, sizeof(T) * m * m, s>>>(A, F, piv, nblocks, m, info);
+  return true;
+}
+
+template <typename T>
+static bool launch_getrs(int m, const T* F, const int* piv, const double* b, double* x,
+                         long nblocks, hipStream_t s)
+{
+  switch (m)
+  {
+#define GETRS_CASE(MM)                                                                     \
+  case MM:                                                                                 \
+  {                                                                                        \
+    constexpr int BPW = 64 / pow2ceil(MM);                                                 \
+    long g            = (nblocks + BPW - 1) / BPW;                                         \
+    k_getrs<MM, T><<<(unsigned)g, 64, 0, s>>>(F, piv, b, x, nblocks);                      \
+    return true;                                                                           \
+  }
+    LU_SIZES(GETRS_CASE)
+#undef GETRS_CASE
+  default: break;
+  }
+  if (m > 64) { return false; }
+  k_getrs_lds<T><<<(unsigned)nblocks, 64, 0, s>>>(F, piv, b, x, nblocks, m);
+  return true;
+}
+
+// ----------------------------------------------------------------------------
+// SUNMatrix
+// ----------------------------------------------------------------------------
+static SUNMatrix_ID bm_getid(SUNMatrix A)
+{
+  (void)A;
+  return SUNMATRIX_CUSTOM;
+}
+
+static void bm_destroy(SUNMatrix A)
+{
+  if (!A) { return; }
+  if (A->content)
+  {
+    BlockMat* c = BM(A);
+    if (c->dev)
+    {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipFree(c->dev);
+    }
+    free(c->host);
+    free(c);
+    A->content = NULL;
+  }
+  SUNMatFreeEmpty(A);
+}
+
+static SUNMatrix bm_clone(SUNMatrix A)
+{
+  return SUNMatrix_Mi355BlockDense(BM(A)->nblocks, BM(A)->m, (void*)BM(A)->stream, A->sunctx);
+}
+
+static long bm_len(SUNMatrix A) { return (long)(BM(A)->nblocks * BM(A)->m * BM(A)->m); }
+
+static SUNErrCode bm_zero(SUNMatrix A)
+{
+  long n = bm_len(A);
+  k_fill<<<ew_grid(n), EW_BLOCK, 0, BM(A)->stream>>>(BM(A)->dev, n, 0.0);
+  LAUNCHCHK(A->sunctx);
+  return SUN_SUCCESS;
+}
+
+static bool compatible(SUNMatrix A, SUNMatrix B)
+{
+  return B && B->ops->getid == bm_getid && BM(A)->nblocks == BM(B)->nblocks &&
+         BM(A)->m == BM(B)->m;
+}
+
+static SUNErrCode bm_copy(SUNMatrix A, SUNMatrix B)
+{
+  if (!compatible(A, B)) { return SUN_ERR_ARG_DIMSMISMATCH; }
+  HIPCHK(A->sunctx, hipMemcpyAsync(BM(B)->dev, BM(A)->dev, sizeof(double) * bm_len(A),
+                                   hipMemcpyDeviceToDevice, BM(A)->stream));
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode bm_scaleaddi(sunrealtype c, SUNMatrix A)
+{
+  long n = bm_len(A);
+  k_scaleaddi<<<ew_grid(n), EW_BLOCK, 0, BM(A)->stream>>>(BM(A)->dev, n, (long)BM(A)->m, c);
+  LAUNCHCHK(A->sunctx);
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode bm_scaleadd(sunrealtype c, SUNMatrix A, SUNMatrix B)
+{
+  if (!compatible(A, B)) { return SUN_ERR_ARG_DIMSMISMATCH; }
+  long n = bm_len(A);
+  k_scaleadd<<<ew_grid(n), EW_BLOCK, 0, BM(A)->stream>>>(BM(A)->dev, BM(B)->dev, n, c);
+  LAUNCHCHK(A->sunctx);
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode bm_matvec(SUNMatrix A, N_Vector x, N_Vector y)
+{
+  if (N_VGetLength(x) != BM(A)->nblocks * BM(A)->m ||
+      N_VGetLength(y) != BM(A)->nblocks * BM(A)->m)
+  {
+    return SUN_ERR_ARG_DIMSMISMATCH;
+  }
+  double* xd = N_VGetDeviceArrayPointer(x);
+  double* yd = N_VGetDeviceArrayPointer(y);
+  if (!xd || !yd || xd == yd) { return SUN_ERR_ARG_INCOMPATIBLE; }
+  long rows = (long)(BM(A)->nblocks * BM(A)->m);
+  k_matvec<<<ew_grid(rows), EW_BLOCK, 0, BM(A)->stream>>>(BM(A)->dev, xd, yd,
+                                                         (long)BM(A)->nblocks, (long)BM(A)->m);
+  LAUNCHCHK(A->sunctx);
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode bm_space(SUNMatrix A, long int* lenrw, long int* leniw)
+{
+  *lenrw = bm_len(A);
+  *leniw = 2;
+  return SUN_SUCCESS;
+}
+
+extern "C" SUNMatrix SUNMatrix_Mi355BlockDense(sunindextype nblocks, sunindextype m,
+                                               void* stream, SUNContext sunctx)
+{
+  if (nblocks <= 0 || m <= 0 || !sunctx) { return NULL; }
+  SUNMatrix A = SUNMatNewEmpty(sunctx);
+  if (!A) { return NULL; }
+  A->ops->getid     = bm_getid;
+  A->ops->clone     = bm_clone;
+  A->ops->destroy   = bm_destroy;
+  A->ops->zero      = bm_zero;
+  A->ops->copy      = bm_copy;
+  A->ops->scaleadd  = bm_scaleadd;
+  A->ops->scaleaddi = bm_scaleaddi;
+  A->ops->matvec    = bm_matvec;
+  A->ops->space     = bm_space;
+  BlockMat* c       = (BlockMat*)calloc(1, sizeof(BlockMat));
+  if (!c)
+  {
+    SUNMatFreeEmpty(A);
+    return NULL;
+  }
+  A->content = c;
+  c->nblocks = nblocks;
+  c->m       = m;
+  c->stream  = (hipStream_t)stream;
+  hipError_t e = hipMalloc((void**)&c->dev, sizeof(double) * (size_t)(nblocks * m * m));
+  if (e != hipSuccess)
+  {
+    hip_fail(sunctx, e, "hipMalloc(block matrix)", __LINE__);
+    c->dev = NULL;
+    bm_destroy(A);
+    return NULL;
+  }
+  return A;
+}
+
+extern "C" sunindextype SUNMatrix_Mi355BlockDense_NumBlocks(SUNMatrix A) { return BM(A)->nblocks; }
+extern "C" sunindextype SUNMatrix_Mi355BlockDense_BlockSize(SUNMatrix A) { return BM(A)->m; }
+extern "C" sunrealtype* SUNMatrix_Mi355BlockDense_DeviceData(SUNMatrix A) { return BM(A)->dev; }
+
+extern "C" sunrealtype* SUNMatrix_Mi355BlockDense_HostData(SUNMatrix A)
+{
+  if (!BM(A)->host)
+  {
+    BM(A)->host = (double*)calloc((size_t)bm_len(A), sizeof(double));
+  }
+  return BM(A)->host;
+}
+
+extern "C" SUNErrCode SUNMatrix_Mi355BlockDense_CopyToDevice(SUNMatrix A)
+{
+  double* h = SUNMatrix_Mi355BlockDense_HostData(A);
+  if (!h) { return SUN_ERR_MALLOC_FAIL; }
+  hipError_t e = hipMemcpyAsync(BM(A)->dev, h, sizeof(double) * bm_len(A),
+                                hipMemcpyHostToDevice, BM(A)->stream);
+  if (e == hipSuccess) { e = hipStreamSynchronize(BM(A)->stream); }
+  if (e != hipSuccess)
+  {
+    hip_fail(A->sunctx, e, "matrix copy to device", __LINE__);
+    return SUN_ERR_EXT_FAIL;
+  }
+  return SUN_SUCCESS;
+}
+
+extern "C" SUNErrCode SUNMatrix_Mi355BlockDense_CopyFromDevice(SUNMatrix A)
+{
+  double* h = SUNMatrix_Mi355BlockDense_HostData(A);
+  if (!h) { return SUN_ERR_MALLOC_FAIL; }
+  hipError_t e = hipMemcpyAsync(h, BM(A)->dev, sizeof(double) * bm_len(A),
+                                hipMemcpyDeviceToHost, BM(A)->stream);
+  if (e == hipSuccess) { e = hipStreamSynchronize(BM(A)->stream); }
+  if (e != hipSuccess)
+  {
+    hip_fail(A->sunctx, e, "matrix copy from device", __LINE__);
+    return SUN_ERR_EXT_FAIL;
+  }
+  return SUN_SUCCESS;
+}
+
+// ----------------------------------------------------------------------------
+// SUNLinearSolver
+// ----------------------------------------------------------------------------
+struct BlockLU
+{
+  sunindextype nblocks, m;
+  int precision;
+  int* piv;                  // device, nblocks*m
+  float* f32;                // device FP32 factors (FP32 mode)
+  unsigned long long* dinfo; // device
+  unsigned long long* hinfo; // pinned
+  sunindextype last_flag;
+  hipStream_t stream;
+};
+#define LU(S) ((BlockLU*)((S)->content))
+
+static SUNLinearSolver_Type lu_gettype(SUNLinearSolver S)
+{
+  (void)S;
+  return SUNLINEARSOLVER_DIRECT;
+}
+static SUNLinearSolver_ID lu_getid(SUNLinearSolver S)
+{
+  (void)S;
+  return SUNLINEARSOLVER_CUSTOM;
+}
+static SUNErrCode lu_initialize(SUNLinearSolver S)
+{
+  LU(S)->last_flag = SUN_SUCCESS;
+  return SUN_SUCCESS;
+}
+
+static int lu_setup(SUNLinearSolver S, SUNMatrix A)
+{
+  BlockLU* c = LU(S);
+  if (A->ops->getid != bm_getid || BM(A)->nblocks != c->nblocks || BM(A)->m != c->m)
+  {
+    return SUN_ERR_ARG_INCOMPATIBLE;
+  }
+  k_info_reset<<<1, 1, 0, c->stream>>>(c->dinfo);
+  bool ok;
+  if (c->precision == SUNAMD_LU_FP32)
+  {
+    ok = launch_getrf<float>((int)c->m, BM(A)->dev, c->f32, c->piv, (long)c->nblocks, c->dinfo,
+                             c->stream);
+  }
+  else
+  {
+    ok = launch_getrf<double>((int)c->m, BM(A)->dev, BM(A)->dev, c->piv, (long)c->nblocks,
+                              c->dinfo, c->stream);
+  }
+  if (!ok) { return SUN_ERR_ARG_OUTOFRANGE; }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+  {
+    e = hipMemcpyAsync(c->hinfo, c->dinfo, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                       c->stream);
+  }
+  if (e == hipSuccess) { e = hipStreamSynchronize(c->stream); }
+  if (e != hipSuccess)
+  {
+    hip_fail(S->sunctx, e, "block LU setup", __LINE__);
+    return SUN_ERR_EXT_FAIL;
+  }
+  if (*c->hinfo != ULLONG_MAX)
+  {
+    c->last_flag = (sunindextype)(*c->hinfo);
+    return SUNLS_LUFACT_FAIL;
+  }
+  c->last_flag = SUN_SUCCESS;
+  return SUN_SUCCESS;
+}
+
+static int lu_solve(SUNLinearSolver S, SUNMatrix A, N_Vector x, N_Vector b, sunrealtype tol)
+{
+  (void)tol;
+  BlockLU* c = LU(S);
+  double* xd = N_VGetDeviceArrayPointer(x);
+  double* bd = N_VGetDeviceArrayPointer(b);
+  if (!xd || !bd) { return SUN_ERR_ARG_CORRUPT; }
+  bool ok;
+  if (c->precision == SUNAMD_LU_FP32)
+  {
+    ok = launch_getrs<float>((int)c->m, c->f32, c->piv, bd, xd, (long)c->nblocks, c->stream);
+  }
+  else
+  {
+    ok = launch_getrs<double>((int)c->m, BM(A)->dev, c->piv, bd, xd, (long)c->nblocks,
+                              c->stream);
+  }
+  if (!ok) { return SUN_ERR_ARG_OUTOFRANGE; }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+  {
+    hip_fail(S->sunctx, e, "block LU solve", __LINE__);
+    return SUN_ERR_EXT_FAIL;
+  }
+  c->last_flag = SUN_SUCCESS;
+  return SUN_SUCCESS;
+}
+
+static sunindextype lu_lastflag(SUNLinearSolver S) { return LU(S)->last_flag; }
+
+static SUNErrCode lu_space(SUNLinearSolver S, long int* lenrw, long int* leniw)
+{
+  *lenrw = (LU(S)->precision == SUNAMD_LU_FP32) ? (long)(LU(S)->nblocks * LU(S)->m * LU(S)->m / 2) : 0;
+  *leniw = (long)(LU(S)->nblocks * LU(S)->m);
+  return SUN_SUCCESS;
+}
+
+static SUNErrCode lu_free(SUNLinearSolver S)
+{
+  if (!S) { return SUN_SUCCESS; }
+  if (S->content)
+  {
+    BlockLU* c = LU(S);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->piv) { (void)hipFree(c->piv); }
+    if (c->f32) { (void)hipFree(c->f32); }
+    if (c->dinfo) { (void)hipFree(c->dinfo); }
+    if (c->hinfo) { (void)hipHostFree(c->hinfo); }
+    free(c);
+    S->content = NULL;
+  }
+  SUNLinSolFreeEmpty(S);
+  return SUN_SUCCESS;
+}
+
+extern "C" SUNLinearSolver SUNLinSol_Mi355BlockLU(N_Vector y, SUNMatrix A, int precision,
+                                                  SUNContext sunctx)
+{
+  if (!A || A->ops->getid != bm_getid) { return NULL; }
+  if (precision != SUNAMD_LU_FP64 && precision != SUNAMD_LU_FP32) { return NULL; }
+  if (N_VGetLength(y) != BM(A)->nblocks * BM(A)->m) { return NULL; }
+  if (BM(A)->m > 64) { return NULL; }
+  SUNLinearSolver S = SUNLinSolNewEmpty(sunctx);
+  if (!S) { return NULL; }
+  S->ops->gettype    = lu_gettype;
+  S->ops->getid      = lu_getid;
+  S->ops->initialize = lu_initialize;
+  S->ops->setup      = lu_setup;
+  S->ops->solve      = lu_solve;
+  S->ops->lastflag   = lu_lastflag;
+  S->ops->space      = lu_space;
+  S->ops->free       = lu_free;
+  BlockLU* c         = (BlockLU*)calloc(1, sizeof(BlockLU));
+  if (!c)
+  {
+    SUNLinSolFreeEmpty(S);
+    return NULL;
+  }
+  S->content   = c;
+  c->nblocks   = BM(A)->nblocks;
+  c->m         = BM(A)->m;
+  c->precision = precision;
+  c->stream    = BM(A)->stream;
+  hipError_t e = hipMalloc((void**)&c->piv, sizeof(int) * (size_t)(c->nblocks * c->m));
+  if (e == hipSuccess && precision == SUNAMD_LU_FP32)
+  {
+    e = hipMalloc((void**)&c->f32, sizeof(float) * (size_t)(c->nblocks * c->m * c->m));
+  }
+  if (e == hipSuccess) { e = hipMalloc((void**)&c->dinfo, sizeof(unsigned long long)); }
+  if (e == hipSuccess)
+  {
+    e = hipHostMalloc((void**)&c->hinfo, sizeof(unsigned long long), hipHostMallocDefault);
+  }
+  if (e != hipSuccess)
+  {
+    hip_fail(sunctx, e, "block LU allocation", __LINE__);
+    lu_free(S);
+    return NULL;
+  }
+  return S;
+}
+
+extern "C" int* SUNLinSol_Mi355BlockLU_DevicePivots(SUNLinearSolver S) { return LU(S)->piv; }

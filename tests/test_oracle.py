@@ -1,0 +1,219 @@
+"""Pins of the CPU oracle (serial N_Vector, dense LU, BDF integrator,
+Newton) against the reference's own golden outputs and known answers.
+
+- cvRoberts_dns: every counter and step size of the published run
+  (examples/cvode/serial/cvRoberts_dns.out, cvRoberts_dns_stats.csv).
+- N_Vector known answers from test/unit_tests/nvector/test_nvector.c
+  (constant fills with closed-form results).
+- Dense LU: block-diagonal full-dense GETRF equals per-block GETRF bit for
+  bit; P*A = L*U; solve residual with pivoting forced as in
+  test/unit_tests/sunlinsol/dense/test_sunlinsol_dense.c:93-135.
+- LinearSum special cases reduce to the per-element formulas the HIP vector
+  uses (checked on random data against explicit numpy evaluation).
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from sundials_amd import _lib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PD = C.POINTER(C.c_double)
+
+
+def dp(a):
+    return a.ctypes.data_as(PD)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return _lib.load_oracle()
+
+
+@pytest.fixture(scope="module")
+def core():
+    return _lib.core()
+
+
+@pytest.fixture
+def ctx():
+    c = _lib.new_context()
+    yield c
+    _lib.free_context(c)
+
+
+# ----------------------------------------------------------------------------
+# BDF integrator pinned by the published cvRoberts_dns run
+# ----------------------------------------------------------------------------
+def test_roberts_matches_published_run(orc):
+    g = json.load(open(os.path.join(HERE, "golden", "cvRoberts_dns.json")))
+    normal = [r for r in g["returns"] if not r["root"]]
+    out = np.zeros(12 * 20)
+    assert orc.oracle_roberts_run(12, dp(out)) == 0
+    for k, ref in enumerate(normal):
+        row = out[20 * k: 20 * k + 20]
+        s = ref["stats"]
+        assert row[18] == 0  # CV_SUCCESS
+        assert row[0] == pytest.approx(ref["t"], rel=1e-12)
+        # printed with 7 significant digits
+        np.testing.assert_allclose(row[1:4], ref["y"], rtol=6e-7, atol=0)
+        # tn agrees to the 16 significant digits the CSV prints
+        assert row[4] == pytest.approx(s["Time"], rel=1e-15)
+        assert int(row[5]) == s["Steps"]
+        assert int(row[6]) == s["Error test fails"]
+        assert int(row[7]) == s["NLS step fails"]
+        assert int(row[13]) == s["RHS fn evals"]
+        assert int(row[14]) == s["NLS iters"]
+        assert int(row[15]) == s["NLS fails"]
+        assert int(row[16]) == s["LS setups"]
+        assert int(row[17]) == s["Jac fn evals"]
+        assert int(row[11]) == s["Last method order"]
+        assert int(row[12]) == s["Current method order"]
+        assert row[8] == pytest.approx(s["Initial step size"], rel=1e-15)
+        assert row[9] == pytest.approx(s["Last step size"], rel=1e-15)
+        assert row[10] == pytest.approx(s["Current step size"], rel=1e-15)
+    f = g["final"]
+    last = out[20 * 11: 20 * 12]
+    assert (int(last[5]), int(last[14]), int(last[16]), int(last[17]), int(last[6])) == (
+        f["Steps"], f["NLS iters"], f["LS setups"], f["Jac fn evals"], f["Error test fails"])
+
+
+# ----------------------------------------------------------------------------
+# serial vector known answers (test_nvector.c)
+# ----------------------------------------------------------------------------
+def _serial(orc, ctx, vals):
+    v = orc.N_VNew_Serial(len(vals), ctx)
+    C.memmove(_lib.core().N_VGetArrayPointer(v), dp(np.ascontiguousarray(vals, np.float64)),
+              8 * len(vals))
+    return v
+
+
+def _arr(core, v, n):
+    return np.ctypeslib.as_array(core.N_VGetArrayPointer(v), shape=(n,)).copy()
+
+
+@pytest.mark.parametrize("n", [1, 1000, 10000])
+def test_serial_known_answers(orc, core, ctx, n):
+    # Test_N_VWrmsNorm: x = -1/2, w = 1/2 -> 1/4   (test_nvector.c:1503-1536)
+    x = _serial(orc, ctx, np.full(n, -0.5))
+    w = _serial(orc, ctx, np.full(n, 0.5))
+    assert core.N_VWrmsNorm(x, w) == 0.25
+    # Test_N_VDotProdMulti: x = 2, Y = (1/2, -1/2, 1) -> N, -N, 2N  (:2613-2701)
+    x2 = _serial(orc, ctx, np.full(n, 2.0))
+    Y = [_serial(orc, ctx, np.full(n, v)) for v in (0.5, -0.5, 1.0)]
+    d = np.zeros(3)
+    Ya = (C.c_void_p * 3)(*Y)
+    for fused in (0, 1):
+        orc.N_VEnableFusedOps_Serial(x2, fused)
+        assert core.N_VDotProdMulti(3, x2, Ya, dp(d)) == 0
+        assert list(d) == [n, -n, 2 * n]
+    # Test_N_VMaxNorm / N_VMin / N_VL1Norm with one distinguished entry
+    z = _serial(orc, ctx, np.concatenate([np.full(n - 1, -0.5), [-2.0]]))
+    assert core.N_VMaxNorm(z) == 2.0
+    assert core.N_VMin(z) == -2.0
+    assert core.N_VL1Norm(z) == 0.5 * (n - 1) + 2.0
+    for v in [x, w, x2, z] + Y:
+        core.N_VDestroy(v)
+
+
+def test_serial_linearsum_special_cases(orc, core, ctx):
+    """Every LinearSum branch equals one of a*x+b*y, a*(x+y), a*(x-y)
+    evaluated element by element without fused multiply-add."""
+    rng = np.random.default_rng(5)
+    n = 257
+    xs, ys = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    cases = [(1, 1), (1, -1), (-1, 1), (-1, -1), (1, 0.3), (0.3, 1), (-1, 0.7), (0.7, -1),
+             (2.5, 2.5), (2.5, -2.5), (0.3, 0.7), (0.0, 0.0), (0.0, 2.0)]
+    for a, b in cases:
+        for alias in ("none", "y", "x"):
+            x = _serial(orc, ctx, xs)
+            y = _serial(orc, ctx, ys)
+            z = {"none": _serial(orc, ctx, np.zeros(n)), "y": y, "x": x}[alias]
+            core.N_VLinearSum(a, x, b, y, z)
+            got = _arr(core, z, n)
+            if a == b:
+                ref = a * (xs + ys)
+            elif a == -b:
+                ref = a * (xs - ys)
+            else:
+                ref = (a * xs) + (b * ys)
+            assert np.array_equal(got, ref), (a, b, alias)
+            for v in {id(x): x, id(y): y, id(z): z}.values():
+                core.N_VDestroy(v)
+
+
+# ----------------------------------------------------------------------------
+# dense LU
+# ----------------------------------------------------------------------------
+def _forced_pivot_matrix(rng, m):
+    """rand/cols + anti-identity, as in test_sunlinsol_dense.c:93-135"""
+    A = rng.uniform(0, 1, (m, m)) / m
+    A[np.arange(m), m - 1 - np.arange(m)] += 1.0
+    return A
+
+
+def _getrf(orc, A):
+    m = A.shape[0]
+    a = np.asfortranarray(A).ravel(order="F").copy()
+    p = np.zeros(m, np.int64)
+    info = orc.oracle_block_getrf(dp(a), m, p.ctypes.data_as(C.POINTER(C.c_int64)))
+    return a.reshape((m, m), order="F"), p, info
+
+
+@pytest.mark.parametrize("m", [1, 3, 8, 32, 64])
+def test_getrf_factorisation_invariant(orc, m):
+    rng = np.random.default_rng(m)
+    A = _forced_pivot_matrix(rng, m)
+    F, p, info = _getrf(orc, A)
+    assert info == 0
+    L = np.tril(F, -1) + np.eye(m)
+    U = np.triu(F)
+    PA = A.copy()
+    for k in range(m):  # apply the row swaps in order
+        PA[[k, p[k]]] = PA[[p[k], k]]
+    assert np.abs(PA - L @ U).max() <= 8 * m * np.finfo(float).eps * np.abs(A).max()
+    # solve residual, as test_sunlinsol.c:463-545 (b = A x_true)
+    xt = rng.uniform(-1, 1, m)
+    b = A @ xt
+    bb = b.copy()
+    orc.oracle_block_getrs(dp(np.asfortranarray(F).ravel(order="F").copy()), m,
+                           p.ctypes.data_as(C.POINTER(C.c_int64)), dp(bb))
+    np.testing.assert_allclose(bb, xt, rtol=0, atol=1e-10)
+
+
+def test_getrf_zero_pivot_reports_column(orc):
+    A = np.array([[1.0, 2.0, 3.0], [2.0, 4.0, 6.0], [0.0, 0.0, 0.0]]).T  # rank-deficient
+    A = np.array([[1.0, 2.0, 0.0], [1.0, 2.0, 0.0], [0.0, 0.0, 1.0]])
+    _, _, info = _getrf(orc, A)
+    assert info == 2  # second column has no nonzero pivot candidate
+
+
+def test_block_getrf_equals_full_dense_getrf(orc, core, ctx):
+    """Full-dense GETRF of a block-diagonal matrix is bit-identical to GETRF
+    applied block by block (the property the block oracle relies on)."""
+    rng = np.random.default_rng(11)
+    m, nb = 5, 7
+    n = m * nb
+    blocks = [_forced_pivot_matrix(rng, m) * rng.uniform(0.5, 2) for _ in range(nb)]
+    # tie-prone entries: duplicate magnitudes inside a block
+    blocks[3][:, 1] = blocks[3][0, 1]
+    full = np.zeros((n, n))
+    for b, B in enumerate(blocks):
+        full[b * m:(b + 1) * m, b * m:(b + 1) * m] = B
+    A = orc.SUNDenseMatrix(n, n, ctx)
+    C.memmove(orc.SUNDenseMatrix_Data(A), dp(full.ravel(order="F").copy()), 8 * n * n)
+    y = orc.N_VNew_Serial(n, ctx)
+    LS = orc.SUNLinSol_Dense(y, A, ctx)
+    assert core.SUNLinSolSetup(LS, A) == 0
+    Ffull = np.ctypeslib.as_array(orc.SUNDenseMatrix_Data(A), shape=(n * n,)).reshape(
+        (n, n), order="F")
+    for b, B in enumerate(blocks):
+        Fb, pb, info = _getrf(orc, B)
+        assert info == 0
+        assert np.array_equal(Ffull[b * m:(b + 1) * m, b * m:(b + 1) * m], Fb)
+    core.SUNLinSolFree(LS)
+    core.SUNMatDestroy(A)
+    core.N_VDestroy(y)
