@@ -50,6 +50,17 @@ int SUNAMD_MassActionSolveCoupled_Mi355(const SUNAMD_MassAction* p, double rtol,
                                         double atol, double tout, int lu_precision,
                                         double* yout, SUNAMD_SolveStats* stats);
 
+/* the same in three parts, so repeated integrations reuse the resident
+   problem, vectors, matrices and integrator: create (allocate + upload),
+   run (reset y to y0 on the device, re-initialise, integrate to tout),
+   free.  run may be called any number of times. */
+typedef struct SUNAMD_CoupledRun_ SUNAMD_CoupledRun;
+SUNAMD_CoupledRun* SUNAMD_CoupledCreate_Mi355(const SUNAMD_MassAction* p, double rtol,
+                                              double atol, int lu_precision);
+int SUNAMD_CoupledRun_Mi355(SUNAMD_CoupledRun* r, double tout, SUNAMD_SolveStats* stats);
+int SUNAMD_CoupledGetState_Mi355(SUNAMD_CoupledRun* r, double* yout);
+void SUNAMD_CoupledFree_Mi355(SUNAMD_CoupledRun* r);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,49 @@
+/* -----------------------------------------------------------------------------
+ * sunamd_profiler.h -- HIP-event timing of selected kernel families, and the
+ * N_Vector microbenchmark.
+ *
+ * The reference times with host wall clocks around whole functions
+ * (SUNProfiler, src/sundials/sundials_profiler.c:171-612) and needs
+ * synchronous launches to attribute GPU time.  Here, when enabled, the
+ * library records a HIP event pair around each launch of a timed kernel
+ * family on the stream it is launched on; durations are read back when
+ * queried, so timing does not serialise the stream.
+ * ---------------------------------------------------------------------------*/
+#ifndef SUNAMD_PROFILER_H
+#define SUNAMD_PROFILER_H
+
+#include "sundials_core.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* timed kernel families */
+#define SUNAMD_K_GETRS     0 /* batched block LU solve */
+#define SUNAMD_K_GETRF     1 /* batched block LU factorisation */
+#define SUNAMD_K_LINSUM    2 /* N_VLinearSum */
+#define SUNAMD_K_WRMS      3 /* N_VWrmsNorm */
+#define SUNAMD_K_DOTMULTI  4 /* N_VDotProdMulti */
+#define SUNAMD_K_RHS       5 /* mass-action right-hand side */
+#define SUNAMD_K_BDF_BATCH 6 /* batched per-system BDF */
+#define SUNAMD_K_COUNT     8
+
+void SUNAMD_ProfilerEnable(int on);
+void SUNAMD_ProfilerReset(void);
+/* total milliseconds and launch count recorded for a family (synchronises
+   on the recorded events) */
+int SUNAMD_ProfilerGet(int family, double* total_ms, long int* count);
+
+/* N_Vector microbenchmark on n FP64 values: x, y ~ U(-1,1), w ~ U(1e-3,1),
+   generated on the device.  Average milliseconds per call over `reps`
+   timed calls (after one warm-up), measured with HIP events on the vector
+   stream: out_ms[0] LinearSum z = a x + b y, [1] WrmsNorm(x, w),
+   [2] DotProdMulti(nvec_multi, x, Y), [3] LinearSum y = x + y (in place),
+   [4] DotProd(x, y). */
+int SUNAMD_BenchVectorOps(sunindextype n, int nvec_multi, int reps, double* out_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

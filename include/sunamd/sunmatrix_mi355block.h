@@ -49,6 +49,8 @@ SUNLinearSolver SUNLinSol_Mi355BlockLU(N_Vector y, SUNMatrix A, int precision,
                                        SUNContext sunctx);
 /* device pivot array (int32, nblocks*m, pivot row of step k of block b at b*m+k) */
 int* SUNLinSol_Mi355BlockLU_DevicePivots(SUNLinearSolver S);
+/* copy the pivots to a host array of nblocks*m ints (synchronous) */
+SUNErrCode SUNLinSol_Mi355BlockLU_GetPivots(SUNLinearSolver S, int* host_out);
 
 #ifdef __cplusplus
 }

@@ -165,6 +165,15 @@ def hip():
     _sig(lib, "SUNAMD_MassActionJacDevArrays", C.c_int, [c_void_p, c_void_p, c_void_p])
     _sig(lib, "SUNAMD_MassActionSolveCoupled_Mi355", C.c_int,
          [P(MassAction), c_real, c_real, c_real, C.c_int, P(c_real), P(SolveStats)])
+    _sig(lib, "SUNAMD_CoupledCreate_Mi355", c_void_p, [P(MassAction), c_real, c_real, C.c_int])
+    _sig(lib, "SUNAMD_CoupledRun_Mi355", C.c_int, [c_void_p, c_real, P(SolveStats)])
+    _sig(lib, "SUNAMD_CoupledGetState_Mi355", C.c_int, [c_void_p, P(c_real)])
+    _sig(lib, "SUNAMD_CoupledFree_Mi355", None, [c_void_p])
+    _sig(lib, "SUNLinSol_Mi355BlockLU_GetPivots", C.c_int, [c_void_p, P(C.c_int)])
+    _sig(lib, "SUNAMD_ProfilerEnable", None, [C.c_int])
+    _sig(lib, "SUNAMD_ProfilerReset", None, [])
+    _sig(lib, "SUNAMD_ProfilerGet", C.c_int, [C.c_int, P(C.c_double), P(C.c_long)])
+    _sig(lib, "SUNAMD_BenchVectorOps", C.c_int, [c_index, C.c_int, C.c_int, P(c_real)])
     _hip = lib
     return lib
 

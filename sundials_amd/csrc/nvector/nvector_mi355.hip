@@ -29,7 +29,9 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "../core/profiler_internal.h"
 #include "sunamd/nvector_mi355.h"
+#include "sunamd/sunamd_profiler.h"
 
 #define NV_MAX_MULTI 32      // vectors per fused call handled in one kernel
 #define NV_BLOCK 256         // threads per workgroup
@@ -648,6 +650,7 @@ static void mi_linearsum(sunrealtype a, N_Vector x, sunrealtype b, N_Vector y, N
   if (n == 0) { return; }
   Mi355Exec* ex = EXEC(z);
   int g         = stream_grid(ex, n >> 1);
+  sunamd_prof_begin(SUNAMD_K_LINSUM, ex->stream);
   if (a == b)
   {
     k_linearsum<LS_SCALESUM><<<g, NV_BLOCK, 0, ex->stream>>>(a, DDATA(x), b, DDATA(y), DDATA(z), n);
@@ -660,6 +663,7 @@ static void mi_linearsum(sunrealtype a, N_Vector x, sunrealtype b, N_Vector y, N
   {
     k_linearsum<LS_GENERIC><<<g, NV_BLOCK, 0, ex->stream>>>(a, DDATA(x), b, DDATA(y), DDATA(z), n);
   }
+  sunamd_prof_end(SUNAMD_K_LINSUM, ex->stream);
   LAUNCHCHK(z);
 }
 
@@ -709,9 +713,12 @@ static double mi_reduce(N_Vector x, N_Vector y, N_Vector id, double identity)
   Mi355Exec* ex = EXEC(x);
   if (n == 0) { return identity; }
   int g = red_grid(ex, n >> 1);
+  const int fam = (K == RK_WSQR) ? SUNAMD_K_WRMS : -1;
+  if (fam >= 0) { sunamd_prof_begin(fam, ex->stream); }
   k_reduce<K><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), y ? DDATA(y) : DDATA(x),
                                               id ? DDATA(id) : DDATA(x), n, identity,
                                               ex->d_partials, ex->d_ticket, ex->h_result);
+  if (fam >= 0) { sunamd_prof_end(fam, ex->stream); }
   LAUNCHCHK(x);
   HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
   return ex->h_result[0];
@@ -797,6 +804,7 @@ static SUNErrCode mi_dotprodmulti(int nvec, N_Vector x, N_Vector* Y, sunrealtype
     for (int k = 0; k < nv; k++) { P.p[k] = DDATA(Y[base + k]); }
     int g        = red_grid(ex, n >> 1);
     double* slot = ex->h_result + (base - group);
+    sunamd_prof_begin(SUNAMD_K_DOTMULTI, ex->stream);
     switch (nv)
     {
 #define DM_CASE(NVV)                                                                       \
@@ -807,6 +815,7 @@ static SUNErrCode mi_dotprodmulti(int nvec, N_Vector x, N_Vector* Y, sunrealtype
       NV_CASES(DM_CASE)
 #undef DM_CASE
     }
+    sunamd_prof_end(SUNAMD_K_DOTMULTI, ex->stream);
     LAUNCHCHK(x);
     int filled = base + nv - group;
     if (filled == NV_MAX_MULTI || base + nv == nvec)

@@ -19,7 +19,9 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "../core/profiler_internal.h"
 #include "sunamd/cvode_sunamd.h"
+#include "sunamd/sunamd_profiler.h"
 #include "sunamd/massaction_mi355.h"
 #include "sunamd/nvector_mi355.h"
 #include "sunamd/sunmatrix_mi355block.h"
@@ -41,26 +43,29 @@ static int hipfail(hipError_t e, const char* what)
   return -1;
 }
 
-#define RHS_CPB 64
+// cells per workgroup (one lane per cell): 64, or 32 above 32 species so the
+// two LDS tiles stay within 64 KiB
+static inline int rhs_cpb(int m) { return m <= 32 ? 64 : 32; }
 
-__global__ void __launch_bounds__(RHS_CPB) k_ma_rhs(const double* __restrict__ y,
-                                                     double* __restrict__ f,
-                                                     const int* __restrict__ idx,
-                                                     const double* __restrict__ kf,
-                                                     const double* __restrict__ kb, int m,
-                                                     int R, long ncells)
+__global__ void __launch_bounds__(64) k_ma_rhs(const double* __restrict__ y,
+                                               double* __restrict__ f,
+                                               const int* __restrict__ idx,
+                                               const double* __restrict__ kf,
+                                               const double* __restrict__ kb, int m, int R,
+                                               long ncells)
 {
   extern __shared__ double sm[];
+  const int cpb    = blockDim.x;
   const int ld     = m + 1;
   double* ys       = sm;
-  double* fs       = sm + RHS_CPB * ld;
-  const long cell0 = (long)blockIdx.x * RHS_CPB;
+  double* fs       = sm + cpb * ld;
+  const long cell0 = (long)blockIdx.x * cpb;
   const long rem   = ncells - cell0;
-  const int ncb    = rem < RHS_CPB ? (int)rem : RHS_CPB;
+  const int ncb    = rem < cpb ? (int)rem : cpb;
   const double* yb = y + cell0 * m;
-  for (int e = threadIdx.x; e < ncb * m; e += RHS_CPB)
+  for (int e = threadIdx.x; e < ncb * m; e += cpb)
   {
-    int c            = e / m;
+    int c                  = e / m;
     ys[c * ld + e - c * m] = yb[e];
   }
   __syncthreads();
@@ -85,7 +90,7 @@ __global__ void __launch_bounds__(RHS_CPB) k_ma_rhs(const double* __restrict__ y
   }
   __syncthreads();
   double* fb = f + cell0 * m;
-  for (int e = threadIdx.x; e < ncb * m; e += RHS_CPB)
+  for (int e = threadIdx.x; e < ncb * m; e += cpb)
   {
     int cc = e / m;
     fb[e]  = fs[cc * ld + e - cc * m];
@@ -198,10 +203,13 @@ extern "C" void SUNAMD_MassActionDevFree(SUNAMD_MassActionDev* d)
 extern "C" int SUNAMD_MassActionRhsDevArrays(SUNAMD_MassActionDev* d, const double* dy,
                                              double* df)
 {
-  long g      = (d->ncells + RHS_CPB - 1) / RHS_CPB;
-  size_t shm  = sizeof(double) * 2 * RHS_CPB * (size_t)(d->m + 1);
-  k_ma_rhs<<<(unsigned)g, RHS_CPB, shm, d->stream>>>(dy, df, d->idx, d->kf, d->kb, d->m, d->R,
-                                                     d->ncells);
+  const int cpb = rhs_cpb(d->m);
+  long g        = (d->ncells + cpb - 1) / cpb;
+  size_t shm    = sizeof(double) * 2 * cpb * (size_t)(d->m + 1);
+  sunamd_prof_begin(SUNAMD_K_RHS, d->stream);
+  k_ma_rhs<<<(unsigned)g, cpb, shm, d->stream>>>(dy, df, d->idx, d->kf, d->kb, d->m, d->R,
+                                                 d->ncells);
+  sunamd_prof_end(SUNAMD_K_RHS, d->stream);
   hipError_t e = hipGetLastError();
   return (e == hipSuccess) ? 0 : hipfail(e, "rhs launch");
 }
@@ -242,77 +250,117 @@ extern "C" int SUNAMD_MassActionJacDev(sunrealtype t, N_Vector y, N_Vector fy, S
                                        SUNMatrix_Mi355BlockDense_DeviceData(J));
 }
 
+struct SUNAMD_CoupledRun_
+{
+  SUNContext ctx;
+  sunindextype N;
+  N_Vector y, y0;
+  SUNAMD_MassActionDev* dev;
+  SUNMatrix A;
+  SUNLinearSolver LS;
+  void* cv;
+};
+
+extern "C" void SUNAMD_CoupledFree_Mi355(SUNAMD_CoupledRun* r)
+{
+  if (!r) { return; }
+  if (r->cv) { CVodeFree(&r->cv); }
+  if (r->LS) { SUNLinSolFree(r->LS); }
+  if (r->A) { SUNMatDestroy(r->A); }
+  if (r->dev) { SUNAMD_MassActionDevFree(r->dev); }
+  if (r->y0) { N_VDestroy(r->y0); }
+  if (r->y) { N_VDestroy(r->y); }
+  if (r->ctx) { SUNContext_Free(&r->ctx); }
+  free(r);
+}
+
+extern "C" SUNAMD_CoupledRun* SUNAMD_CoupledCreate_Mi355(const SUNAMD_MassAction* p,
+                                                         double rtol, double atol,
+                                                         int lu_precision)
+{
+  SUNAMD_CoupledRun* r = (SUNAMD_CoupledRun*)calloc(1, sizeof(SUNAMD_CoupledRun));
+  if (!r) { return NULL; }
+  bool ok = SUNContext_Create(SUN_COMM_NULL, &r->ctx) == 0;
+  r->N    = (sunindextype)p->ncells * p->m;
+  if (ok) { ok = (r->y = N_VNew_Mi355(r->N, r->ctx)) != NULL; }
+  if (ok) { ok = (r->y0 = N_VClone(r->y)) != NULL; }
+  if (ok) { ok = N_VCopyFromHostArray_Mi355(r->y0, p->y0) == 0; }
+  if (ok) { N_VScale(1.0, r->y0, r->y); }
+  if (ok) { ok = (r->dev = SUNAMD_MassActionToDevice(p, N_VGetStream_Mi355(r->y))) != NULL; }
+  if (ok)
+  {
+    ok = (r->A = SUNMatrix_Mi355BlockDense(p->ncells, p->m, N_VGetStream_Mi355(r->y),
+                                           r->ctx)) != NULL;
+  }
+  if (ok) { ok = (r->LS = SUNLinSol_Mi355BlockLU(r->y, r->A, lu_precision, r->ctx)) != NULL; }
+  if (ok) { ok = (r->cv = CVodeCreate(CV_BDF, r->ctx)) != NULL; }
+  if (ok) { ok = CVodeInit(r->cv, SUNAMD_MassActionRhsDev, 0.0, r->y) == 0; }
+  if (ok) { ok = CVodeSStolerances(r->cv, rtol, atol) == 0; }
+  if (ok) { ok = CVodeSetUserData(r->cv, r->dev) == 0; }
+  if (ok) { ok = CVodeSetLinearSolver(r->cv, r->LS, r->A) == 0; }
+  if (ok) { ok = CVodeSetJacFn(r->cv, SUNAMD_MassActionJacDev) == 0; }
+  if (ok) { ok = CVodeSetMaxNumSteps(r->cv, 1000000) == 0; }
+  if (ok) { ok = hipDeviceSynchronize() == hipSuccess; }
+  if (!ok)
+  {
+    SUNAMD_CoupledFree_Mi355(r);
+    return NULL;
+  }
+  return r;
+}
+
+extern "C" int SUNAMD_CoupledRun_Mi355(SUNAMD_CoupledRun* r, double tout, SUNAMD_SolveStats* st)
+{
+  sunrealtype t = 0.0;
+  if (hipDeviceSynchronize() != hipSuccess) { return -1; }
+  long l0 = N_VGetNumLaunches_Mi355(r->y);
+  auto t0 = std::chrono::steady_clock::now();
+  N_VScale(1.0, r->y0, r->y);
+  int flag = CVodeReInit(r->cv, 0.0, r->y);
+  if (flag == 0) { flag = CVode(r->cv, tout, r->y, &t, CV_NORMAL); }
+  if (hipDeviceSynchronize() != hipSuccess) { return -1; }
+  auto t1 = std::chrono::steady_clock::now();
+  if (st)
+  {
+    long counts[8];
+    int orders[2];
+    double steps[4];
+    CVodeGetAllStats(r->cv, counts, orders, steps);
+    st->nst      = counts[0];
+    st->nfe      = counts[1];
+    st->nsetups  = counts[2];
+    st->netf     = counts[3];
+    st->nni      = counts[4];
+    st->nnf      = counts[5];
+    st->ncfn     = counts[6];
+    st->nje      = counts[7];
+    st->qlast    = orders[0];
+    st->qcur     = orders[1];
+    st->hinused  = steps[0];
+    st->hlast    = steps[1];
+    st->hcur     = steps[2];
+    st->tcur     = steps[3];
+    st->wall_s   = std::chrono::duration<double>(t1 - t0).count();
+    st->launches = N_VGetNumLaunches_Mi355(r->y) - l0;
+    st->flag     = flag;
+  }
+  return (flag < 0) ? flag : 0;
+}
+
+extern "C" int SUNAMD_CoupledGetState_Mi355(SUNAMD_CoupledRun* r, double* yout)
+{
+  return N_VCopyToHostArray_Mi355(r->y, yout) == 0 ? 0 : -1;
+}
+
 extern "C" int SUNAMD_MassActionSolveCoupled_Mi355(const SUNAMD_MassAction* p, double rtol,
                                                    double atol, double tout,
                                                    int lu_precision, double* yout,
                                                    SUNAMD_SolveStats* st)
 {
-  SUNContext ctx = NULL;
-  if (SUNContext_Create(SUN_COMM_NULL, &ctx)) { return -1; }
-  const sunindextype N = (sunindextype)p->ncells * p->m;
-  int rc               = -1;
-  N_Vector y           = N_VNew_Mi355(N, ctx);
-  SUNAMD_MassActionDev* dev = NULL;
-  SUNMatrix A          = NULL;
-  SUNLinearSolver LS   = NULL;
-  void* cv             = NULL;
-  sunrealtype t        = 0.0;
-  if (!y) { goto done; }
-  if (N_VCopyFromHostArray_Mi355(y, p->y0)) { goto done; }
-  dev = SUNAMD_MassActionToDevice(p, N_VGetStream_Mi355(y));
-  if (!dev) { goto done; }
-  A = SUNMatrix_Mi355BlockDense(p->ncells, p->m, N_VGetStream_Mi355(y), ctx);
-  if (!A) { goto done; }
-  LS = SUNLinSol_Mi355BlockLU(y, A, lu_precision, ctx);
-  if (!LS) { goto done; }
-  cv = CVodeCreate(CV_BDF, ctx);
-  if (!cv) { goto done; }
-  if (CVodeInit(cv, SUNAMD_MassActionRhsDev, 0.0, y)) { goto done; }
-  if (CVodeSStolerances(cv, rtol, atol)) { goto done; }
-  if (CVodeSetUserData(cv, dev)) { goto done; }
-  if (CVodeSetLinearSolver(cv, LS, A)) { goto done; }
-  if (CVodeSetJacFn(cv, SUNAMD_MassActionJacDev)) { goto done; }
-  if (CVodeSetMaxNumSteps(cv, 1000000)) { goto done; }
-  {
-    if (hipDeviceSynchronize() != hipSuccess) { goto done; }
-    long l0 = N_VGetNumLaunches_Mi355(y);
-    auto t0 = std::chrono::steady_clock::now();
-    int flag = CVode(cv, tout, y, &t, CV_NORMAL);
-    if (hipDeviceSynchronize() != hipSuccess) { goto done; }
-    auto t1 = std::chrono::steady_clock::now();
-    if (st)
-    {
-      long counts[8];
-      int orders[2];
-      double steps[4];
-      CVodeGetAllStats(cv, counts, orders, steps);
-      st->nst      = counts[0];
-      st->nfe      = counts[1];
-      st->nsetups  = counts[2];
-      st->netf     = counts[3];
-      st->nni      = counts[4];
-      st->nnf      = counts[5];
-      st->ncfn     = counts[6];
-      st->nje      = counts[7];
-      st->qlast    = orders[0];
-      st->qcur     = orders[1];
-      st->hinused  = steps[0];
-      st->hlast    = steps[1];
-      st->hcur     = steps[2];
-      st->tcur     = steps[3];
-      st->wall_s   = std::chrono::duration<double>(t1 - t0).count();
-      st->launches = N_VGetNumLaunches_Mi355(y) - l0;
-      st->flag     = flag;
-    }
-    if (yout && N_VCopyToHostArray_Mi355(y, yout)) { goto done; }
-    rc = (flag < 0) ? flag : 0;
-  }
-done:
-  if (cv) { CVodeFree(&cv); }
-  if (LS) { SUNLinSolFree(LS); }
-  if (A) { SUNMatDestroy(A); }
-  if (dev) { SUNAMD_MassActionDevFree(dev); }
-  if (y) { N_VDestroy(y); }
-  SUNContext_Free(&ctx);
+  SUNAMD_CoupledRun* r = SUNAMD_CoupledCreate_Mi355(p, rtol, atol, lu_precision);
+  if (!r) { return -1; }
+  int rc = SUNAMD_CoupledRun_Mi355(r, tout, st);
+  if (rc == 0 && yout) { rc = SUNAMD_CoupledGetState_Mi355(r, yout); }
+  SUNAMD_CoupledFree_Mi355(r);
   return rc;
 }

@@ -36,7 +36,9 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "../core/profiler_internal.h"
 #include "sunamd/nvector_mi355.h"
+#include "sunamd/sunamd_profiler.h"
 #include "sunamd/sunmatrix_mi355block.h"
 
 #define EW_BLOCK 256
@@ -647,6 +649,7 @@ static int lu_setup(SUNLinearSolver S, SUNMatrix A)
   }
   k_info_reset<<<1, 1, 0, c->stream>>>(c->dinfo);
   bool ok;
+  sunamd_prof_begin(SUNAMD_K_GETRF, c->stream);
   if (c->precision == SUNAMD_LU_FP32)
   {
     ok = launch_getrf<float>((int)c->m, BM(A)->dev, c->f32, c->piv, (long)c->nblocks, c->dinfo,
@@ -657,6 +660,7 @@ static int lu_setup(SUNLinearSolver S, SUNMatrix A)
     ok = launch_getrf<double>((int)c->m, BM(A)->dev, BM(A)->dev, c->piv, (long)c->nblocks,
                               c->dinfo, c->stream);
   }
+  sunamd_prof_end(SUNAMD_K_GETRF, c->stream);
   if (!ok) { return SUN_ERR_ARG_OUTOFRANGE; }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess)
@@ -687,6 +691,7 @@ static int lu_solve(SUNLinearSolver S, SUNMatrix A, N_Vector x, N_Vector b, sunr
   double* bd = N_VGetDeviceArrayPointer(b);
   if (!xd || !bd) { return SUN_ERR_ARG_CORRUPT; }
   bool ok;
+  sunamd_prof_begin(SUNAMD_K_GETRS, c->stream);
   if (c->precision == SUNAMD_LU_FP32)
   {
     ok = launch_getrs<float>((int)c->m, c->f32, c->piv, bd, xd, (long)c->nblocks, c->stream);
@@ -696,6 +701,7 @@ static int lu_solve(SUNLinearSolver S, SUNMatrix A, N_Vector x, N_Vector b, sunr
     ok = launch_getrs<double>((int)c->m, BM(A)->dev, c->piv, bd, xd, (long)c->nblocks,
                               c->stream);
   }
+  sunamd_prof_end(SUNAMD_K_GETRS, c->stream);
   if (!ok) { return SUN_ERR_ARG_OUTOFRANGE; }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
@@ -782,3 +788,17 @@ extern "C" SUNLinearSolver SUNLinSol_Mi355BlockLU(N_Vector y, SUNMatrix A, int p
 }
 
 extern "C" int* SUNLinSol_Mi355BlockLU_DevicePivots(SUNLinearSolver S) { return LU(S)->piv; }
+
+extern "C" SUNErrCode SUNLinSol_Mi355BlockLU_GetPivots(SUNLinearSolver S, int* host_out)
+{
+  BlockLU* c   = LU(S);
+  hipError_t e = hipMemcpyAsync(host_out, c->piv, sizeof(int) * (size_t)(c->nblocks * c->m),
+                                hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) { e = hipStreamSynchronize(c->stream); }
+  if (e != hipSuccess)
+  {
+    hip_fail(S->sunctx, e, "pivot copy", __LINE__);
+    return SUN_ERR_EXT_FAIL;
+  }
+  return SUN_SUCCESS;
+}

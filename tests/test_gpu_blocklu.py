@@ -1,0 +1,193 @@
+"""Parity of the block-diagonal SUNMatrix and batched LU against the oracle.
+
+FP64: factors, pivots and solutions are bit-identical to the reference
+GETRF / GETRS applied block by block (oracle_block_getrf / _getrs), on
+matrices with forced pivoting (rand/cols + anti-identity,
+test_sunlinsol_dense.c:93-135) and with tied pivot candidates.  FP32: the
+solve residual |A x - b| is at single-precision level.  Matrix ops
+(Zero, Copy, ScaleAddI, ScaleAdd, Matvec) are bit-identical to the dense
+matrix's element formulas.  A zero pivot makes setup return
+SUNLS_LUFACT_FAIL with LastFlag = first failing global column + 1.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from sundials_amd import _lib
+from helpers import dp, gpu_get, gpu_vec
+
+pytestmark = pytest.mark.gpu
+
+SUNLS_LUFACT_FAIL = 808
+
+
+@pytest.fixture(scope="module")
+def libs():
+    import torch
+
+    assert torch.cuda.is_available()
+    return _lib.core(), _lib.hip(), _lib.load_oracle()
+
+
+@pytest.fixture
+def ctx():
+    c = _lib.new_context()
+    yield c
+    _lib.free_context(c)
+
+
+def _blocks(rng, nb, m, ties=True):
+    A = rng.uniform(0, 1, (nb, m, m)) / m
+    A[:, np.arange(m), m - 1 - np.arange(m)] += 1.0
+    A *= rng.uniform(0.5, 2.0, (nb, 1, 1))
+    A[::3] *= -1.0
+    if ties and m > 2:
+        A[1::4, :, 0] = A[1::4, 0:1, 0]          # whole first column tied
+        A[2::5, 1:, 1] = -A[2::5, 1:2, 1]       # ties of opposite sign
+    return A
+
+
+def _to_blockmajor(A):
+    # (nb, m, m) row-major numpy -> block-contiguous column-major
+    return np.ascontiguousarray(np.transpose(A, (0, 2, 1))).ravel()
+
+
+def _from_blockmajor(a, nb, m):
+    return np.transpose(a.reshape(nb, m, m), (0, 2, 1))
+
+
+def _gpu_matrix(hip, ctx, y, A):
+    nb, m, _ = A.shape
+    M = hip.SUNMatrix_Mi355BlockDense(nb, m, hip.N_VGetStream_Mi355(y), ctx)
+    h = hip.SUNMatrix_Mi355BlockDense_HostData(M)
+    flat = _to_blockmajor(A)
+    C.memmove(h, dp(flat), 8 * flat.size)
+    assert hip.SUNMatrix_Mi355BlockDense_CopyToDevice(M) == 0
+    return M
+
+
+def _gpu_matrix_values(hip, M, nb, m):
+    assert hip.SUNMatrix_Mi355BlockDense_CopyFromDevice(M) == 0
+    h = hip.SUNMatrix_Mi355BlockDense_HostData(M)
+    return np.ctypeslib.as_array(h, shape=(nb * m * m,)).copy()
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 7, 8, 11, 16, 24, 32, 48, 64])
+@pytest.mark.parametrize("nb", [1, 37])
+def test_fp64_factor_and_solve_bitwise(libs, ctx, m, nb):
+    core, hip, orc = libs
+    rng = np.random.default_rng(1000 * m + nb)
+    A = _blocks(rng, nb, m)
+    n = nb * m
+    b = rng.uniform(-1, 1, n)
+    y = gpu_vec(ctx, np.zeros(n))
+    M = _gpu_matrix(hip, ctx, y, A)
+    LS = hip.SUNLinSol_Mi355BlockLU(y, M, 0, ctx)
+    assert LS
+    assert core.SUNLinSolInitialize(LS) == 0
+    assert core.SUNLinSolSetup(LS, M) == 0
+    F = _gpu_matrix_values(hip, M, nb, m)
+    piv = np.zeros(n, np.int32)
+    assert hip.SUNLinSol_Mi355BlockLU_GetPivots(LS, piv.ctypes.data_as(C.POINTER(C.c_int))) == 0
+    gb = gpu_vec(ctx, b)
+    gx = gpu_vec(ctx, np.zeros(n))
+    assert core.SUNLinSolSolve(LS, M, gx, gb, 0.0) == 0
+    x = gpu_get(gx)
+    # in-place solve (x == b) gives the same answer
+    assert core.SUNLinSolSolve(LS, M, gb, gb, 0.0) == 0
+    assert np.array_equal(gpu_get(gb), x)
+    for k in range(nb):
+        a = _to_blockmajor(A[k:k + 1]).copy()
+        p = np.zeros(m, np.int64)
+        assert orc.oracle_block_getrf(dp(a), m, p.ctypes.data_as(C.POINTER(C.c_int64))) == 0
+        assert np.array_equal(F[k * m * m:(k + 1) * m * m], a), ("factor", k)
+        assert np.array_equal(piv[k * m:(k + 1) * m], p), ("pivots", k)
+        bb = b[k * m:(k + 1) * m].copy()
+        orc.oracle_block_getrs(dp(a), m, p.ctypes.data_as(C.POINTER(C.c_int64)), dp(bb))
+        assert np.array_equal(x[k * m:(k + 1) * m], bb), ("solve", k)
+    core.SUNLinSolFree(LS)
+    core.SUNMatDestroy(M)
+    for v in (y, gb, gx):
+        core.N_VDestroy(v)
+
+
+@pytest.mark.parametrize("m", [3, 8, 16, 32, 48, 64])
+def test_fp32_solve_residual(libs, ctx, m):
+    core, hip, orc = libs
+    nb = 129
+    rng = np.random.default_rng(m)
+    A = _blocks(rng, nb, m, ties=False)
+    n = nb * m
+    xt = rng.uniform(-1, 1, n)
+    b = np.einsum("kij,kj->ki", A, xt.reshape(nb, m)).ravel()
+    y = gpu_vec(ctx, np.zeros(n))
+    M = _gpu_matrix(hip, ctx, y, A)
+    LS = hip.SUNLinSol_Mi355BlockLU(y, M, 1, ctx)
+    assert core.SUNLinSolSetup(LS, M) == 0
+    # FP32 mode leaves the FP64 matrix untouched
+    assert np.array_equal(_gpu_matrix_values(hip, M, nb, m), _to_blockmajor(A))
+    gb, gx = gpu_vec(ctx, b), gpu_vec(ctx, np.zeros(n))
+    assert core.SUNLinSolSolve(LS, M, gx, gb, 0.0) == 0
+    x = gpu_get(gx)
+    err = np.abs(x - xt).max()
+    assert err < 2e-4 * m, err
+    core.SUNLinSolFree(LS)
+    core.SUNMatDestroy(M)
+    for v in (y, gb, gx):
+        core.N_VDestroy(v)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_zero_pivot_reported(libs, ctx, prec):
+    core, hip, orc = libs
+    nb, m = 6, 4
+    rng = np.random.default_rng(3)
+    A = _blocks(rng, nb, m, ties=False)
+    A[4, :, 2] = 0.0          # block 4, column 2 singular
+    A[5, :, 1] = 0.0          # block 5, column 1 singular
+    y = gpu_vec(ctx, np.zeros(nb * m))
+    M = _gpu_matrix(hip, ctx, y, A)
+    LS = hip.SUNLinSol_Mi355BlockLU(y, M, prec, ctx)
+    assert core.SUNLinSolSetup(LS, M) == SUNLS_LUFACT_FAIL
+    # first zero pivot in the global column order: block 4, column 2
+    assert core.SUNLinSolLastFlag(LS) == 4 * m + 2 + 1
+    core.SUNLinSolFree(LS)
+    core.SUNMatDestroy(M)
+    core.N_VDestroy(y)
+
+
+def test_matrix_ops_bitwise(libs, ctx):
+    core, hip, orc = libs
+    nb, m = 33, 5
+    rng = np.random.default_rng(8)
+    A = rng.uniform(-1, 1, (nb, m, m))
+    B = rng.uniform(-1, 1, (nb, m, m))
+    y = gpu_vec(ctx, np.zeros(nb * m))
+    MA = _gpu_matrix(hip, ctx, y, A)
+    MB = _gpu_matrix(hip, ctx, y, B)
+    c = -0.3125
+    assert core.SUNMatScaleAddI(c, MA) == 0
+    ref = _to_blockmajor(A) * c
+    diag = (np.arange(nb * m * m) % (m * m)) % (m + 1) == 0
+    ref[diag] = ref[diag] + 1.0
+    assert np.array_equal(_gpu_matrix_values(hip, MA, nb, m), ref)
+    assert core.SUNMatScaleAdd(2.5, MA, MB) == 0
+    ref = 2.5 * ref + _to_blockmajor(B)
+    assert np.array_equal(_gpu_matrix_values(hip, MA, nb, m), ref)
+    assert core.SUNMatCopy(MB, MA) == 0
+    assert np.array_equal(_gpu_matrix_values(hip, MA, nb, m), _to_blockmajor(B))
+    xs = rng.uniform(-1, 1, nb * m)
+    gx, gy = gpu_vec(ctx, xs), gpu_vec(ctx, np.zeros(nb * m))
+    assert core.SUNMatMatvec(MA, gx, gy) == 0
+    ref = np.zeros(nb * m)
+    for k in range(nb):  # column order of the dense matvec
+        for j in range(m):
+            ref[k * m:(k + 1) * m] = ref[k * m:(k + 1) * m] + B[k, :, j] * xs[k * m + j]
+    assert np.array_equal(gpu_get(gy), ref)
+    assert core.SUNMatZero(MA) == 0
+    assert not _gpu_matrix_values(hip, MA, nb, m).any()
+    for M in (MA, MB):
+        core.SUNMatDestroy(M)
+    for v in (y, gx, gy):
+        core.N_VDestroy(v)

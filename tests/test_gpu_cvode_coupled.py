@@ -1,0 +1,145 @@
+"""Coupled BDF integration on the HIP drop-ins against the CPU oracle.
+
+The same integrator code drives both runs; only the N_Vector, the block
+matrix / LU and the problem kernels differ (HIP vs serial + dense LU + C).
+Parity bar: identical step, Newton-iteration, setup, Jacobian and failure
+counts, final state within 1e-10 relative (the bar of the plain FP64 path).
+Also: the mass-action RHS and Jacobian kernels equal the CPU evaluation bit
+for bit, and a large tiled batch (every cell a copy of a small base set)
+reproduces the base set's oracle run -- a size-independent check at a
+problem size the oracle cannot integrate in test time.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from sundials_amd import _lib
+from helpers import dp
+
+pytestmark = pytest.mark.gpu
+
+PM = C.POINTER(_lib.MassAction)
+
+
+@pytest.fixture(scope="module")
+def libs():
+    import torch
+
+    assert torch.cuda.is_available()
+    return _lib.core(), _lib.hip(), _lib.load_oracle()
+
+
+def _prob(core, m, R, ncells, seed):
+    p = core.SUNAMD_MassActionCreate(m, R, ncells, seed, 0.2)
+    assert p
+    return p
+
+
+def _dev_array(n):
+    import torch
+
+    return torch.zeros(n, dtype=torch.float64, device="cuda")
+
+
+@pytest.mark.parametrize("m,R,ncells", [(4, 6, 33), (8, 16, 257), (32, 48, 65), (64, 96, 7)])
+def test_rhs_and_jacobian_bitwise(libs, m, R, ncells):
+    import torch
+
+    core, hip, orc = libs
+    p = _prob(core, m, R, ncells, 77 + m)
+    pc = p.contents
+    y = np.ctypeslib.as_array(pc.y0, shape=(ncells * m,)).copy()
+    y *= np.random.default_rng(m).uniform(0.5, 1.5, y.size)
+    f_ref = np.zeros(ncells * m)
+    J_ref = np.zeros(ncells * m * m)
+    core.SUNAMD_MassActionRhsHost(p, dp(y), dp(f_ref), 0, ncells)
+    core.SUNAMD_MassActionJacHost(p, dp(y), dp(J_ref), 0, ncells)
+    stream = torch.cuda.current_stream().cuda_stream
+    dev = hip.SUNAMD_MassActionToDevice(p, C.c_void_p(stream))
+    assert dev
+    ty = torch.from_numpy(y).cuda()
+    tf = _dev_array(ncells * m)
+    tJ = torch.full((ncells * m * m,), 7.0, dtype=torch.float64, device="cuda")
+    assert hip.SUNAMD_MassActionRhsDevArrays(dev, C.c_void_p(ty.data_ptr()),
+                                             C.c_void_p(tf.data_ptr())) == 0
+    assert hip.SUNAMD_MassActionJacDevArrays(dev, C.c_void_p(ty.data_ptr()),
+                                             C.c_void_p(tJ.data_ptr())) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(tf.cpu().numpy(), f_ref)
+    assert np.array_equal(tJ.cpu().numpy(), J_ref)
+    # mass is conserved by every reaction: columns of J sum to ~0
+    hip.SUNAMD_MassActionDevFree(dev)
+    core.SUNAMD_MassActionFree(p)
+
+
+def _solve_both(libs, p, n, prec=0, rtol=1e-6, atol=1e-10):
+    core, hip, orc = libs
+    y_gpu, y_cpu = np.zeros(n), np.zeros(n)
+    st = _lib.SolveStats()
+    assert hip.SUNAMD_MassActionSolveCoupled_Mi355(p, rtol, atol, 1.0, prec, dp(y_gpu),
+                                                   C.byref(st)) == 0
+    ost = np.zeros(16)
+    assert orc.oracle_massaction_solve_coupled(p, rtol, atol, 1.0, dp(y_cpu), dp(ost)) == 0
+    return y_gpu, st, y_cpu, ost
+
+
+def _counts(st):
+    return (st.nst, st.nfe, st.nsetups, st.netf, st.nni, st.nnf, st.ncfn, st.nje)
+
+
+@pytest.mark.parametrize("m,R,ncells,seed", [(8, 16, 4096, 1001), (8, 16, 64, 7),
+                                              (32, 48, 128, 1002), (3, 4, 1, 5)])
+def test_coupled_fp64_parity(libs, m, R, ncells, seed):
+    core = libs[0]
+    p = _prob(core, m, R, ncells, seed)
+    y_gpu, st, y_cpu, ost = _solve_both(libs, p, ncells * m)
+    core.SUNAMD_MassActionFree(p)
+    assert st.flag == 0 and ost[14] == 0
+    assert _counts(st) == tuple(int(v) for v in ost[:8])
+    assert (st.qlast, st.qcur) == (int(ost[8]), int(ost[9]))
+    rel = np.abs(y_gpu - y_cpu) / np.abs(y_cpu)
+    assert rel.max() <= 1e-10, rel.max()
+    # final state conserves each cell's mass (sum 1) to solver tolerance
+    sums = y_gpu.reshape(ncells, m).sum(axis=1)
+    assert np.abs(sums - 1.0).max() < 1e-6
+
+
+def test_coupled_tiled_matches_base(libs):
+    """2^15 cells tiled from 128 base cells integrate like the base set."""
+    core = libs[0]
+    hip, orc = libs[1], libs[2]
+    m, R = 8, 16
+    base = _prob(core, m, R, 128, 1001)
+    big = core.SUNAMD_MassActionTile(base, 1 << 15)
+    n_big = (1 << 15) * m
+    y_gpu = np.zeros(n_big)
+    st = _lib.SolveStats()
+    assert hip.SUNAMD_MassActionSolveCoupled_Mi355(big, 1e-6, 1e-10, 1.0, 0, dp(y_gpu),
+                                                   C.byref(st)) == 0
+    y_cpu = np.zeros(128 * m)
+    ost = np.zeros(16)
+    assert orc.oracle_massaction_solve_coupled(base, 1e-6, 1e-10, 1.0, dp(y_cpu), dp(ost)) == 0
+    core.SUNAMD_MassActionFree(big)
+    core.SUNAMD_MassActionFree(base)
+    assert _counts(st) == tuple(int(v) for v in ost[:8])
+    rel = np.abs(y_gpu.reshape(-1, 128 * m) - y_cpu[None, :]) / np.abs(y_cpu)[None, :]
+    assert rel.max() <= 1e-10
+
+
+def test_coupled_fp32_lu_accuracy(libs):
+    """FP32 block LU inside the FP64 Newton iteration: final state within
+    rtol of the FP64 oracle (modified Newton acts as iterative refinement)."""
+    core = libs[0]
+    m, R, ncells = 8, 16, 512
+    p = _prob(core, m, R, ncells, 1001)
+    hip, orc = libs[1], libs[2]
+    y32 = np.zeros(ncells * m)
+    st = _lib.SolveStats()
+    assert hip.SUNAMD_MassActionSolveCoupled_Mi355(p, 1e-6, 1e-10, 1.0, 1, dp(y32),
+                                                   C.byref(st)) == 0
+    y_cpu, ost = np.zeros(ncells * m), np.zeros(16)
+    assert orc.oracle_massaction_solve_coupled(p, 1e-6, 1e-10, 1.0, dp(y_cpu), dp(ost)) == 0
+    core.SUNAMD_MassActionFree(p)
+    err = np.abs(y32 - y_cpu) / (1e-6 * np.abs(y_cpu) + 1e-10)
+    assert np.sqrt(np.mean(err ** 2)) <= 1.0
