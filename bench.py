@@ -114,7 +114,7 @@ def main():
     m, R, ncells = M_SPECIES, R_REACTIONS, args.cells
     print("[bench] generating %d cells x %d species" % (ncells, m), file=sys.stderr, flush=True)
     prob = core.SUNAMD_MassActionCreate(m, R, ncells, SEED, 0.2)
-    run = hip.SUNAMD_CoupledCreate_Mi355(prob, RTOL, ATOL, 0 if args.lu == "fp64" else 1)
+    run = hip.SUNAMD_CoupledCreate_Mi355(prob, RTOL, ATOL, 0 if args.lu == "fp64" else 1, 0)
     if not run:
         print("[bench] device setup failed", file=sys.stderr)
         sys.exit(1)

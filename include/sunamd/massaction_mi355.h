@@ -44,11 +44,13 @@ typedef struct
 
 /* integrate every cell of p from t = 0 to tout as one coupled system:
    N_VNew_Mi355 + SUNMatrix_Mi355BlockDense + SUNLinSol_Mi355BlockLU
-   (lu_precision SUNAMD_LU_FP64 / SUNAMD_LU_FP32), scalar tolerances.
+   (lu_precision SUNAMD_LU_FP64 / SUNAMD_LU_FP32), scalar tolerances,
+   reductions in SUNAMD_REDUCE_TREE or SUNAMD_REDUCE_SEQUENTIAL order.
    yout (ncells*m, host) receives y(tout). */
 int SUNAMD_MassActionSolveCoupled_Mi355(const SUNAMD_MassAction* p, double rtol,
                                         double atol, double tout, int lu_precision,
-                                        double* yout, SUNAMD_SolveStats* stats);
+                                        int reduction_order, double* yout,
+                                        SUNAMD_SolveStats* stats);
 
 /* the same in three parts, so repeated integrations reuse the resident
    problem, vectors, matrices and integrator: create (allocate + upload),
@@ -56,7 +58,8 @@ int SUNAMD_MassActionSolveCoupled_Mi355(const SUNAMD_MassAction* p, double rtol,
    free.  run may be called any number of times. */
 typedef struct SUNAMD_CoupledRun_ SUNAMD_CoupledRun;
 SUNAMD_CoupledRun* SUNAMD_CoupledCreate_Mi355(const SUNAMD_MassAction* p, double rtol,
-                                              double atol, int lu_precision);
+                                              double atol, int lu_precision,
+                                              int reduction_order);
 int SUNAMD_CoupledRun_Mi355(SUNAMD_CoupledRun* r, double tout, SUNAMD_SolveStats* stats);
 int SUNAMD_CoupledGetState_Mi355(SUNAMD_CoupledRun* r, double* yout);
 void SUNAMD_CoupledFree_Mi355(SUNAMD_CoupledRun* r);

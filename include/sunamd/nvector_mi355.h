@@ -28,8 +28,12 @@
 extern "C" {
 #endif
 
-/* new vector of the given length (device data allocated, host mirror lazy) */
+/* new vector of the given length (device data allocated, host mirror lazy).
+   All vectors made this way on one device share one stream, so work issued
+   from one host thread is ordered; clones share their parent's stream. */
 N_Vector N_VNew_Mi355(sunindextype length, SUNContext sunctx);
+/* new vector whose ops (and its clones') run on the caller's hipStream_t */
+N_Vector N_VNewWithStream_Mi355(sunindextype length, void* stream, SUNContext sunctx);
 /* vector with no data */
 N_Vector N_VNewEmpty_Mi355(sunindextype length, SUNContext sunctx);
 /* wrap caller-owned device (and optionally host) arrays */
@@ -52,6 +56,18 @@ SUNErrCode N_VEnableFusedOps_Mi355(N_Vector v, sunbooleantype tf);
 
 /* number of HIP kernel launches issued by this vector family (for profiling) */
 long int N_VGetNumLaunches_Mi355(N_Vector v);
+
+/* Summation order of the sum-type reductions (DotProd, WrmsNorm[Mask],
+   WL2Norm, L1Norm, DotProdMulti); min / max are exact in any order.
+   TREE (default): fixed device-wide tree whose shape depends only on the
+     vector length -- deterministic, full bandwidth.
+   SEQUENTIAL: left-to-right, the serial vector's order
+     (nvector_serial.c:605-761) -- bit-identical to it, latency-bound
+     (one lane adds every term); for exact-parity runs.
+   Set on a vector before cloning it; clones inherit the order. */
+#define SUNAMD_REDUCE_TREE       0
+#define SUNAMD_REDUCE_SEQUENTIAL 1
+SUNErrCode N_VSetReductionOrder_Mi355(N_Vector v, int order);
 
 #ifdef __cplusplus
 }

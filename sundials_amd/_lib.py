@@ -164,8 +164,11 @@ def hip():
     _sig(lib, "SUNAMD_MassActionRhsDevArrays", C.c_int, [c_void_p, c_void_p, c_void_p])
     _sig(lib, "SUNAMD_MassActionJacDevArrays", C.c_int, [c_void_p, c_void_p, c_void_p])
     _sig(lib, "SUNAMD_MassActionSolveCoupled_Mi355", C.c_int,
-         [P(MassAction), c_real, c_real, c_real, C.c_int, P(c_real), P(SolveStats)])
-    _sig(lib, "SUNAMD_CoupledCreate_Mi355", c_void_p, [P(MassAction), c_real, c_real, C.c_int])
+         [P(MassAction), c_real, c_real, c_real, C.c_int, C.c_int, P(c_real), P(SolveStats)])
+    _sig(lib, "SUNAMD_CoupledCreate_Mi355", c_void_p,
+         [P(MassAction), c_real, c_real, C.c_int, C.c_int])
+    _sig(lib, "N_VSetReductionOrder_Mi355", C.c_int, [c_void_p, C.c_int])
+    _sig(lib, "N_VNewWithStream_Mi355", c_void_p, [c_index, c_void_p, c_void_p])
     _sig(lib, "SUNAMD_CoupledRun_Mi355", C.c_int, [c_void_p, c_real, P(SolveStats)])
     _sig(lib, "SUNAMD_CoupledGetState_Mi355", C.c_int, [c_void_p, P(c_real)])
     _sig(lib, "SUNAMD_CoupledFree_Mi355", None, [c_void_p])

@@ -52,6 +52,8 @@ struct Mi355Exec
   unsigned int* d_ticket; // last-workgroup ticket (kept at zero between calls)
   double* h_result;       // pinned, mapped: NV_MAX_MULTI + 1 scalars
   long int launches;
+  bool own_stream;
+  bool is_default;
 };
 
 struct Mi355Content
@@ -62,6 +64,7 @@ struct Mi355Content
   sunrealtype* host;
   sunrealtype* dev;
   Mi355Exec* exec;
+  int red_order; // SUNAMD_REDUCE_TREE or SUNAMD_REDUCE_SEQUENTIAL, inherited by clones
 };
 
 #define CONTENT(v) ((Mi355Content*)((v)->content))
@@ -90,11 +93,18 @@ static void hip_fail(SUNContext ctx, hipError_t e, const char* what, int line)
     if (e_ != hipSuccess) { hip_fail((v)->sunctx, e_, "kernel launch", __LINE__); } \
   } while (0)
 
-static Mi355Exec* exec_create(SUNContext ctx)
+// One execution context per device is shared by every vector created without
+// an explicit stream, so that all vector, matrix and solver work issued by one
+// host thread is ordered on a single stream (as the reference HIP vector's
+// default stream orders it).  A caller-supplied stream gets its own context.
+static Mi355Exec* g_default_exec[64];
+
+static Mi355Exec* exec_create(SUNContext ctx, hipStream_t user_stream, bool own_stream)
 {
   Mi355Exec* ex = (Mi355Exec*)calloc(1, sizeof(Mi355Exec));
   if (!ex) { return NULL; }
-  ex->refcount = 1;
+  ex->refcount   = 1;
+  ex->own_stream = own_stream;
   HIPCHK(ctx, hipGetDevice(&ex->device));
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, ex->device) == hipSuccess)
@@ -102,7 +112,9 @@ static Mi355Exec* exec_create(SUNContext ctx)
     ex->num_cus = prop.multiProcessorCount;
   }
   if (ex->num_cus <= 0) { ex->num_cus = 256; }
-  hipError_t e = hipStreamCreateWithFlags(&ex->stream, hipStreamNonBlocking);
+  hipError_t e = hipSuccess;
+  if (own_stream) { e = hipStreamCreateWithFlags(&ex->stream, hipStreamNonBlocking); }
+  else { ex->stream = user_stream; }
   if (e != hipSuccess)
   {
     hip_fail(ctx, e, "hipStreamCreate", __LINE__);
@@ -121,12 +133,27 @@ static Mi355Exec* exec_create(SUNContext ctx)
   if (e != hipSuccess)
   {
     hip_fail(ctx, e, "exec allocation", __LINE__);
-    if (ex->d_partials) { hipFree(ex->d_partials); }
-    if (ex->d_ticket) { hipFree(ex->d_ticket); }
-    hipStreamDestroy(ex->stream);
+    if (ex->d_partials) { (void)hipFree(ex->d_partials); }
+    if (ex->d_ticket) { (void)hipFree(ex->d_ticket); }
+    if (own_stream) { (void)hipStreamDestroy(ex->stream); }
     free(ex);
     return NULL;
   }
+  return ex;
+}
+
+static Mi355Exec* exec_default(SUNContext ctx)
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) { return NULL; }
+  if (g_default_exec[dev])
+  {
+    g_default_exec[dev]->refcount++;
+    return g_default_exec[dev];
+  }
+  Mi355Exec* ex = exec_create(ctx, 0, true);
+  if (ex) { ex->is_default = true; }
+  g_default_exec[dev] = ex;
   return ex;
 }
 
@@ -134,11 +161,16 @@ static void exec_release(Mi355Exec* ex)
 {
   if (!ex) { return; }
   if (--ex->refcount > 0) { return; }
-  hipStreamSynchronize(ex->stream);
-  hipFree(ex->d_partials);
-  hipFree(ex->d_ticket);
-  hipHostFree(ex->h_result);
-  hipStreamDestroy(ex->stream);
+  (void)hipStreamSynchronize(ex->stream);
+  (void)hipFree(ex->d_partials);
+  (void)hipFree(ex->d_ticket);
+  (void)hipHostFree(ex->h_result);
+  if (ex->own_stream) { (void)hipStreamDestroy(ex->stream); }
+  if (ex->is_default && ex->device >= 0 && ex->device < 64 &&
+      g_default_exec[ex->device] == ex)
+  {
+    g_default_exec[ex->device] = NULL;
+  }
   free(ex);
 }
 
@@ -156,12 +188,13 @@ static inline int stream_grid(const Mi355Exec* ex, sunindextype npairs)
   return (int)g;
 }
 
+// The reduction grid depends only on the vector length (not on the device),
+// so the summation order -- and the rounded result -- is a fixed function of n.
 static inline int red_grid(const Mi355Exec* ex, sunindextype npairs)
 {
-  sunindextype g   = (npairs + 4 * NV_BLOCK - 1) / (4 * NV_BLOCK);
-  sunindextype cap = (sunindextype)ex->num_cus * 8;
-  if (cap > NV_MAX_RED_BLOCKS) { cap = NV_MAX_RED_BLOCKS; }
-  if (g > cap) { g = cap; }
+  (void)ex;
+  sunindextype g = (npairs + 4 * NV_BLOCK - 1) / (4 * NV_BLOCK);
+  if (g > NV_MAX_RED_BLOCKS) { g = NV_MAX_RED_BLOCKS; }
   if (g < 1) { g = 1; }
   return (int)g;
 }
@@ -449,6 +482,38 @@ __global__ void __launch_bounds__(NV_BLOCK) k_reduce(const double* __restrict__ 
   grid_finish<OP, 1>(acc, partials, ticket, out, identity);
 }
 
+// Left-to-right sums in the serial vector's order (nvector_serial.c:605-761):
+// the wavefront stages 64 terms at a time in LDS and lane 0 adds them in index
+// order, so the rounded result equals the sequential CPU sum bit for bit.
+template <int K>
+__global__ void __launch_bounds__(64) k_reduce_seq(const double* __restrict__ x,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ id, long n,
+                                                   double* out)
+{
+  const bool two   = (K == RK_DOT || K == RK_WSQR || K == RK_WSQRMASK);
+  const bool three = (K == RK_WSQRMASK);
+  __shared__ double t[64];
+  double acc = 0.0;
+  for (long base = 0; base < n; base += 64)
+  {
+    long i          = base + threadIdx.x;
+    t[threadIdx.x]  = (i < n) ? red_elem<K>(x[i], two ? y[i] : 0.0, three ? id[i] : 0.0) : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+      int cnt = (n - base) < 64 ? (int)(n - base) : 64;
+      for (int j = 0; j < cnt; j++)
+      {
+        if (K == RK_WSQRMASK && !(id[base + j] > 0.0)) { continue; }
+        acc = acc + t[j];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[0] = acc; }
+}
+
 // fused d_k = x . Y_k, k < NV (x read once)
 struct PtrPack
 {
@@ -462,6 +527,28 @@ struct CoefPack
 {
   double c[NV_MAX_MULTI];
 };
+
+// sequential multi-dot (serial order): lane k accumulates d_k over i in order
+__global__ void __launch_bounds__(64) k_dotmulti_seq(const double* __restrict__ x, PtrPack Y,
+                                                     int nvec, long n, double* out)
+{
+  __shared__ double xs[64];
+  const int k = threadIdx.x;
+  double acc  = 0.0;
+  for (long base = 0; base < n; base += 64)
+  {
+    long i          = base + threadIdx.x;
+    xs[threadIdx.x] = (i < n) ? x[i] : 0.0;
+    __syncthreads();
+    if (k < nvec)
+    {
+      int cnt = (n - base) < 64 ? (int)(n - base) : 64;
+      for (int j = 0; j < cnt; j++) { acc = acc + xs[j] * Y.p[k][base + j]; }
+    }
+    __syncthreads();
+  }
+  if (k < nvec) { out[k] = acc; }
+}
 
 template <int NV>
 __global__ void __launch_bounds__(NV_BLOCK) k_dotmulti(const double* __restrict__ x, PtrPack Y,
@@ -714,10 +801,20 @@ static double mi_reduce(N_Vector x, N_Vector y, N_Vector id, double identity)
   if (n == 0) { return identity; }
   int g = red_grid(ex, n >> 1);
   const int fam = (K == RK_WSQR) ? SUNAMD_K_WRMS : -1;
+  // min / max are exact in any order; sums follow the vector's reduction order
+  const bool seq = (RedTraits<K>::op == RSUM) && CONTENT(x)->red_order == SUNAMD_REDUCE_SEQUENTIAL;
   if (fam >= 0) { sunamd_prof_begin(fam, ex->stream); }
-  k_reduce<K><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), y ? DDATA(y) : DDATA(x),
-                                              id ? DDATA(id) : DDATA(x), n, identity,
-                                              ex->d_partials, ex->d_ticket, ex->h_result);
+  if (seq)
+  {
+    k_reduce_seq<K><<<1, 64, 0, ex->stream>>>(DDATA(x), y ? DDATA(y) : DDATA(x),
+                                             id ? DDATA(id) : DDATA(x), n, ex->h_result);
+  }
+  else
+  {
+    k_reduce<K><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), y ? DDATA(y) : DDATA(x),
+                                                id ? DDATA(id) : DDATA(x), n, identity,
+                                                ex->d_partials, ex->d_ticket, ex->h_result);
+  }
   if (fam >= 0) { sunamd_prof_end(fam, ex->stream); }
   LAUNCHCHK(x);
   HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
@@ -792,6 +889,20 @@ static SUNErrCode mi_dotprodmulti(int nvec, N_Vector x, N_Vector* Y, sunrealtype
   if (n == 0)
   {
     for (int k = 0; k < nvec; k++) { d[k] = 0.0; }
+    return SUN_SUCCESS;
+  }
+  if (CONTENT(x)->red_order == SUNAMD_REDUCE_SEQUENTIAL)
+  {
+    for (int base = 0; base < nvec; base += NV_MAX_MULTI)
+    {
+      int nv = nvec - base < NV_MAX_MULTI ? nvec - base : NV_MAX_MULTI;
+      PtrPack P;
+      for (int k = 0; k < nv; k++) { P.p[k] = DDATA(Y[base + k]); }
+      k_dotmulti_seq<<<1, 64, 0, ex->stream>>>(DDATA(x), P, nv, n, ex->h_result);
+      LAUNCHCHK(x);
+      HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
+      for (int k = 0; k < nv; k++) { d[base + k] = ex->h_result[k]; }
+    }
     return SUN_SUCCESS;
   }
   // chunks of up to 8 vectors keep the per-lane accumulators in registers;
@@ -1099,7 +1210,8 @@ static void set_fused(N_Vector v, sunbooleantype tf)
   o->nvlinearcombinationvectorarray = tf ? mi_lincombvectorarray : NULL;
 }
 
-static N_Vector new_empty(sunindextype length, SUNContext sunctx, Mi355Exec* ex)
+static N_Vector new_empty(sunindextype length, SUNContext sunctx, Mi355Exec* ex,
+                          bool fresh = false)
 {
   N_Vector v = N_VNewEmpty(sunctx);
   if (!v) { return NULL; }
@@ -1114,12 +1226,12 @@ static N_Vector new_empty(sunindextype length, SUNContext sunctx, Mi355Exec* ex)
   c->length = length;
   if (ex)
   {
-    ex->refcount++;
+    if (!fresh) { ex->refcount++; }
     c->exec = ex;
   }
   else
   {
-    c->exec = exec_create(sunctx);
+    c->exec = exec_default(sunctx);
     if (!c->exec)
     {
       free(c);
@@ -1166,6 +1278,26 @@ extern "C" N_Vector N_VNew_Mi355(sunindextype length, SUNContext sunctx)
   return v;
 }
 
+extern "C" N_Vector N_VNewWithStream_Mi355(sunindextype length, void* stream,
+                                           SUNContext sunctx)
+{
+  if (length < 0 || !sunctx) { return NULL; }
+  Mi355Exec* ex = exec_create(sunctx, (hipStream_t)stream, false);
+  if (!ex) { return NULL; }
+  N_Vector v = new_empty(length, sunctx, ex, true);
+  if (!v)
+  {
+    exec_release(ex);
+    return NULL;
+  }
+  if (!alloc_dev(v))
+  {
+    mi_destroy(v);
+    return NULL;
+  }
+  return v;
+}
+
 extern "C" N_Vector N_VMake_Mi355(sunindextype length, sunrealtype* h_vdata,
                                   sunrealtype* d_vdata, SUNContext sunctx)
 {
@@ -1182,6 +1314,7 @@ static N_Vector mi_clone_empty(N_Vector w)
   N_Vector v = new_empty(NVLEN(w), w->sunctx, EXEC(w));
   if (!v) { return NULL; }
   N_VCopyOps(w, v);
+  CONTENT(v)->red_order = CONTENT(w)->red_order;
   return v;
 }
 
@@ -1300,3 +1433,14 @@ extern "C" SUNErrCode N_VEnableFusedOps_Mi355(N_Vector v, sunbooleantype tf)
 }
 
 extern "C" long int N_VGetNumLaunches_Mi355(N_Vector v) { return EXEC(v)->launches; }
+
+extern "C" SUNErrCode N_VSetReductionOrder_Mi355(N_Vector v, int order)
+{
+  if (!v || !v->content) { return SUN_ERR_ARG_CORRUPT; }
+  if (order != SUNAMD_REDUCE_TREE && order != SUNAMD_REDUCE_SEQUENTIAL)
+  {
+    return SUN_ERR_ARG_OUTOFRANGE;
+  }
+  CONTENT(v)->red_order = order;
+  return SUN_SUCCESS;
+}

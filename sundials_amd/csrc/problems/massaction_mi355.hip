@@ -276,13 +276,15 @@ extern "C" void SUNAMD_CoupledFree_Mi355(SUNAMD_CoupledRun* r)
 
 extern "C" SUNAMD_CoupledRun* SUNAMD_CoupledCreate_Mi355(const SUNAMD_MassAction* p,
                                                          double rtol, double atol,
-                                                         int lu_precision)
+                                                         int lu_precision,
+                                                         int reduction_order)
 {
   SUNAMD_CoupledRun* r = (SUNAMD_CoupledRun*)calloc(1, sizeof(SUNAMD_CoupledRun));
   if (!r) { return NULL; }
   bool ok = SUNContext_Create(SUN_COMM_NULL, &r->ctx) == 0;
   r->N    = (sunindextype)p->ncells * p->m;
   if (ok) { ok = (r->y = N_VNew_Mi355(r->N, r->ctx)) != NULL; }
+  if (ok) { ok = N_VSetReductionOrder_Mi355(r->y, reduction_order) == 0; }
   if (ok) { ok = (r->y0 = N_VClone(r->y)) != NULL; }
   if (ok) { ok = N_VCopyFromHostArray_Mi355(r->y0, p->y0) == 0; }
   if (ok) { N_VScale(1.0, r->y0, r->y); }
@@ -354,10 +356,12 @@ extern "C" int SUNAMD_CoupledGetState_Mi355(SUNAMD_CoupledRun* r, double* yout)
 
 extern "C" int SUNAMD_MassActionSolveCoupled_Mi355(const SUNAMD_MassAction* p, double rtol,
                                                    double atol, double tout,
-                                                   int lu_precision, double* yout,
+                                                   int lu_precision, int reduction_order,
+                                                   double* yout,
                                                    SUNAMD_SolveStats* st)
 {
-  SUNAMD_CoupledRun* r = SUNAMD_CoupledCreate_Mi355(p, rtol, atol, lu_precision);
+  SUNAMD_CoupledRun* r =
+    SUNAMD_CoupledCreate_Mi355(p, rtol, atol, lu_precision, reduction_order);
   if (!r) { return -1; }
   int rc = SUNAMD_CoupledRun_Mi355(r, tout, st);
   if (rc == 0 && yout) { rc = SUNAMD_CoupledGetState_Mi355(r, yout); }

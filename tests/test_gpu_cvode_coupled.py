@@ -73,11 +73,14 @@ def test_rhs_and_jacobian_bitwise(libs, m, R, ncells):
     core.SUNAMD_MassActionFree(p)
 
 
-def _solve_both(libs, p, n, prec=0, rtol=1e-6, atol=1e-10):
+SEQ, TREE = 1, 0
+
+
+def _solve_both(libs, p, n, prec=0, order=SEQ, rtol=1e-6, atol=1e-10):
     core, hip, orc = libs
     y_gpu, y_cpu = np.zeros(n), np.zeros(n)
     st = _lib.SolveStats()
-    assert hip.SUNAMD_MassActionSolveCoupled_Mi355(p, rtol, atol, 1.0, prec, dp(y_gpu),
+    assert hip.SUNAMD_MassActionSolveCoupled_Mi355(p, rtol, atol, 1.0, prec, order, dp(y_gpu),
                                                    C.byref(st)) == 0
     ost = np.zeros(16)
     assert orc.oracle_massaction_solve_coupled(p, rtol, atol, 1.0, dp(y_cpu), dp(ost)) == 0
@@ -88,25 +91,53 @@ def _counts(st):
     return (st.nst, st.nfe, st.nsetups, st.netf, st.nni, st.nnf, st.ncfn, st.nje)
 
 
+def _wrms_err(y, yref, rtol=1e-6, atol=1e-10):
+    return np.sqrt(np.mean(((y - yref) / (rtol * np.abs(yref) + atol)) ** 2))
+
+
 @pytest.mark.parametrize("m,R,ncells,seed", [(8, 16, 4096, 1001), (8, 16, 64, 7),
                                               (32, 48, 128, 1002), (3, 4, 1, 5)])
-def test_coupled_fp64_parity(libs, m, R, ncells, seed):
+def test_coupled_fp64_parity_sequential(libs, m, R, ncells, seed):
+    """Reductions in the serial vector's order: the whole integration is
+    bit-identical to the oracle (every counter and every final value)."""
     core = libs[0]
     p = _prob(core, m, R, ncells, seed)
-    y_gpu, st, y_cpu, ost = _solve_both(libs, p, ncells * m)
+    y_gpu, st, y_cpu, ost = _solve_both(libs, p, ncells * m, order=SEQ)
     core.SUNAMD_MassActionFree(p)
     assert st.flag == 0 and ost[14] == 0
     assert _counts(st) == tuple(int(v) for v in ost[:8])
     assert (st.qlast, st.qcur) == (int(ost[8]), int(ost[9]))
-    rel = np.abs(y_gpu - y_cpu) / np.abs(y_cpu)
-    assert rel.max() <= 1e-10, rel.max()
-    # final state conserves each cell's mass (sum 1) to solver tolerance
+    assert (st.hinused, st.hlast, st.tcur) == (ost[10], ost[11], ost[13])
+    assert np.array_equal(y_gpu, y_cpu)
+    # each cell's mass (sum 1) is conserved to solver tolerance
     sums = y_gpu.reshape(ncells, m).sum(axis=1)
     assert np.abs(sums - 1.0).max() < 1e-6
 
 
+@pytest.mark.parametrize("m,R,ncells,seed", [(8, 16, 1024, 1001), (32, 48, 128, 1002)])
+def test_coupled_fp64_tree_reductions(libs, m, R, ncells, seed):
+    """Full-bandwidth tree reductions: the step controller of this stiff
+    problem amplifies ulp-level norm differences into a different step
+    sequence (the oracle itself moves by ~1-6% in nst when its WRMS is summed
+    pairwise instead of left to right), so the run is checked for accuracy:
+    its error against a reference solution computed at 1000x tighter
+    tolerances is no larger than twice the oracle's own error."""
+    core, orc = libs[0], libs[2]
+    p = _prob(core, m, R, ncells, seed)
+    y_gpu, st, y_cpu, ost = _solve_both(libs, p, ncells * m, order=TREE)
+    y_ref = np.zeros(ncells * m)
+    assert orc.oracle_massaction_solve_coupled(p, 1e-9, 1e-13, 1.0, dp(y_ref), None) == 0
+    core.SUNAMD_MassActionFree(p)
+    assert st.flag == 0
+    assert abs(st.nst - ost[0]) <= 0.1 * ost[0]
+    e_gpu, e_cpu = _wrms_err(y_gpu, y_ref), _wrms_err(y_cpu, y_ref)
+    assert e_gpu <= 2.0 * e_cpu + 0.1, (e_gpu, e_cpu)
+
+
 def test_coupled_tiled_matches_base(libs):
-    """2^15 cells tiled from 128 base cells integrate like the base set."""
+    """2^15 cells tiled from 128 base cells: every tile integrates like the
+    base set (WRMS norms of the tiled vector equal the base set's up to
+    rounding), checked against the oracle run of the 128 base cells."""
     core = libs[0]
     hip, orc = libs[1], libs[2]
     m, R = 8, 16
@@ -115,16 +146,18 @@ def test_coupled_tiled_matches_base(libs):
     n_big = (1 << 15) * m
     y_gpu = np.zeros(n_big)
     st = _lib.SolveStats()
-    assert hip.SUNAMD_MassActionSolveCoupled_Mi355(big, 1e-6, 1e-10, 1.0, 0, dp(y_gpu),
+    assert hip.SUNAMD_MassActionSolveCoupled_Mi355(big, 1e-6, 1e-10, 1.0, 0, TREE, dp(y_gpu),
                                                    C.byref(st)) == 0
     y_cpu = np.zeros(128 * m)
     ost = np.zeros(16)
     assert orc.oracle_massaction_solve_coupled(base, 1e-6, 1e-10, 1.0, dp(y_cpu), dp(ost)) == 0
     core.SUNAMD_MassActionFree(big)
     core.SUNAMD_MassActionFree(base)
-    assert _counts(st) == tuple(int(v) for v in ost[:8])
-    rel = np.abs(y_gpu.reshape(-1, 128 * m) - y_cpu[None, :]) / np.abs(y_cpu)[None, :]
-    assert rel.max() <= 1e-10
+    tiles = y_gpu.reshape(-1, 128 * m)
+    # every tile is bit-identical (cells never interact except through norms)
+    assert all(np.array_equal(tiles[0], t) for t in tiles[1:])
+    assert abs(st.nst - ost[0]) <= 0.1 * ost[0]
+    assert _wrms_err(tiles[0], y_cpu) <= 3.0
 
 
 def test_coupled_fp32_lu_accuracy(libs):
@@ -136,10 +169,13 @@ def test_coupled_fp32_lu_accuracy(libs):
     hip, orc = libs[1], libs[2]
     y32 = np.zeros(ncells * m)
     st = _lib.SolveStats()
-    assert hip.SUNAMD_MassActionSolveCoupled_Mi355(p, 1e-6, 1e-10, 1.0, 1, dp(y32),
+    assert hip.SUNAMD_MassActionSolveCoupled_Mi355(p, 1e-6, 1e-10, 1.0, 1, TREE, dp(y32),
                                                    C.byref(st)) == 0
     y_cpu, ost = np.zeros(ncells * m), np.zeros(16)
     assert orc.oracle_massaction_solve_coupled(p, 1e-6, 1e-10, 1.0, dp(y_cpu), dp(ost)) == 0
+    y_ref = np.zeros(ncells * m)
+    assert orc.oracle_massaction_solve_coupled(p, 1e-9, 1e-13, 1.0, dp(y_ref), None) == 0
     core.SUNAMD_MassActionFree(p)
-    err = np.abs(y32 - y_cpu) / (1e-6 * np.abs(y_cpu) + 1e-10)
-    assert np.sqrt(np.mean(err ** 2)) <= 1.0
+    assert st.flag == 0
+    e32, e_cpu = _wrms_err(y32, y_ref), _wrms_err(y_cpu, y_ref)
+    assert e32 <= 2.0 * e_cpu + 0.1, (e32, e_cpu)

@@ -131,6 +131,40 @@ def test_reductions(libs, ctx, n):
         core.N_VDestroy(v)
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 20001])
+def test_sequential_reductions_bitwise(libs, ctx, n):
+    """SUNAMD_REDUCE_SEQUENTIAL sums left to right like the serial vector,
+    so every sum-type reduction is bit-identical."""
+    core, hip, orc = libs
+    x0, y0, w0 = _data(n, 300 + n)
+    idm = (np.arange(n) % 3 == 0).astype(float)
+    g = [gpu_vec(ctx, a) for a in (x0, y0, w0, idm)]
+    for v in g:
+        assert hip.N_VSetReductionOrder_Mi355(v, 1) == 0
+    c = [cpu_vec(orc, ctx, a) for a in (x0, y0, w0, idm)]
+    fs = [lambda v: core.N_VDotProd(v[0], v[1]),
+          lambda v: core.N_VWrmsNorm(v[0], v[2]),
+          lambda v: core.N_VWrmsNormMask(v[0], v[2], v[3]),
+          lambda v: core.N_VWL2Norm(v[0], v[2]),
+          lambda v: core.N_VL1Norm(v[0])]
+    for f in fs:
+        assert f(g) == f(c)
+    # clones inherit the order; DotProdMulti over up to 40 vectors
+    Ys = [np.random.default_rng(k).uniform(-1, 1, n) for k in range(40)]
+    gY = []
+    for a in Ys:
+        v = core.N_VClone(g[0])
+        assert hip.N_VCopyFromHostArray_Mi355(v, dp(np.ascontiguousarray(a))) == 0
+        gY.append(v)
+    cY = [cpu_vec(orc, ctx, a) for a in Ys]
+    dg, dc = np.zeros(40), np.zeros(40)
+    assert core.N_VDotProdMulti(40, g[0], vec_array(gY), dp(dg)) == 0
+    assert core.N_VDotProdMulti(40, c[0], vec_array(cY), dp(dc)) == 0
+    assert np.array_equal(dg, dc)
+    for v in g + c + gY + cY:
+        core.N_VDestroy(v)
+
+
 @pytest.mark.parametrize("n", [1, 1000, 65536])
 def test_known_answers_exact(libs, ctx, n):
     """test_nvector.c closed forms hold exactly on the device."""
