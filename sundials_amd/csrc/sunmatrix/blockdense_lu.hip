@@ -45,10 +45,10 @@
 
 static void hip_fail(SUNContext ctx, hipError_t e, const char* what, int line)
 {
+  (void)ctx;
   fprintf(stderr, "[sundials_amd blockdense] %s failed (line %d): %s\n", what, line,
           hipGetErrorString(e));
   fflush(stderr);
-  SUNAMD_SetLastError(ctx, SUN_ERR_EXT_FAIL);
 }
 #define HIPCHK(ctx, call)                                           \
   do {                                                              \
