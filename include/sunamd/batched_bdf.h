@@ -1,0 +1,85 @@
+/* -----------------------------------------------------------------------------
+ * batched_bdf.h -- thousands of independent stiff systems of uneven size,
+ * each integrated by its own BDF / modified-Newton / dense-LU integrator with
+ * its own step size and order, all inside one persistent launch per size
+ * bucket on the MI355X.
+ *
+ * Each system follows the reference CVODE algorithm exactly as the host
+ * restatement (cvode_bdf.c) does for a serial N_Vector of length m with a
+ * dense m x m LU (cvode.c cvStep / cvNls / error test / eta selection,
+ * cvode_nls.c, cvode_ls.c, sunnonlinsol_newton.c, sundials_dense.c):
+ *   - one group of pow2ceil(m) lanes per system, lane = species;
+ *   - WRMS norms summed in species order (the serial vector's order);
+ *   - the Newton matrix I - gamma*J factored by the register-resident
+ *     pivoting LU of the block solver, bit-identical in FP64;
+ *   - the saved Jacobian is kept as the state it was evaluated at and
+ *     rebuilt when reused (the same values as a stored copy);
+ *   - x^y in the step-size ratios is the correctly rounded sunamd_pow used by
+ *     the host integrator.
+ * In FP64 mode every system's step count, iteration counts and final state
+ * equal the host integrator's on the same system, bit for bit.  In FP32 mode
+ * the Newton matrix is factored in single precision (residuals, updates and
+ * norms stay FP64).
+ * ---------------------------------------------------------------------------*/
+#ifndef SUNAMD_BATCHED_BDF_H
+#define SUNAMD_BATCHED_BDF_H
+
+#include "massaction.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SUNAMD_BATCH_MAXB 8
+
+/* systems grouped in size buckets; bucket b holds nb[b] systems of m[b]
+   species, stored as the cells of its own mass-action network */
+typedef struct
+{
+  int nbuckets;
+  SUNAMD_MassAction* bucket[SUNAMD_BATCH_MAXB];
+  int64_t nsys;
+  int* sys_bucket;     /* bucket of system s */
+  int64_t* sys_cell;   /* cell index of system s inside its bucket */
+  int64_t* sys_yoff;   /* offset of system s in the concatenated state */
+  int64_t ytotal;      /* sum of all m */
+  uint64_t seed;
+} SUNAMD_Batch;
+
+/* nsys systems; m drawn from m_values with the given weights (counter-based,
+   stream 5); R = round(rfactor * m); bucket b's network uses seed
+   seed + 7919*(b+1); systems of a bucket are its cells 0, 1, ... in order */
+SUNAMD_Batch* SUNAMD_BatchCreate(int64_t nsys, uint64_t seed, int nvalues, const int* m_values,
+                                 const double* weights, double rfactor, double sigma);
+void SUNAMD_BatchFree(SUNAMD_Batch* b);
+
+/* per-system counters, 10 per system, in this order */
+#define SUNAMD_BS_NST     0
+#define SUNAMD_BS_NFE     1
+#define SUNAMD_BS_NSETUPS 2
+#define SUNAMD_BS_NETF    3
+#define SUNAMD_BS_NNI     4
+#define SUNAMD_BS_NNF     5
+#define SUNAMD_BS_NCFN    6
+#define SUNAMD_BS_NJE     7
+#define SUNAMD_BS_QLAST   8
+#define SUNAMD_BS_FLAG    9
+#define SUNAMD_BS_COUNT   10
+
+typedef struct SUNAMD_BatchedBDF_ SUNAMD_BatchedBDF;
+
+/* upload the batch and allocate results (lu_precision: SUNAMD_LU_FP64/FP32) */
+SUNAMD_BatchedBDF* SUNAMD_BatchedBDFCreate(const SUNAMD_Batch* batch, double rtol,
+                                           double atol, int lu_precision);
+/* integrate every system from t = 0 to tout; returns 0 when every system
+   succeeded; *wall_s receives the device time of the whole batch */
+int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* wall_s);
+/* results in system order: y (ytotal), counters (nsys*SUNAMD_BS_COUNT) */
+int SUNAMD_BatchedBDFGetResults(SUNAMD_BatchedBDF* B, double* y, long int* counters);
+void SUNAMD_BatchedBDFFree(SUNAMD_BatchedBDF* B);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -22,6 +22,7 @@
 
 #include "oracle.h"
 #include "sunamd/cvode_sunamd.h"
+#include "sunamd/batched_bdf.h"
 #include "sunamd/massaction.h"
 
 static double now_s(void)
@@ -272,4 +273,50 @@ int oracle_massaction_solve_per_cell(const SUNAMD_MassAction* p, double rtol, do
   }
   if (wall) { *wall = now_s() - t0; }
   return nfail ? -1 : 0;
+}
+
+/* ---- uneven batch of independent systems ----------------------------------
+   Each bucket's systems go through oracle_massaction_solve_per_cell (one
+   integration per system, its own CVodeReInit) and the results are scattered
+   back to system order: y has batch->ytotal entries, counters 10 per system
+   (nst nfe nsetups netf nni nnf ncfn nje qlast flag). */
+int oracle_batch_solve(const SUNAMD_Batch* batch, double rtol, double atol, double tout,
+                       int nthreads, double* y, long int* counters, double* wall)
+{
+  double t0 = now_s();
+  int rc    = 0;
+  for (int b = 0; b < batch->nbuckets; b++)
+  {
+    const SUNAMD_MassAction* p = batch->bucket[b];
+    if (p->ncells <= 0) { continue; }
+    double* yb = (double*)malloc(sizeof(double) * (size_t)(p->ncells * p->m));
+    double* sb = (double*)malloc(sizeof(double) * 16 * (size_t)p->ncells);
+    if (!yb || !sb)
+    {
+      free(yb);
+      free(sb);
+      return -1;
+    }
+    if (oracle_massaction_solve_per_cell(p, rtol, atol, tout, nthreads, yb, sb, NULL) != 0)
+    {
+      rc = -1;
+    }
+    for (int64_t s = 0; s < batch->nsys; s++)
+    {
+      if (batch->sys_bucket[s] != b) { continue; }
+      int64_t c = batch->sys_cell[s];
+      if (y) { memcpy(y + batch->sys_yoff[s], yb + c * p->m, sizeof(double) * (size_t)p->m); }
+      if (counters)
+      {
+        long int* o = counters + 10 * s;
+        for (int k = 0; k < 8; k++) { o[k] = (long int)sb[16 * c + k]; }
+        o[8] = (long int)sb[16 * c + 8];
+        o[9] = (long int)sb[16 * c + 14];
+      }
+    }
+    free(yb);
+    free(sb);
+  }
+  if (wall) { *wall = now_s() - t0; }
+  return rc;
 }

@@ -52,6 +52,25 @@ class SolveStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+BATCH_MAXB = 8
+BS_NAMES = ("nst", "nfe", "nsetups", "netf", "nni", "nnf", "ncfn", "nje", "qlast", "flag")
+BS_COUNT = len(BS_NAMES)
+
+
+class Batch(C.Structure):
+    """Mirror of SUNAMD_Batch (include/sunamd/batched_bdf.h)."""
+    _fields_ = [
+        ("nbuckets", C.c_int),
+        ("bucket", C.POINTER(MassAction) * BATCH_MAXB),
+        ("nsys", C.c_int64),
+        ("sys_bucket", C.POINTER(C.c_int)),
+        ("sys_cell", C.POINTER(C.c_int64)),
+        ("sys_yoff", C.POINTER(C.c_int64)),
+        ("ytotal", C.c_int64),
+        ("seed", C.c_uint64),
+    ]
+
+
 def _sig(lib, name, restype, argtypes):
     f = getattr(lib, name)
     f.restype = restype
@@ -127,6 +146,9 @@ def core():
          [P(MassAction), P(c_real), P(c_real), C.c_int64, C.c_int64])
     _sig(lib, "SUNAMD_MassActionJacHost", None,
          [P(MassAction), P(c_real), P(c_real), C.c_int64, C.c_int64])
+    _sig(lib, "SUNAMD_BatchCreate", P(Batch),
+         [C.c_int64, C.c_uint64, C.c_int, P(C.c_int), P(c_real), c_real, c_real])
+    _sig(lib, "SUNAMD_BatchFree", None, [P(Batch)])
     _core = lib
     return lib
 
@@ -177,6 +199,10 @@ def hip():
     _sig(lib, "SUNAMD_ProfilerReset", None, [])
     _sig(lib, "SUNAMD_ProfilerGet", C.c_int, [C.c_int, P(C.c_double), P(C.c_long)])
     _sig(lib, "SUNAMD_BenchVectorOps", C.c_int, [c_index, C.c_int, C.c_int, P(c_real)])
+    _sig(lib, "SUNAMD_BatchedBDFCreate", c_void_p, [P(Batch), c_real, c_real, C.c_int])
+    _sig(lib, "SUNAMD_BatchedBDFRun", C.c_int, [c_void_p, c_real, P(c_real)])
+    _sig(lib, "SUNAMD_BatchedBDFGetResults", C.c_int, [c_void_p, P(c_real), P(C.c_long)])
+    _sig(lib, "SUNAMD_BatchedBDFFree", None, [c_void_p])
     _hip = lib
     return lib
 
@@ -206,6 +232,8 @@ def load_oracle():
          [P(MassAction), c_real, c_real, c_real, P(c_real), P(c_real)])
     _sig(lib, "oracle_massaction_solve_per_cell", C.c_int,
          [P(MassAction), c_real, c_real, c_real, C.c_int, P(c_real), P(c_real), P(c_real)])
+    _sig(lib, "oracle_batch_solve", C.c_int,
+         [P(Batch), c_real, c_real, c_real, C.c_int, P(c_real), P(C.c_long), P(c_real)])
     return lib
 
 

@@ -34,6 +34,7 @@
 #include <string.h>
 
 #include "sunamd/cvode_sunamd.h"
+#include "sunamd_pow.h"
 
 /* ---- constants (values from cvode_impl.h, cvode.c, cvode_ls_impl.h) ------ */
 #define BDF_Q_MAX 5
@@ -113,7 +114,10 @@
 #define SUNMIN(a, b) ((a) < (b) ? (a) : (b))
 #define SUNRabs(x)   fabs(x)
 #define SUNRsqrt(x)  ((x) <= 0.0 ? 0.0 : sqrt((x)))
-#define SUNRpowerR(b, e) pow((b), (e))
+/* SUNRpowerR is pow() in the reference (sundials_math.h:235); the correctly
+   rounded sunamd_pow gives the same result wherever libm's pow is correctly
+   rounded, and the same result on host and device everywhere */
+#define SUNRpowerR(b, e) sunamd_pow((b), (e))
 
 /* ---- integrator state ---------------------------------------------------- */
 typedef struct

@@ -1,0 +1,83 @@
+"""Uneven batch of independent stiff systems, integrated by the persistent
+device BDF (one lane group per system, per-system step size and order)
+against the CPU oracle (serial vector + dense LU, one CVode run per system).
+
+FP64 bar: every system's step, RHS, setup, error-test, Newton-iteration,
+Newton-failure, step-failure and Jacobian counts and its last order equal
+the oracle's, and the final state is identical bit for bit (stronger than
+the 1e-10 relative bar).
+FP32-LU: the Newton matrix is factored in single precision while residuals,
+updates and norms stay FP64; the modified Newton iteration absorbs the
+factor's rounding, so every system still succeeds and its final state stays
+within the integration tolerance of the FP64 oracle (weighted RMS of the
+difference, weights 1/(rtol |y| + atol), below 1 -- the difference between
+two runs at rtol 1e-6 is of the order of the local error control itself).
+"""
+import numpy as np
+import pytest
+
+from batch_util import RTOL, ATOL, make_batch, free_batch, run_device, run_oracle, \
+    system_slices
+
+pytestmark = pytest.mark.gpu
+
+LU_FP64, LU_FP32 = 0, 1
+
+
+@pytest.fixture(scope="module")
+def batch_and_oracle():
+    import torch
+
+    assert torch.cuda.is_available()
+    b = make_batch(700, 4004)
+    y, cnt, rc, _ = run_oracle(b, nthreads=16)
+    assert rc == 0
+    yield b, y, cnt
+    free_batch(b)
+
+
+def test_batched_fp64_bitwise_vs_oracle(batch_and_oracle):
+    b, y_ref, c_ref = batch_and_oracle
+    y, cnt, rc, wall = run_device(b, LU_FP64)
+    assert rc == 0
+    assert np.all(cnt[:, 9] == 0)
+    bad = np.nonzero(np.any(cnt != c_ref, axis=1))[0]
+    assert bad.size == 0, "counter mismatch in systems %s: gpu %s oracle %s" % (
+        bad[:5], cnt[bad[:2]], c_ref[bad[:2]])
+    assert np.array_equal(y, y_ref)
+    # every bucket was exercised
+    ms = {m for (_, _, m, _) in system_slices(b)}
+    assert ms == {4, 8, 16, 24, 32, 48, 64}
+
+
+def test_batched_fp32_lu_accuracy(batch_and_oracle):
+    b, y_ref, c_ref = batch_and_oracle
+    y, cnt, rc, wall = run_device(b, LU_FP32)
+    assert rc == 0
+    assert np.all(cnt[:, 9] == 0)
+    for (k, c, m, off) in system_slices(b):
+        d = y[off:off + m] - y_ref[off:off + m]
+        w = 1.0 / (RTOL * np.abs(y_ref[off:off + m]) + ATOL)
+        assert np.sqrt(np.mean((d * w) ** 2)) < 1.0, (k, c, m)
+    # Newton iterations per step stay close to the FP64 run
+    nni32 = cnt[:, 4].sum() / cnt[:, 0].sum()
+    nni64 = c_ref[:, 4].sum() / c_ref[:, 0].sum()
+    assert nni32 < 1.25 * nni64
+
+
+def test_batched_repeatable_and_reuse():
+    import torch
+
+    assert torch.cuda.is_available()
+    b = make_batch(97, 77, m_values=(5, 12), weights=(1.0, 1.0))
+    try:
+        y1, c1, rc1, _ = run_device(b, LU_FP64)
+        y2, c2, rc2, _ = run_device(b, LU_FP64)
+        assert rc1 == 0 and rc2 == 0
+        assert np.array_equal(y1, y2) and np.array_equal(c1, c2)
+        y_ref, c_ref, rc, _ = run_oracle(b, nthreads=8)
+        assert rc == 0
+        assert np.array_equal(c1, c_ref)
+        assert np.array_equal(y1, y_ref)
+    finally:
+        free_batch(b)
