@@ -71,6 +71,10 @@ typedef struct SUNAMD_BatchedBDF_ SUNAMD_BatchedBDF;
 /* upload the batch and allocate results (lu_precision: SUNAMD_LU_FP64/FP32) */
 SUNAMD_BatchedBDF* SUNAMD_BatchedBDFCreate(const SUNAMD_Batch* batch, double rtol,
                                            double atol, int lu_precision);
+/* largest number of steps per system (CVodeSetMaxNumSteps, cvode_io.c):
+   0 restores the default 500, a negative value disables the limit; a system
+   that reaches it stops with flag -1 (CV_TOO_MUCH_WORK) */
+int SUNAMD_BatchedBDFSetMaxNumSteps(SUNAMD_BatchedBDF* B, long int mxsteps);
 /* integrate every system from t = 0 to tout; returns 0 when every system
    succeeded; *wall_s receives the device time of the whole batch */
 int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* wall_s);

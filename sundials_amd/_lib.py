@@ -200,6 +200,7 @@ def hip():
     _sig(lib, "SUNAMD_ProfilerGet", C.c_int, [C.c_int, P(C.c_double), P(C.c_long)])
     _sig(lib, "SUNAMD_BenchVectorOps", C.c_int, [c_index, C.c_int, C.c_int, P(c_real)])
     _sig(lib, "SUNAMD_BatchedBDFCreate", c_void_p, [P(Batch), c_real, c_real, C.c_int])
+    _sig(lib, "SUNAMD_BatchedBDFSetMaxNumSteps", C.c_int, [c_void_p, C.c_long])
     _sig(lib, "SUNAMD_BatchedBDFRun", C.c_int, [c_void_p, c_real, P(c_real)])
     _sig(lib, "SUNAMD_BatchedBDFGetResults", C.c_int, [c_void_p, P(c_real), P(C.c_long)])
     _sig(lib, "SUNAMD_BatchedBDFFree", None, [c_void_p])

@@ -1,1159 +1,28 @@
 // -----------------------------------------------------------------------------
-// Batched per-system BDF on the MI355X: one persistent launch per size bucket,
-// one group of G = pow2ceil(m) lanes per system (lane = species), groups pull
-// systems from a device work queue until the bucket is done.
-//
-// Every decision follows the reference CVODE exactly as the host integrator
-// (cvode_bdf.c) restates it for a serial vector of length m and a dense LU:
-//   CVode first call / loop        cvode.c:1037-1670
-//   cvHin, cvUpperBoundH0, cvYddNorm cvode.c:2161-2340
-//   cvStep, cvAdjustParams/Order, cvIncreaseBDF, cvDecreaseBDF, cvRescale,
-//   cvPredict, cvSet/SetBDF/SetTqBDF  cvode.c:2367-3053
-//   cvNls, cvHandleNFlag, cvRestore, cvDoErrorTest, cvCompleteStep,
-//   cvPrepareNextStep, cvSetEta, cvComputeEtaqm1/qp1, cvChooseEta
-//                                  cvode.c:3068-3711
-//   cvEwtSetSS                     cvode.c:4791-4817
-//   residual / conv. test / setup  cvode_nls.c:256-399
-//   J reuse, A = I - gamma J, solve scaling  cvode_ls.c:1311-1850
-//   Newton loop                    sunnonlinsol_newton.c:179-324
-//   dense GETRF / GETRS            sundials_dense.c:85-186
-// Vector operations are per-lane arithmetic with the serial vector's
-// per-element formulas (LinearSum's three rounding forms), norms are summed in
-// species order through LDS, so in FP64 each system's result equals the host
-// integrator's bit for bit.
+// Batched per-system BDF on the MI355X (host part): uploads a batch of size
+// buckets, launches one persistent kernel per bucket on its own stream
+// (largest systems first) and gathers results back into system order.  The
+// integrator itself is in bdf_device.h; each (m, precision) kernel is compiled
+// in one of the bdf_inst_*.hip units.
 // -----------------------------------------------------------------------------
 #include <hip/hip_runtime.h>
 
 #include <chrono>
-#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
 #include "../core/profiler_internal.h"
-#include "../core/sunamd_pow.h"
 #include "sunamd/batched_bdf.h"
 #include "sunamd/sunamd_profiler.h"
 #include "sunamd/sunmatrix_mi355block.h"
 
-namespace
-{
+#include "bdf_args.h"
 
-// ---- constants (cvode_impl.h, cvode.c, cvode_ls_impl.h, cvode_nls.c) ------
-constexpr int QMAX             = 5;
-constexpr double ETA_MIN_FX    = 0.0;
-constexpr double ETA_MAX_FX    = 1.5;
-constexpr double ETA_MAX_FS    = 10000.0;
-constexpr double ETA_MAX_ES    = 10.0;
-constexpr double ETA_MAX_GS    = 10.0;
-constexpr double ETA_MIN       = 0.1;
-constexpr double ETA_MAX_EF    = 0.2;
-constexpr double ETA_MIN_EF    = 0.1;
-constexpr double ETA_CF        = 0.25;
-constexpr long SMALL_NST       = 10;
-constexpr int SMALL_NEF        = 2;
-constexpr double ONEPSM        = 1.000001;
-constexpr double ADDON         = 0.000001;
-constexpr double BIAS1         = 6.0;
-constexpr double BIAS2         = 6.0;
-constexpr double BIAS3         = 10.0;
-constexpr int LONG_WAIT        = 10;
-constexpr int MXNCF            = 10;
-constexpr int MXNEF            = 7;
-constexpr int MXNEF1           = 3;
-constexpr long MSBP            = 20;
-constexpr double DGMAX_LSETUP  = 0.3;
-constexpr long MSBJ            = 51;
-constexpr double DGMAX_JBAD    = 0.2;
-constexpr double CORTES        = 0.1;
-constexpr int NLS_MAXCOR       = 3;
-constexpr double CRDOWN        = 0.3;
-constexpr double RDIV          = 2.0;
-constexpr double HLB_FACTOR    = 100.0;
-constexpr double HUB_FACTOR    = 0.1;
-constexpr double H_BIAS        = 0.5;
-constexpr int MAX_ITERS        = 4;
-constexpr double UROUND        = 2.2204460492503131e-16;
-constexpr long MXSTEP          = 1000000;
+using sunamd_bdf::BucketArgs;
+using sunamd_bdf::pow2ceil;
 
-// flags (cvode.h / cvode_impl.h / sundials_nonlinearsolver.h)
-constexpr int CV_SUCCESS          = 0;
-constexpr int CV_TOO_MUCH_WORK    = -1;
-constexpr int CV_TOO_MUCH_ACC     = -2;
-constexpr int CV_ERR_FAILURE      = -3;
-constexpr int CV_CONV_FAILURE     = -4;
-constexpr int CV_LSETUP_FAIL      = -6;
-constexpr int CV_TOO_CLOSE        = -27;
-constexpr int DO_ERROR_TEST       = 2;
-constexpr int PREDICT_AGAIN       = 3;
-constexpr int TRY_AGAIN           = 5;
-constexpr int FIRST_CALL          = 6;
-constexpr int PREV_CONV_FAIL      = 7;
-constexpr int PREV_ERR_FAIL       = 9;
-constexpr int NLS_CONTINUE        = 901;
-constexpr int NLS_CONV_RECVR      = 902;
-constexpr int CV_NO_FAILURES      = 0;
-constexpr int CV_FAIL_BAD_J       = 1;
-constexpr int CV_FAIL_OTHER       = 2;
-constexpr int LUFACT_FAIL         = 808;
-
-__host__ __device__ constexpr int pow2ceil(int m)
-{
-  return m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : m <= 8 ? 8 : m <= 16 ? 16 : m <= 32 ? 32 : 64;
-}
-
-// serial-vector LinearSum rounding: its nine branches reduce to these three
-__device__ __forceinline__ double lsum(double a, double x, double b, double y)
-{
-  if (a == b) { return a * (x + y); }
-  if (a == -b) { return a * (x - y); }
-  return (a * x) + (b * y);
-}
-
-__device__ __forceinline__ double sqrt0(double v) { return (v <= 0.0) ? 0.0 : sqrt(v); }
-
-// wavefront-level ordering point for a group's LDS exchange
-__device__ __forceinline__ void gsync()
-{
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-struct BucketArgs
-{
-  int m, R;
-  long ncells;
-  const int* idx;     // 4R
-  const double* kf;   // ncells*R, cell-major
-  const double* kb;
-  const double* y0;   // ncells*m
-  double* yout;       // ncells*m
-  long* stats;        // ncells*SUNAMD_BS_COUNT
-  unsigned long long* next;
-  double rtol, atol, tout;
-};
-
-// per-group scalar arrays live in LDS (indexed by the order q at run time)
-struct GroupScalars
-{
-  double tau[QMAX + 2];
-  double tq[6];
-  double l[QMAX + 1];
-  double cvals[QMAX + 2];
-};
-
-template <int M, typename T>
-struct Sys
-{
-  // lane-owned vector elements (species i of the group's system)
-  double zn[QMAX + 1];
-  double ewt, acor, tempv, ftemp, y, delta, ysav;
-  // Newton-matrix factor row held by this lane, and its physical row
-  T lu[M];
-  int pos;
-  // integrator scalars (uniform across the group)
-  double tn, h, hscale, hprime, eta, hu, h0u;
-  double rl1, gamma, gammap, gamrat, crate, delp, acnrm;
-  double etaqm1, etaq, etaqp1, etamax, saved_tq5;
-  int q, qprime, qwait, L, indx_acor, convfail, qu;
-  long nst, nfe, ncfn, netf, nni, nnf, nsetups, nstlp, nje, nstlj;
-  bool nls_jcur, acnrmcur;
-};
-
-template <int M, typename T>
-struct Grp
-{
-  int i;       // lane within group (= species when valid)
-  int base;    // first lane of the group in the wavefront
-  bool valid;  // i < M
-  double* red; // G doubles of LDS
-  T* prow;     // G entries of LDS
-  int* inv;    // G ints of LDS
-  GroupScalars* sc;
-  const BucketArgs* a;
-  long cell;
-  const double* kf;
-  const double* kb;
-};
-
-// ---- group reductions (species order = the serial vector's order) --------
-template <int M, typename T>
-__device__ __forceinline__ double g_wrms(const Grp<M, T>& g, double x, double w)
-{
-  double p = x * w;
-  gsync();
-  if (g.valid) { g.red[g.i] = p * p; }
-  gsync();
-  double s = 0.0;
-#pragma unroll
-  for (int k = 0; k < M; k++) { s += g.red[k]; }
-  gsync();
-  return sqrt0(s / (double)M);
-}
-
-template <int M, typename T>
-__device__ __forceinline__ double g_maxabs(const Grp<M, T>& g, double x)
-{
-  gsync();
-  if (g.valid) { g.red[g.i] = fabs(x); }
-  gsync();
-  double mx = 0.0;
-#pragma unroll
-  for (int k = 0; k < M; k++)
-  {
-    double v = g.red[k];
-    if (v > mx) { mx = v; }
-  }
-  gsync();
-  return mx;
-}
-
-// ---- mass-action RHS: f_i accumulated in reaction order ------------------
-template <int M, typename T>
-__device__ __forceinline__ double g_rhs(const Grp<M, T>& g, double yi)
-{
-  gsync();
-  if (g.valid) { g.red[g.i] = yi; }
-  gsync();
-  double f       = 0.0;
-  const int* idx = g.a->idx;
-  const int R    = g.a->R;
-  for (int r = 0; r < R; r++)
-  {
-    const int ia = idx[4 * r], ib = idx[4 * r + 1], ic = idx[4 * r + 2], id = idx[4 * r + 3];
-    if (ia != g.i && ib != g.i && ic != g.i && id != g.i) { continue; }
-    double wf = (g.kf[r] * g.red[ia]) * g.red[ib];
-    double wb = (g.kb[r] * g.red[ic]) * g.red[id];
-    double w  = wf - wb;
-    if (ia == g.i) { f = f - w; }
-    if (ib == g.i) { f = f - w; }
-    if (ic == g.i) { f = f + w; }
-    if (id == g.i) { f = f + w; }
-  }
-  gsync();
-  return f;
-}
-
-// ---- Jacobian row i in reaction / target / column order, then the Newton
-// matrix row (I - gamma*J)_i with the dense ScaleAddI rounding --------------
-template <int M, typename T>
-__device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, double gamma, double (&a)[M])
-{
-  gsync();
-  if (g.valid) { g.red[g.i] = yj; }
-  gsync();
-#pragma unroll
-  for (int j = 0; j < M; j++) { a[j] = 0.0; }
-  const int* idx = g.a->idx;
-  const int R    = g.a->R;
-  for (int r = 0; r < R; r++)
-  {
-    int ix[4] = {idx[4 * r], idx[4 * r + 1], idx[4 * r + 2], idx[4 * r + 3]};
-    if (ix[0] != g.i && ix[1] != g.i && ix[2] != g.i && ix[3] != g.i) { continue; }
-    double kfv   = g.kf[r], kbv = g.kb[r];
-    double dw[4] = {kfv * g.red[ix[1]], kfv * g.red[ix[0]], -(kbv * g.red[ix[3]]),
-                    -(kbv * g.red[ix[2]])};
-    for (int t = 0; t < 4; t++)
-    {
-      if (ix[t] != g.i) { continue; }
-      for (int q = 0; q < 4; q++)
-      {
-        const int col = ix[q];
-        const double v = dw[q];
-#pragma unroll
-        for (int j = 0; j < M; j++)
-        {
-          if (j == col) { a[j] = (t < 2) ? (a[j] - v) : (a[j] + v); }
-        }
-      }
-    }
-  }
-  gsync();
-  const double c = -gamma;
-#pragma unroll
-  for (int j = 0; j < M; j++)
-  {
-    a[j] = a[j] * c;
-    if (j == g.i) { a[j] = a[j] + 1.0; }
-  }
-}
-
-// ---- LU of the group's matrix (rows in lanes, implicit pivoting) ---------
-template <int M, typename T>
-__device__ __forceinline__ int g_getrf(const Grp<M, T>& g, Sys<M, T>& S, const double (&a)[M])
-{
-  constexpr int G = pow2ceil(M);
-#pragma unroll
-  for (int j = 0; j < M; j++) { S.lu[j] = (T)a[j]; }
-  int pos  = g.i;
-  int fail = 0;
-#pragma unroll
-  for (int k = 0; k < M; k++)
-  {
-    T v    = (g.valid && pos >= k) ? (T)fabs((double)S.lu[k]) : (T)-1;
-    int bp = g.valid ? pos : INT_MAX;
-#pragma unroll
-    for (int off = G / 2; off >= 1; off >>= 1)
-    {
-      T ov   = __shfl_xor(v, off, 64);
-      int op = __shfl_xor(bp, off, 64);
-      if (ov > v || (ov == v && op < bp))
-      {
-        v  = ov;
-        bp = op;
-      }
-    }
-    if (v == (T)0 && fail == 0) { fail = k + 1; }
-    if (pos == k) { pos = bp; }
-    else if (pos == bp) { pos = k; }
-    gsync();
-    if (g.valid && pos == k)
-    {
-#pragma unroll
-      for (int j = k; j < M; j++) { g.prow[j] = S.lu[j]; }
-    }
-    gsync();
-    if (g.valid && pos > k && fail == 0)
-    {
-      const T mult = (T)1 / g.prow[k];
-      const T l    = S.lu[k] * mult;
-      S.lu[k]      = l;
-#pragma unroll
-      for (int j = k + 1; j < M; j++)
-      {
-        const T pj = g.prow[j];
-        if (pj != (T)0) { S.lu[j] = S.lu[j] - pj * l; }
-      }
-    }
-  }
-  gsync();
-  if (g.valid) { g.inv[pos] = g.i; }
-  gsync();
-  S.pos = pos;
-  return fail;
-}
-
-// x = (LU)^{-1} P b for b held as one element per lane
-template <int M, typename T>
-__device__ __forceinline__ double g_getrs(const Grp<M, T>& g, const Sys<M, T>& S, double b)
-{
-  // the row swaps carried logical row r to physical row pos_r, and b's
-  // element r to the same position: lane r already holds b'[pos_r]
-  const int pos = S.pos;
-#pragma unroll
-  for (int k = 0; k < M - 1; k++)
-  {
-    double bk = __shfl(b, g.base + g.inv[k], 64);
-    if (pos > k) { b = b - (double)S.lu[k] * bk; }
-  }
-#pragma unroll
-  for (int k = M - 1; k > 0; k--)
-  {
-    if (pos == k) { b = b / (double)S.lu[k]; }
-    double bk = __shfl(b, g.base + g.inv[k], 64);
-    if (pos < k) { b = b - (double)S.lu[k] * bk; }
-  }
-  if (pos == 0) { b = b / (double)S.lu[0]; }
-  // variable i sits in the lane holding physical row i
-  return __shfl(b, g.base + g.inv[g.valid ? g.i : 0], 64);
-}
-
-// ---- integrator pieces -----------------------------------------------------
-template <int M, typename T>
-__device__ __forceinline__ double g_ewt(const Grp<M, T>& g, double yi)
-{
-  double t = fabs(yi);
-  t        = g.a->rtol * t;
-  t        = t + g.a->atol;
-  return 1.0 / t;
-}
-
-// The Nordsieck columns stay in registers only while every access uses a
-// constant index.  A runtime column index is resolved by value: each column is
-// passed through an empty asm so the optimiser cannot turn the choice back
-// into an indexed load from the state struct (which would move the whole
-// integrator state to scratch memory).
-__device__ __forceinline__ double opaque(double x)
-{
-  asm("" : "+v"(x));
-  return x;
-}
-
-template <int M, typename T>
-__device__ __forceinline__ double zget(const Sys<M, T>& S, int j)
-{
-  double v = opaque(S.zn[0]);
-#pragma unroll
-  for (int k = 1; k <= QMAX; k++)
-  {
-    const double zk = opaque(S.zn[k]);
-    v               = (j == k) ? zk : v;
-  }
-  return v;
-}
-
-template <int M, typename T>
-__device__ __forceinline__ void zset(Sys<M, T>& S, int j, double v)
-{
-#pragma unroll
-  for (int k = 0; k <= QMAX; k++) { S.zn[k] = (j == k) ? v : S.zn[k]; }
-}
-
-template <int M, typename T>
-__device__ __forceinline__ void g_rescale(const Grp<M, T>& g, Sys<M, T>& S)
-{
-  double c = S.eta;
-#pragma unroll
-  for (int j = 1; j <= QMAX; j++)
-  {
-    if (j <= S.q) { S.zn[j] = c * S.zn[j]; }
-    c = S.eta * c;
-  }
-  S.h      = S.hscale * S.eta;
-  S.hscale = S.h;
-}
-
-template <int M, typename T>
-__device__ __forceinline__ void g_increase_bdf(const Grp<M, T>& g, Sys<M, T>& S)
-{
-  double* l = g.sc->l;
-  for (int i = 0; i <= QMAX; i++) { l[i] = 0.0; }
-  double alpha0, alpha1, prod, xi, xiold, hsum, A1;
-  l[2] = alpha1 = prod = xiold = 1.0;
-  alpha0                       = -1.0;
-  hsum                         = S.hscale;
-  if (S.q > 1)
-  {
-    for (int j = 1; j < S.q; j++)
-    {
-      hsum += g.sc->tau[j + 1];
-      xi = hsum / S.hscale;
-      prod *= xi;
-      alpha0 -= 1.0 / (j + 1);
-      alpha1 += 1.0 / xi;
-      for (int i = j + 2; i >= 2; i--) { l[i] = l[i] * xiold + l[i - 1]; }
-      xiold = xi;
-    }
-  }
-  A1         = (-alpha0 - alpha1) / prod;
-  double zL  = A1 * zget(S, S.indx_acor);
-  zset(S, S.L, zL);
-#pragma unroll
-  for (int j = 2; j <= QMAX; j++)
-  {
-    if (j <= S.q) { S.zn[j] = lsum(l[j], zL, 1.0, S.zn[j]); }
-  }
-}
-
-template <int M, typename T>
-__device__ __forceinline__ void g_decrease_bdf(const Grp<M, T>& g, Sys<M, T>& S)
-{
-  double* l = g.sc->l;
-  for (int i = 0; i <= QMAX; i++) { l[i] = 0.0; }
-  l[2]        = 1.0;
-  double hsum = 0.0;
-  for (int j = 1; j <= S.q - 2; j++)
-  {
-    hsum += g.sc->tau[j];
-    double xi = hsum / S.hscale;
-    for (int i = j + 2; i >= 2; i--) { l[i] = l[i] * xi + l[i - 1]; }
-  }
-  const double zq = zget(S, S.q);
-#pragma unroll
-  for (int j = 2; j <= QMAX; j++)
-  {
-    if (j < S.q) { S.zn[j] = lsum(-l[j], zq, 1.0, S.zn[j]); }
-  }
-}
-
-template <int M, typename T>
-__device__ __forceinline__ void g_adjust_order(const Grp<M, T>& g, Sys<M, T>& S, int deltaq)
-{
-  if ((S.q == 2) && (deltaq != 1)) { return; }
-  if (deltaq == 1) { g_increase_bdf(g, S); }
-  else if (deltaq == -1) { g_decrease_bdf(g, S); }
-}
-
-template <int M, typename T>
-__device__ __forceinline__ void g_predict(Sys<M, T>& S)
-{
-  S.tn += S.h;
-#pragma unroll
-  for (int k = 1; k <= QMAX; k++)
-  {
-#pragma unroll
-    for (int j = QMAX; j >= 1; j--)
-    {
-      if (k <= S.q && j >= k && j <= S.q) { S.zn[j - 1] = S.zn[j - 1] + S.zn[j]; }
-    }
-  }
-}
-
-template <int M, typename T>
-__device__ __forceinline__ void g_restore(Sys<M, T>& S, double saved_t)
-{
-  S.tn = saved_t;
-#pragma unroll
-  for (int k = 1; k <= QMAX; k++)
-  {
-#pragma unroll
-    for (int j = QMAX; j >= 1; j--)
-    {
-      if (k <= S.q && j >= k && j <= S.q) { S.zn[j - 1] = S.zn[j - 1] - S.zn[j]; }
-    }
-  }
-}
-
-template <int M, typename T>
-__device__ __forceinline__ void g_set(const Grp<M, T>& g, Sys<M, T>& S, long nst)
-{
-  double* l  = g.sc->l;
-  double* tq = g.sc->tq;
-  double alpha0, alpha0_hat, xi_inv, xistar_inv, hsum;
-  l[0] = l[1] = xi_inv = xistar_inv = 1.0;
-  for (int i = 2; i <= S.q; i++) { l[i] = 0.0; }
-  alpha0 = alpha0_hat = -1.0;
-  hsum                = S.h;
-  if (S.q > 1)
-  {
-    for (int j = 2; j < S.q; j++)
-    {
-      hsum += g.sc->tau[j - 1];
-      xi_inv = S.h / hsum;
-      alpha0 -= 1.0 / j;
-      for (int i = j; i >= 1; i--) { l[i] += l[i - 1] * xi_inv; }
-    }
-    alpha0 -= 1.0 / S.q;
-    xistar_inv = -l[1] - alpha0;
-    hsum += g.sc->tau[S.q - 1];
-    xi_inv     = S.h / hsum;
-    alpha0_hat = -l[1] - xi_inv;
-    for (int i = S.q; i >= 1; i--) { l[i] += l[i - 1] * xistar_inv; }
-  }
-  // cvSetTqBDF
-  double A1 = 1.0 - alpha0_hat + alpha0;
-  double A2 = 1.0 + S.q * A1;
-  tq[2]     = fabs(A1 / (alpha0 * A2));
-  tq[5]     = fabs(A2 * xistar_inv / (l[S.q] * xi_inv));
-  if (S.qwait == 1)
-  {
-    if (S.q > 1)
-    {
-      double C     = xistar_inv / l[S.q];
-      double A3    = alpha0 + 1.0 / S.q;
-      double A4    = alpha0_hat + xi_inv;
-      double Cpinv = (1.0 - A4 + A3) / A3;
-      tq[1]        = fabs(C * Cpinv);
-    }
-    else { tq[1] = 1.0; }
-    hsum += g.sc->tau[S.q];
-    xi_inv        = S.h / hsum;
-    double A5     = alpha0 - (1.0 / (S.q + 1));
-    double A6     = alpha0_hat - xi_inv;
-    double Cppinv = (1.0 - A6 + A5) / A2;
-    tq[3]         = fabs(Cppinv / (xi_inv * (S.q + 2) * A5));
-  }
-  tq[4] = CORTES / tq[2];
-  // cvSet
-  S.rl1   = 1.0 / l[1];
-  S.gamma = S.h * S.rl1;
-  if (nst == 0) { S.gammap = S.gamma; }
-  S.gamrat = (nst > 0) ? S.gamma / S.gammap : 1.0;
-}
-
-// ---- Newton pieces ------------------------------------------------------------
-// residual: y = zn0 + ycor; ftemp = f(y); delta = rl1*zn1 + ycor - gamma*ftemp
-template <int M, typename T>
-__device__ __forceinline__ void g_residual(const Grp<M, T>& g, Sys<M, T>& S)
-{
-  S.y     = lsum(1.0, S.zn[0], 1.0, S.acor);
-  S.ftemp = g_rhs(g, S.y);
-  S.nfe++;
-  double res = lsum(S.rl1, S.zn[1], 1.0, S.acor);
-  S.delta    = lsum(-S.gamma, S.ftemp, 1.0, res);
-}
-
-// linear-solver setup (cvNlsLSetup + cvLsSetup + cvLsLinSys + GETRF)
-template <int M, typename T>
-__device__ __forceinline__ int g_lsetup(const Grp<M, T>& g, Sys<M, T>& S, bool jbad_nls)
-{
-  if (jbad_nls) { S.convfail = CV_FAIL_BAD_J; }
-  double dgamma = fabs((S.gamma / S.gammap) - 1.0);
-  bool ls_jbad  = (S.nst == 0) || (S.nst >= S.nstlj + MSBJ) ||
-                 ((S.convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_JBAD)) ||
-                 (S.convfail == CV_FAIL_OTHER);
-  bool jcur;
-  double a[M];
-  if (!ls_jbad)
-  {
-    jcur = false;
-    g_newton_matrix(g, S.ysav, S.gamma, a); // saved J rebuilt from its state
-  }
-  else
-  {
-    jcur   = true;
-    S.ysav = S.y;
-    g_newton_matrix(g, S.ysav, S.gamma, a);
-  }
-  if (jcur)
-  {
-    S.nje++;
-    S.nstlj = S.nst;
-  }
-  int fail = g_getrf(g, S, a);
-  S.nsetups++;
-  S.nls_jcur = jcur;
-  S.gamrat   = 1.0;
-  S.gammap   = S.gamma;
-  S.crate    = 1.0;
-  S.nstlp    = S.nst;
-  return fail ? NLS_CONV_RECVR : CV_SUCCESS;
-}
-
-template <int M, typename T>
-__device__ __forceinline__ void g_lsolve(const Grp<M, T>& g, Sys<M, T>& S)
-{
-  double x = g_getrs(g, S, S.delta);
-  S.delta  = x;
-  if (S.gamrat != 1.0) { S.delta = (2.0 / (1.0 + S.gamrat)) * S.delta; }
-}
-
-template <int M, typename T>
-__device__ __forceinline__ int g_ctest(const Grp<M, T>& g, Sys<M, T>& S, int m, double tol)
-{
-  double del = g_wrms(g, S.delta, S.ewt);
-  if (m > 0)
-  {
-    double r = del / S.delp;
-    S.crate  = (CRDOWN * S.crate > r) ? CRDOWN * S.crate : r;
-  }
-  double dcon = del * ((1.0 < S.crate) ? 1.0 : S.crate) / tol;
-  if (dcon <= 1.0)
-  {
-    S.acnrm    = (m == 0) ? del : g_wrms(g, S.acor, S.ewt);
-    S.acnrmcur = true;
-    return CV_SUCCESS;
-  }
-  if ((m >= 1) && (del > RDIV * S.delp)) { return NLS_CONV_RECVR; }
-  S.delp = del;
-  return NLS_CONTINUE;
-}
-
-template <int M, typename T>
-__device__ __forceinline__ int g_newton(const Grp<M, T>& g, Sys<M, T>& S, bool callLSetup)
-{
-  bool jbad        = false;
-  long niters      = 0;
-  long nconvfails  = 0;
-  int retval       = CV_SUCCESS;
-  const double tol = g.sc->tq[4];
-  for (;;)
-  {
-    int curiter = 0;
-    g_residual(g, S);
-    if (callLSetup)
-    {
-      retval = g_lsetup(g, S, jbad);
-      if (retval != CV_SUCCESS) { break; }
-    }
-    for (;;)
-    {
-      niters++;
-      S.delta = -1.0 * S.delta;
-      g_lsolve(g, S);
-      S.acor = lsum(1.0, S.acor, 1.0, S.delta);
-      retval = g_ctest(g, S, curiter, tol);
-      curiter++;
-      if (retval == CV_SUCCESS)
-      {
-        S.nls_jcur = false;
-        S.nni += niters;
-        S.nnf += nconvfails;
-        return CV_SUCCESS;
-      }
-      if (retval != NLS_CONTINUE) { break; }
-      if (curiter >= NLS_MAXCOR)
-      {
-        retval = NLS_CONV_RECVR;
-        break;
-      }
-      g_residual(g, S);
-    }
-    if ((retval > 0) && !S.nls_jcur)
-    {
-      nconvfails++;
-      callLSetup = true;
-      jbad       = true;
-      S.acor     = 0.0;
-      continue;
-    }
-    break;
-  }
-  nconvfails++;
-  S.nni += niters;
-  S.nnf += nconvfails;
-  return retval;
-}
-
-template <int M, typename T>
-__device__ __forceinline__ int g_nls(const Grp<M, T>& g, Sys<M, T>& S, int nflag)
-{
-  S.convfail = ((nflag == FIRST_CALL) || (nflag == PREV_ERR_FAIL)) ? CV_NO_FAILURES : CV_FAIL_OTHER;
-  bool callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (S.nst == 0) ||
-                   (S.nst >= S.nstlp + MSBP) || (fabs(S.gamrat - 1.0) > DGMAX_LSETUP);
-  S.acor   = 0.0;
-  int flag = g_newton(g, S, callSetup);
-  if (flag != CV_SUCCESS) { return flag; }
-  S.y = lsum(1.0, S.zn[0], 1.0, S.acor);
-  if (!S.acnrmcur) { S.acnrm = g_wrms(g, S.acor, S.ewt); }
-  return flag;
-}
-
-template <int M, typename T>
-__device__ __forceinline__ int g_handle_nflag(const Grp<M, T>& g, Sys<M, T>& S, int& nflag, double saved_t,
-                              int& ncf)
-{
-  if (nflag == CV_SUCCESS) { return DO_ERROR_TEST; }
-  S.ncfn++;
-  g_restore(S, saved_t);
-  if (nflag < 0) { return CV_LSETUP_FAIL; }
-  ncf++;
-  S.etamax = 1.0;
-  if ((fabs(S.h) <= 0.0 * ONEPSM) || (ncf == MXNCF))
-  {
-    if (nflag == NLS_CONV_RECVR) { return CV_CONV_FAILURE; }
-  }
-  S.eta = (ETA_CF > 0.0) ? ETA_CF : 0.0;
-  nflag = PREV_CONV_FAIL;
-  g_rescale(g, S);
-  return PREDICT_AGAIN;
-}
-
-template <int M, typename T>
-__device__ __forceinline__ int g_error_test(const Grp<M, T>& g, Sys<M, T>& S, int& nflag, double saved_t,
-                            int& nef, double& dsm_out)
-{
-  double dsm = S.acnrm * g.sc->tq[2];
-  dsm_out    = dsm;
-  if (dsm <= 1.0) { return CV_SUCCESS; }
-  nef++;
-  S.netf++;
-  nflag = PREV_ERR_FAIL;
-  g_restore(S, saved_t);
-  if ((fabs(S.h) <= 0.0 * ONEPSM) || (nef == MXNEF)) { return CV_ERR_FAILURE; }
-  S.etamax = 1.0;
-  if (nef <= MXNEF1)
-  {
-    S.eta = 1.0 / (sunamd_pow(BIAS2 * dsm, 1.0 / S.L) + ADDON);
-    double e2 = (S.eta > 0.0) ? S.eta : 0.0;
-    S.eta     = (ETA_MIN_EF > e2) ? ETA_MIN_EF : e2;
-    if (nef >= SMALL_NEF) { S.eta = (S.eta < ETA_MAX_EF) ? S.eta : ETA_MAX_EF; }
-    g_rescale(g, S);
-    return TRY_AGAIN;
-  }
-  if (S.q > 1)
-  {
-    S.eta = (ETA_MIN_EF > 0.0) ? ETA_MIN_EF : 0.0;
-    g_adjust_order(g, S, -1);
-    S.L = S.q;
-    S.q--;
-    S.qwait = S.L;
-    g_rescale(g, S);
-    return TRY_AGAIN;
-  }
-  S.eta    = (ETA_MIN_EF > 0.0) ? ETA_MIN_EF : 0.0;
-  S.h *= S.eta;
-  S.hscale = S.h;
-  S.qwait  = LONG_WAIT;
-  S.tempv  = g_rhs(g, S.zn[0]);
-  S.nfe++;
-  S.zn[1] = S.h * S.tempv;
-  return TRY_AGAIN;
-}
-
-template <int M, typename T>
-__device__ __forceinline__ void g_complete_step(const Grp<M, T>& g, Sys<M, T>& S)
-{
-  S.nst++;
-  S.hu       = S.h;
-  S.qu       = S.q;
-  double* tau = g.sc->tau;
-  for (int i = S.q; i >= 2; i--) { tau[i] = tau[i - 1]; }
-  if ((S.q == 1) && (S.nst > 1)) { tau[2] = tau[1]; }
-  tau[1] = S.h;
-  const double* l = g.sc->l;
-#pragma unroll
-  for (int j = 0; j <= QMAX; j++)
-  {
-    if (j <= S.q) { S.zn[j] = lsum(l[j], S.acor, 1.0, S.zn[j]); }
-  }
-  S.qwait--;
-  if ((S.qwait == 1) && (S.q != QMAX))
-  {
-    S.zn[QMAX]  = S.acor;
-    S.saved_tq5 = g.sc->tq[5];
-    S.indx_acor = QMAX;
-  }
-}
-
-template <int M, typename T>
-__device__ __forceinline__ void g_set_eta(Sys<M, T>& S)
-{
-  if ((S.eta > ETA_MIN_FX) && (S.eta < ETA_MAX_FX))
-  {
-    S.eta    = 1.0;
-    S.hprime = S.h;
-  }
-  else
-  {
-    if (S.eta >= ETA_MAX_FX)
-    {
-      S.eta = (S.eta < S.etamax) ? S.eta : S.etamax;
-      double d = fabs(S.h) * 0.0 * S.eta;
-      S.eta /= (1.0 > d) ? 1.0 : d;
-    }
-    else
-    {
-      S.eta = (S.eta > ETA_MIN) ? S.eta : ETA_MIN;
-      S.eta = (S.eta > 0.0) ? S.eta : 0.0;
-    }
-    S.hprime = S.h * S.eta;
-  }
-}
-
-template <int M, typename T>
-__device__ __forceinline__ void g_prepare_next(const Grp<M, T>& g, Sys<M, T>& S, double dsm)
-{
-  if (S.etamax == 1.0)
-  {
-    S.qwait  = (S.qwait > 2) ? S.qwait : 2;
-    S.qprime = S.q;
-    S.hprime = S.h;
-    S.eta    = 1.0;
-    return;
-  }
-  S.etaq = 1.0 / (sunamd_pow(BIAS2 * dsm, 1.0 / S.L) + ADDON);
-  if (S.qwait != 0)
-  {
-    S.eta    = S.etaq;
-    S.qprime = S.q;
-    g_set_eta(S);
-    return;
-  }
-  S.qwait = 2;
-  // cvComputeEtaqm1
-  S.etaqm1 = 0.0;
-  if (S.q > 1)
-  {
-    double ddn = g_wrms(g, zget(S, S.q), S.ewt) * g.sc->tq[1];
-    S.etaqm1   = 1.0 / (sunamd_pow(BIAS1 * ddn, 1.0 / S.q) + ADDON);
-  }
-  // cvComputeEtaqp1
-  S.etaqp1 = 0.0;
-  if (S.q != QMAX && S.saved_tq5 != 0.0)
-  {
-    double hr = S.h / g.sc->tau[2];
-    double pw = 1.0;
-    for (int i = 1; i <= S.L; i++) { pw *= hr; } // SUNRpowerI
-    double cquot = (g.sc->tq[5] / S.saved_tq5) * pw;
-    S.tempv      = lsum(-cquot, S.zn[QMAX], 1.0, S.acor);
-    double dup   = g_wrms(g, S.tempv, S.ewt) * g.sc->tq[3];
-    S.etaqp1     = 1.0 / (sunamd_pow(BIAS3 * dup, 1.0 / (S.L + 1)) + ADDON);
-  }
-  // cvChooseEta
-  double em   = (S.etaq > S.etaqp1) ? S.etaq : S.etaqp1;
-  double etam = (S.etaqm1 > em) ? S.etaqm1 : em;
-  if ((etam > ETA_MIN_FX) && (etam < ETA_MAX_FX))
-  {
-    S.eta    = 1.0;
-    S.qprime = S.q;
-  }
-  else if (etam == S.etaq)
-  {
-    S.eta    = S.etaq;
-    S.qprime = S.q;
-  }
-  else if (etam == S.etaqm1)
-  {
-    S.eta    = S.etaqm1;
-    S.qprime = S.q - 1;
-  }
-  else
-  {
-    S.eta      = S.etaqp1;
-    S.qprime   = S.q + 1;
-    S.zn[QMAX] = S.acor;
-  }
-  g_set_eta(S);
-}
-
-template <int M, typename T>
-__device__ __forceinline__ int g_step(const Grp<M, T>& g, Sys<M, T>& S)
-{
-  int ncf = 0, nef = 0;
-  double dsm = 0.0;
-  if ((S.nst > 0) && (S.hprime != S.h))
-  {
-    if (S.qprime != S.q)
-    {
-      g_adjust_order(g, S, S.qprime - S.q);
-      S.q     = S.qprime;
-      S.L     = S.q + 1;
-      S.qwait = S.L;
-    }
-    g_rescale(g, S);
-  }
-  const double saved_t = S.tn;
-  int nflag            = FIRST_CALL;
-  for (;;)
-  {
-    g_predict(S);
-    g_set(g, S, S.nst);
-    nflag     = g_nls(g, S, nflag);
-    int kflag = g_handle_nflag(g, S, nflag, saved_t, ncf);
-    if (kflag == PREDICT_AGAIN) { continue; }
-    if (kflag != DO_ERROR_TEST) { return kflag; }
-    int eflag = g_error_test(g, S, nflag, saved_t, nef, dsm);
-    if (eflag == TRY_AGAIN) { continue; }
-    if (eflag != CV_SUCCESS) { return eflag; }
-    break;
-  }
-  g_complete_step(g, S);
-  g_prepare_next(g, S, dsm);
-  S.etamax = (S.nst <= SMALL_NST) ? ETA_MAX_ES : ETA_MAX_GS;
-  S.acor   = g.sc->tq[2] * S.acor;
-  return CV_SUCCESS;
-}
-
-// initial step (cvHin with cvUpperBoundH0 and cvYddNorm)
-template <int M, typename T>
-__device__ __forceinline__ int g_hin(const Grp<M, T>& g, Sys<M, T>& S, double tout)
-{
-  double tdiff = tout - S.tn;
-  if (tdiff == 0.0) { return CV_TOO_CLOSE; }
-  int sign      = (tdiff > 0.0) ? 1 : -1;
-  double tdist  = fabs(tdiff);
-  double tround = UROUND * ((fabs(S.tn) > fabs(tout)) ? fabs(S.tn) : fabs(tout));
-  if (tdist < 2.0 * tround) { return CV_TOO_CLOSE; }
-  double hlb = HLB_FACTOR * tround;
-  // cvUpperBoundH0
-  double temp2  = fabs(S.zn[0]);
-  double temp1  = g_ewt(g, S.zn[0]);
-  temp1         = 1.0 / temp1;
-  temp1         = lsum(HUB_FACTOR, temp2, 1.0, temp1);
-  temp2         = fabs(S.zn[1]);
-  temp1         = temp2 / temp1;
-  double hubinv = g_maxabs(g, temp1);
-  double hub    = HUB_FACTOR * tdist;
-  if (hub * hubinv > 1.0) { hub = 1.0 / hubinv; }
-  double hg = sqrt0(hlb * hub);
-  if (hub < hlb)
-  {
-    S.h = (sign == -1) ? -hg : hg;
-    return CV_SUCCESS;
-  }
-  double hs = hg, hnew = 0.0, yddnrm = 0.0;
-  for (int count1 = 1; count1 <= MAX_ITERS; count1++)
-  {
-    // cvYddNorm (the RHS never fails here)
-    double hgs = hg * sign;
-    S.y        = lsum(hgs, S.zn[1], 1.0, S.zn[0]);
-    S.tempv    = g_rhs(g, S.y);
-    S.nfe++;
-    S.tempv = lsum(1.0 / hgs, S.tempv, -1.0 / hgs, S.zn[1]);
-    yddnrm  = g_wrms(g, S.tempv, S.ewt);
-    hs      = hg;
-    hnew    = (yddnrm * hub * hub > 2.0) ? sqrt0(2.0 / yddnrm) : sqrt0(hg * hub);
-    if (count1 == MAX_ITERS) { break; }
-    double hrat = hnew / hg;
-    if ((hrat > 0.5) && (hrat < 2.0)) { break; }
-    if ((count1 > 1) && (hrat > 2.0))
-    {
-      hnew = hg;
-      break;
-    }
-    hg = hnew;
-  }
-  (void)hs;
-  double h0 = H_BIAS * hnew;
-  if (h0 < hlb) { h0 = hlb; }
-  if (h0 > hub) { h0 = hub; }
-  if (sign == -1) { h0 = -h0; }
-  S.h = h0;
-  return CV_SUCCESS;
-}
-
-// results of a finished system: state at tout (or where it stopped) and counters
-template <int M, typename T>
-__device__ __forceinline__ void g_store(const Grp<M, T>& g, const Sys<M, T>& S, double yout, int flag)
-{
-  const BucketArgs& A = *g.a;
-  if (g.valid) { A.yout[g.cell * M + g.i] = yout; }
-  if (g.i == 0)
-  {
-    long* st              = A.stats + g.cell * SUNAMD_BS_COUNT;
-    st[SUNAMD_BS_NST]     = S.nst;
-    st[SUNAMD_BS_NFE]     = S.nfe;
-    st[SUNAMD_BS_NSETUPS] = S.nsetups;
-    st[SUNAMD_BS_NETF]    = S.netf;
-    st[SUNAMD_BS_NNI]     = S.nni;
-    st[SUNAMD_BS_NNF]     = S.nnf;
-    st[SUNAMD_BS_NCFN]    = S.ncfn;
-    st[SUNAMD_BS_NJE]     = S.nje;
-    st[SUNAMD_BS_QLAST]   = S.qu;
-    st[SUNAMD_BS_FLAG]    = flag;
-  }
-}
-
-// Take the next system from the bucket's queue and run CVode's first-call
-// work on it (CVodeInit state, error weights, f(t0,y0), initial step).
-// Returns false once the queue is empty.
-template <int M, typename T>
-__device__ __forceinline__ bool g_acquire(Grp<M, T>& g, Sys<M, T>& S, double tout)
-{
-  const BucketArgs& A = *g.a;
-  for (;;)
-  {
-    unsigned long long ticket = 0;
-    if (g.i == 0) { ticket = atomicAdd(A.next, 1ULL); }
-    ticket = __shfl(ticket, g.base, 64);
-    if (ticket >= (unsigned long long)A.ncells) { return false; }
-    g.cell = (long)ticket;
-    g.kf   = A.kf + g.cell * A.R;
-    g.kb   = A.kb + g.cell * A.R;
-
-    S.zn[0] = g.valid ? A.y0[g.cell * M + g.i] : 0.0;
-#pragma unroll
-    for (int j = 1; j <= QMAX; j++) { S.zn[j] = 0.0; }
-    S.acor = S.tempv = S.ftemp = S.y = S.delta = S.ysav = 0.0;
-    S.tn       = 0.0;
-    S.h        = 0.0;
-    S.q        = 1;
-    S.L        = 2;
-    S.qwait    = S.L;
-    S.etamax   = ETA_MAX_FS;
-    S.qu       = 0;
-    S.hu       = 0.0;
-    S.nst = S.nfe = S.ncfn = S.netf = S.nni = S.nnf = S.nsetups = S.nstlp = 0;
-    S.nje = S.nstlj = 0;
-    S.saved_tq5 = 0.0;
-    S.indx_acor = 0;
-    S.acnrmcur  = false;
-    S.nls_jcur  = false;
-    S.crate = S.delp = S.acnrm = S.gamma = S.gammap = S.gamrat = S.rl1 = 0.0;
-    S.etaq = S.etaqm1 = S.etaqp1 = S.eta = S.hprime = 0.0;
-    S.qprime   = 1;
-    S.convfail = CV_NO_FAILURES;
-    S.pos      = g.i;
-    for (int k = 0; k < QMAX + 2; k++) { g.sc->tau[k] = 0.0; }
-
-    S.ewt   = g_ewt(g, S.zn[0]);
-    S.zn[1] = g_rhs(g, S.zn[0]);
-    S.nfe++;
-    int flag = g_hin(g, S, tout);
-    if (flag == CV_SUCCESS)
-    {
-      S.hscale = S.h;
-      S.h0u    = S.h;
-      S.hprime = S.h;
-      S.zn[1]  = S.h * S.zn[1];
-      return true;
-    }
-    g_store(g, S, S.zn[0], flag); // initial-step failure: y is still y0
-  }
-}
-
-// One pass of CVode's main loop (CV_NORMAL): error weights, the work and
-// accuracy guards, one step, and the tout check with interpolation.
-// Returns true when the system is finished (yout and flag set).
-template <int M, typename T>
-__device__ __forceinline__ bool g_advance(const Grp<M, T>& g, Sys<M, T>& S, double tout, double& yout, int& flag)
-{
-  if (S.nst > 0) { S.ewt = g_ewt(g, S.zn[0]); }
-  if (S.nst >= MXSTEP) // one CVode call: steps taken here = nst
-  {
-    flag = CV_TOO_MUCH_WORK;
-    yout = S.zn[0];
-    return true;
-  }
-  double nrm = g_wrms(g, S.zn[0], S.ewt);
-  if (UROUND * nrm > 1.0)
-  {
-    flag = CV_TOO_MUCH_ACC;
-    yout = S.zn[0];
-    return true;
-  }
-  int kflag = g_step(g, S);
-  if (kflag != CV_SUCCESS)
-  {
-    flag = kflag;
-    yout = S.zn[0];
-    return true;
-  }
-  if ((S.tn - tout) * S.h >= 0.0)
-  {
-    // CVodeGetDky(tout, 0) through the unfused LinearCombination:
-    // z = c_q zn[q], then z = c_j zn[j] + z for j = q-1 .. 0, c_j = s^j
-    double s = (tout - S.tn) / S.h;
-    double z = 0.0;
-    for (int j = S.q; j >= 0; j--)
-    {
-      double c = 1.0;
-      for (int i = 0; i < j; i++) { c *= s; }
-      double zj = zget(S, j);
-      z         = (j == S.q) ? c * zj : lsum(c, zj, 1.0, z);
-    }
-    yout = z;
-    flag = CV_SUCCESS;
-    return true;
-  }
-  return false;
-}
-
-// One wavefront runs 64/G groups.  A group that finishes its system takes
-// the next one from the queue between steps, so the wavefront's groups stay
-// busy while their neighbours' systems need more steps.
-template <int M, typename T>
-__global__ void __launch_bounds__(64) k_batched_bdf(BucketArgs A)
-{
-  constexpr int G   = pow2ceil(M);
-  constexpr int GPW = 64 / G;
-  __shared__ double s_red[64];
-  __shared__ T s_prow[64];
-  __shared__ int s_inv[64];
-  __shared__ GroupScalars s_sc[GPW];
-
-  const int lane = threadIdx.x;
-  const int grp  = lane / G;
-  Grp<M, T> g;
-  g.i     = lane % G;
-  g.base  = grp * G;
-  g.valid = g.i < M;
-  g.red   = s_red + g.base;
-  g.prow  = s_prow + g.base;
-  g.inv   = s_inv + g.base;
-  g.sc    = &s_sc[grp];
-  g.a     = &A;
-  g.cell  = 0;
-  g.kf    = A.kf;
-  g.kb    = A.kb;
-  Sys<M, T> S;
-
-  bool busy = g_acquire(g, S, A.tout);
-  while (busy)
-  {
-    double yout = 0.0;
-    int flag    = 0;
-    if (g_advance(g, S, A.tout, yout, flag))
-    {
-      g_store(g, S, yout, flag);
-      busy = g_acquire(g, S, A.tout);
-    }
-  }
-}
-
-} // namespace
+static constexpr long MXSTEP_DEFAULT = 500; // CVODE's default step limit (cvode_impl.h)
 
 // -----------------------------------------------------------------------------
 // host side
@@ -1174,6 +43,7 @@ struct SUNAMD_BatchedBDF_
   int nb;
   BucketDev b[SUNAMD_BATCH_MAXB];
   double rtol, atol;
+  long mxstep;
   int prec;
   int num_cus;
   // system-order mapping
@@ -1194,7 +64,8 @@ static int hfail(hipError_t e, const char* what)
 #define BDF_SIZES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(16) X(24) X(32) X(48) X(64)
 
 template <typename T>
-static bool launch_bucket(const BucketDev& d, double rtol, double atol, double tout, int grid)
+static bool launch_bucket(const BucketDev& d, double rtol, double atol, double tout, long mxstep,
+                          int grid)
 {
   BucketArgs A;
   A.m      = d.m;
@@ -1210,10 +81,11 @@ static bool launch_bucket(const BucketDev& d, double rtol, double atol, double t
   A.rtol   = rtol;
   A.atol   = atol;
   A.tout   = tout;
+  A.mxstep = mxstep;
   switch (d.m)
   {
 #define BDF_CASE(MM)                                                             \
-  case MM: k_batched_bdf<MM, T><<<grid, 64, 0, d.stream>>>(A); return true;
+  case MM: sunamd_bdf::launch<MM, T>(A, grid, d.stream); return true;
     BDF_SIZES(BDF_CASE)
 #undef BDF_CASE
   default: return false;
@@ -1230,6 +102,7 @@ extern "C" SUNAMD_BatchedBDF* SUNAMD_BatchedBDFCreate(const SUNAMD_Batch* batch,
   B->rtol = rtol;
   B->atol = atol;
   B->prec = lu_precision;
+  B->mxstep = MXSTEP_DEFAULT;
   int dev = 0;
   hipDeviceProp_t prop;
   B->num_cus = 256;
@@ -1299,6 +172,13 @@ extern "C" SUNAMD_BatchedBDF* SUNAMD_BatchedBDFCreate(const SUNAMD_Batch* batch,
   return B;
 }
 
+extern "C" int SUNAMD_BatchedBDFSetMaxNumSteps(SUNAMD_BatchedBDF* B, long int mxsteps)
+{
+  if (!B) { return -1; }
+  B->mxstep = (mxsteps == 0) ? MXSTEP_DEFAULT : mxsteps;
+  return 0;
+}
+
 extern "C" int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* wall_s)
 {
   if (hipDeviceSynchronize() != hipSuccess) { return -1; }
@@ -1329,8 +209,8 @@ extern "C" int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* w
     long cap      = (long)B->num_cus * 16;
     int grid      = (int)(want < cap ? want : cap);
     sunamd_prof_begin(SUNAMD_K_BDF_BATCH, d.stream);
-    bool ok = (B->prec == SUNAMD_LU_FP32) ? launch_bucket<float>(d, B->rtol, B->atol, tout, grid)
-                                          : launch_bucket<double>(d, B->rtol, B->atol, tout, grid);
+    bool ok = (B->prec == SUNAMD_LU_FP32) ? launch_bucket<float>(d, B->rtol, B->atol, tout, B->mxstep, grid)
+                                          : launch_bucket<double>(d, B->rtol, B->atol, tout, B->mxstep, grid);
     sunamd_prof_end(SUNAMD_K_BDF_BATCH, d.stream);
     if (!ok)
     {

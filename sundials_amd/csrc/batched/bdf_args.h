@@ -1,0 +1,40 @@
+// -----------------------------------------------------------------------------
+// Interface between the batched BDF host code and its per-size kernels.
+// -----------------------------------------------------------------------------
+#ifndef SUNAMD_BDF_ARGS_H
+#define SUNAMD_BDF_ARGS_H
+
+#include <hip/hip_runtime.h>
+
+namespace sunamd_bdf
+{
+
+__host__ __device__ constexpr int pow2ceil(int m)
+{
+  return m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : m <= 8 ? 8 : m <= 16 ? 16 : m <= 32 ? 32 : 64;
+}
+
+// one size bucket as the kernel sees it
+struct BucketArgs
+{
+  int m, R;
+  long ncells;
+  const int* idx;     // 4R
+  const double* kf;   // ncells*R, cell-major
+  const double* kb;
+  const double* y0;   // ncells*m
+  double* yout;       // ncells*m
+  long* stats;        // ncells*SUNAMD_BS_COUNT
+  unsigned long long* next;
+  double rtol, atol, tout;
+  long mxstep;
+};
+
+// launches the persistent kernel for systems of M species (T: LU precision);
+// defined in bdf_device.h, instantiated in the bdf_inst_*.hip units
+template <int M, typename T>
+void launch(const BucketArgs& A, int grid, hipStream_t stream);
+
+} // namespace sunamd_bdf
+
+#endif

@@ -31,12 +31,16 @@ def _out(b):
     return y, cnt
 
 
-def run_device(b, precision, rtol=RTOL, atol=ATOL, tout=TOUT):
+MXSTEP = 100000  # the oracle's limit is 10^6; no test system comes near either
+
+
+def run_device(b, precision, rtol=RTOL, atol=ATOL, tout=TOUT, mxstep=MXSTEP):
     """Integrate the batch with the persistent device integrator."""
     hip = _lib.hip()
     B = hip.SUNAMD_BatchedBDFCreate(b, rtol, atol, precision)
     assert B, "SUNAMD_BatchedBDFCreate failed"
     try:
+        assert hip.SUNAMD_BatchedBDFSetMaxNumSteps(B, mxstep) == 0
         wall = C.c_double(0.0)
         rc = hip.SUNAMD_BatchedBDFRun(B, tout, C.byref(wall))
         assert rc == 0, "SUNAMD_BatchedBDFRun returned %d" % rc
