@@ -1,21 +1,29 @@
 #!/usr/bin/env python3
-"""Benchmark: plain-FP64 BDF integration of the 2^20-cell, 32-species
-mass-action batch (48 reversible reactions, rtol 1e-6, atol 1e-10,
-t in [0, 1]) as one coupled system on one MI355X, through the HIP N_Vector,
-block-diagonal matrix and batched block LU.
+"""Benchmarks on one MI355X.
 
-A "step" is one full integration t = 0 -> 1 of the whole batch, with the
-problem, vectors, matrices and integrator already resident in HBM (each step
-resets y to y0 on the device).  value = cell-steps/s = cells * nst / time.
+Default workload (the north-star metric): an uneven batch of 65,536
+independent stiff mass-action systems, m drawn from {4, 8, 16, 24, 32, 48, 64}
+with weights 30/25/15/10/10/5/5 %, R = 1.5 m reversible reactions, rtol 1e-6,
+atol 1e-10, t in [0, 1].  Every system is integrated by its own BDF /
+modified-Newton / dense-LU integrator with its own step size and order, all
+systems in one persistent launch per size bucket.  A "step" is one
+integration t = 0 -> 1 of the whole batch with the problem resident in HBM.
+value = cell-steps/s = (sum over systems of nst) / time, for the product
+form (FP32 Newton-matrix LU, FP64 residuals / updates / norms); the plain
+FP64 form is timed beside it, with Newton iterations per step for both and
+the FP32 form's final-state error against the FP64 run (which equals the
+CPU oracle bit for bit).
 
-Also reported:
-  roofline     the dominant kernel family (largest share of GPU time over the
-               timed steps, measured with HIP events on its launch stream):
-               algorithmic bytes per launch / average launch duration, vs the
-               8 TB/s HBM peak;
-  microbench   N_Vector LinearSum, WrmsNorm, DotProdMulti(8) on 2^28 FP64;
-  cpu_baseline the CPU oracle (serial vector + block-dense LU + the same
-               integrator) on a bounded sample of the same workload.
+  roofline     the longest-running bucket kernel: algorithmic flops per
+               launch (counted from the per-system counters) / its launch
+               duration (HIP events on its stream), vs the FP64 peak;
+  cpu_baseline the CPU oracle (serial vector + dense LU + the same
+               integrator, one CVode run per system, OpenMP over systems) on
+               the first systems of the same batch.
+
+--workload coupled keeps the earlier benchmark: the 2^20-cell, 32-species
+batch as one coupled system through the HIP N_Vector, block-diagonal matrix
+and batched block LU, plus the N_Vector microbench at N = 2^28.
 """
 import argparse
 import ctypes as C
@@ -31,8 +39,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
 M_SPECIES, R_REACTIONS, SEED = 32, 48, 1002
 RTOL, ATOL, TOUT = 1e-6, 1e-10, 1.0
+
+# uneven batch
+BATCH_M = (4, 8, 16, 24, 32, 48, 64)
+BATCH_W = (30.0, 25.0, 15.0, 10.0, 10.0, 5.0, 5.0)
+BATCH_SEED = 1004
+BATCH_MXSTEP = 100000
+LU_FP64, LU_FP32 = 0, 1
 
 # algorithmic bytes per launch of each timed kernel family
 FAMILIES = {0: "getrs", 1: "getrf", 2: "linearsum", 3: "wrmsnorm", 4: "dotprodmulti",
@@ -82,32 +98,8 @@ def cpu_baseline(core, cells):
                                                            os.cpu_count() or 0)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cells", type=int, default=1 << 20)
-    ap.add_argument("--lu", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--micro-n", type=int, default=1 << 28)
-    ap.add_argument("--cpu-cells", type=int, default=1024)
-    ap.add_argument("--no-micro", action="store_true")
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(cells are independent; see DESIGN.md for how the batch would shard)",
-              file=sys.stderr)
-        sys.exit(2)
-
+def bench_coupled(args):
     import torch
-
-    if not torch.cuda.is_available():
-        print("bench.py needs a GPU", file=sys.stderr)
-        sys.exit(1)
-    torch.cuda.set_device(0)
-
     from sundials_amd import _lib
 
     core, hip = _lib.core(), _lib.hip()
@@ -209,6 +201,212 @@ def main():
         "kernels": kernels,
         "microbench": micro,
     }
+    return line
+
+
+def batch_flops(cnt, msys, rsys):
+    """Algorithmic flops of a batched integration from per-system counters:
+    RHS 9R per evaluation, Jacobian 20R per evaluation, Newton matrix m^2+m and
+    LU (2/3)m^3 - m^2/2 - m/6 per setup, GETRS 2m^2 plus 10m of vector work per
+    Newton iteration, 30m of vector work per step."""
+    m = msys.astype(np.float64)
+    R = rsys.astype(np.float64)
+    nst, nfe, nset, nni, nje = (cnt[:, k].astype(np.float64) for k in (0, 1, 2, 4, 7))
+    lu = (2.0 / 3.0) * m ** 3 - 0.5 * m ** 2 - m / 6.0
+    return (9 * R * nfe + 20 * R * nje + (m * m + m + lu) * nset + (2 * m * m + 10 * m) * nni
+            + 30 * m * nst)
+
+
+def batch_cpu_baseline(nsys_sample, threads):
+    from sundials_amd import _lib
+
+    core, orc = _lib.core(), _lib.load_oracle()
+    n = len(BATCH_M)
+    b = core.SUNAMD_BatchCreate(nsys_sample, BATCH_SEED, n, (C.c_int * n)(*BATCH_M),
+                                (C.c_double * n)(*BATCH_W), 1.5, 0.2)
+    bc = b.contents
+    y = np.zeros(bc.ytotal)
+    cnt = np.zeros((bc.nsys, _lib.BS_COUNT), dtype=np.int64)
+    wall = C.c_double()
+    rc = orc.oracle_batch_solve(b, RTOL, ATOL, TOUT, threads,
+                                y.ctypes.data_as(C.POINTER(C.c_double)),
+                                cnt.ctypes.data_as(C.POINTER(C.c_long)), C.byref(wall))
+    core.SUNAMD_BatchFree(b)
+    if rc != 0:
+        return None
+    nst = int(cnt[:, 0].sum())
+    return {"value": nst / wall.value, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": "CPU oracle (serial vector + dense LU, one CVode run per system, OpenMP "
+                      "over systems) on the first %d of the 65,536 systems (%d cell-steps, "
+                      "%.1f s, %s, nproc=%d)" % (nsys_sample, nst, wall.value, cpu_model(),
+                                                 os.cpu_count() or 0)}
+
+
+def bench_batched(args):
+    import torch
+    from sundials_amd import _lib
+
+    core, hip = _lib.core(), _lib.hip()
+    n = len(BATCH_M)
+    print("[bench] generating %d systems" % args.systems, file=sys.stderr, flush=True)
+    b = core.SUNAMD_BatchCreate(args.systems, BATCH_SEED, n, (C.c_int * n)(*BATCH_M),
+                                (C.c_double * n)(*BATCH_W), 1.5, 0.2)
+    bc = b.contents
+    nb = bc.nbuckets
+    bm = np.array([bc.bucket[k].contents.m for k in range(nb)])
+    bR = np.array([bc.bucket[k].contents.R for k in range(nb)])
+    sb = np.ctypeslib.as_array(bc.sys_bucket, shape=(bc.nsys,)).copy()
+    msys, rsys = bm[sb], bR[sb]
+
+    def make(prec):
+        B = hip.SUNAMD_BatchedBDFCreate(b, RTOL, ATOL, prec)
+        if not B:
+            print("[bench] device setup failed", file=sys.stderr)
+            sys.exit(1)
+        hip.SUNAMD_BatchedBDFSetMaxNumSteps(B, BATCH_MXSTEP)
+        return B
+
+    def results(B):
+        y = np.zeros(bc.ytotal)
+        cnt = np.zeros((bc.nsys, _lib.BS_COUNT), dtype=np.int64)
+        rc = hip.SUNAMD_BatchedBDFGetResults(B, y.ctypes.data_as(C.POINTER(C.c_double)),
+                                             cnt.ctypes.data_as(C.POINTER(C.c_long)))
+        if rc != 0:
+            print("[bench] some systems failed", file=sys.stderr)
+            sys.exit(1)
+        return y, cnt
+
+    def run(B, tag):
+        wall = C.c_double()
+        rc = hip.SUNAMD_BatchedBDFRun(B, TOUT, C.byref(wall))
+        if rc != 0:
+            print("[bench] %s run failed rc=%d" % (tag, rc), file=sys.stderr)
+            sys.exit(1)
+        return wall.value
+
+    # plain FP64 form: warm up, then time one integration
+    B64 = make(LU_FP64)
+    run(B64, "fp64 warmup")
+    t64 = run(B64, "fp64")
+    y64, c64 = results(B64)
+    hip.SUNAMD_BatchedBDFFree(B64)
+    nst64 = int(c64[:, 0].sum())
+    print("[bench] plain fp64: %.3f s, %d cell-steps" % (t64, nst64), file=sys.stderr, flush=True)
+
+    # product form: FP32 Newton-matrix LU
+    B = make(LU_FP32)
+    for w in range(args.warmup):
+        tw = run(B, "warmup")
+        print("[bench] warmup %d: %.3f s" % (w, tw), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    walls = []
+    for k in range(args.steps):
+        walls.append(run(B, "step %d" % k))
+        print("[bench] step %d: %.3f s" % (k, walls[-1]), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    y32, c32 = results(B)
+    kms = np.zeros(nb)
+    hip.SUNAMD_BatchedBDFGetKernelTimes(B, kms.ctypes.data_as(C.POINTER(C.c_double)))
+    hip.SUNAMD_BatchedBDFFree(B)
+    nst32 = int(c32[:, 0].sum())
+
+    # roofline: the longest bucket kernel of the last timed run
+    fl = batch_flops(c32, msys, rsys)
+    kernels = {}
+    for k in range(nb):
+        sel = sb == k
+        kernels["m%d" % bm[k]] = {"systems": int(sel.sum()), "ms": round(float(kms[k]), 3),
+                                  "gflop": round(float(fl[sel].sum()) / 1e9, 3)}
+    dom = int(np.argmax(kms))
+    dflop = float(fl[sb == dom].sum())
+    achieved = dflop / (kms[dom] * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": "k_batched_bdf<%d, float>" % bm[dom],
+                "achieved": round(achieved, 4), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP64_PEAK_TFS, 5), "traffic": None,
+                "flops_per_launch": dflop, "launch_ms": round(float(kms[dom]), 3),
+                "note": "compute/latency-bound persistent integrator; priced against the FP64 "
+                        "vector peak (most of its arithmetic is FP64)"}
+
+    # accuracy of the FP32-LU form against the FP64 form (= CPU oracle bit for bit)
+    yoff = np.ctypeslib.as_array(bc.sys_yoff, shape=(bc.nsys,))
+    w = 1.0 / (RTOL * np.abs(y64) + ATOL)
+    d2 = ((y32 - y64) * w) ** 2
+    csum = np.concatenate(([0.0], np.cumsum(d2)))
+    werr = np.sqrt((csum[yoff + msys] - csum[yoff]) / msys)
+    # relative error of components above the absolute tolerance
+    relmax = float(np.max(np.abs(y32 - y64) / np.maximum(np.abs(y64), ATOL)))
+
+    cpu = None if args.no_cpu else batch_cpu_baseline(args.cpu_systems,
+                                                      min(16, os.cpu_count() or 1))
+    core.SUNAMD_BatchFree(b)
+
+    value = nst32 * args.steps / elapsed
+    v64 = nst64 / t64
+    line = {
+        "metric": "cell-steps/s & Newton iters/step, learned+FP32-LU vs plain FP64, same tol",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32lu",
+        "data": "synthetic",
+        "config": {"workload": "uneven batch: %d independent mass-action systems, m in %s "
+                               "(weights %s %%), R = 1.5 m, per-system BDF, rtol %g atol %g, "
+                               "t in [0,%g], FP32 Newton-matrix LU, polynomial predictor"
+                               % (args.systems, list(BATCH_M), [int(x) for x in BATCH_W],
+                                  RTOL, ATOL, TOUT),
+                   "systems": args.systems, "predictor": "polynomial (learned not enabled)"},
+        "newton_iters_per_step": c32[:, 4].sum() / max(nst32, 1),
+        "cell_steps": nst32,
+        "plain_fp64": {"value": v64, "ms": 1e3 * t64, "cell_steps": nst64,
+                       "newton_iters_per_step": c64[:, 4].sum() / max(nst64, 1)},
+        "speedup_vs_plain_fp64": value / v64,
+        "error_vs_fp64": {"max_weighted_rms": float(werr.max()),
+                          "median_weighted_rms": float(np.median(werr)),
+                          "max_rel": relmax},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["batched", "coupled"], default="batched")
+    ap.add_argument("--systems", type=int, default=65536)
+    ap.add_argument("--cpu-systems", type=int, default=8192)
+    ap.add_argument("--cells", type=int, default=1 << 20)
+    ap.add_argument("--lu", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--micro-n", type=int, default=1 << 28)
+    ap.add_argument("--cpu-cells", type=int, default=1024)
+    ap.add_argument("--no-micro", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(systems are independent; DESIGN.md describes how the batch would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    line = bench_batched(args) if args.workload == "batched" else bench_coupled(args)
     print(json.dumps(line))
 
 

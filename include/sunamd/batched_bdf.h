@@ -80,6 +80,10 @@ int SUNAMD_BatchedBDFSetMaxNumSteps(SUNAMD_BatchedBDF* B, long int mxsteps);
 int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* wall_s);
 /* results in system order: y (ytotal), counters (nsys*SUNAMD_BS_COUNT) */
 int SUNAMD_BatchedBDFGetResults(SUNAMD_BatchedBDF* B, double* y, long int* counters);
+/* kernel time of every bucket in the last run, in bucket order (HIP events
+   on the bucket's stream around its launch; the buckets run concurrently);
+   returns the number of buckets */
+int SUNAMD_BatchedBDFGetKernelTimes(SUNAMD_BatchedBDF* B, double* ms);
 void SUNAMD_BatchedBDFFree(SUNAMD_BatchedBDF* B);
 
 #ifdef __cplusplus

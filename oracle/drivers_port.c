@@ -23,6 +23,7 @@
 #include "oracle.h"
 #include "sunamd/cvode_sunamd.h"
 #include "sunamd/batched_bdf.h"
+#include "sunamd/tridiag_mi355.h"
 #include "sunamd/massaction.h"
 
 static double now_s(void)
@@ -319,4 +320,43 @@ int oracle_batch_solve(const SUNAMD_Batch* batch, double rtol, double atol, doub
   }
   if (wall) { *wall = now_s() - t0; }
   return rc;
+}
+
+/* ---- SPGMR test problem (test_sunlinsol_spgmr_serial.c:433-503) ------------
+   A = S1^{-1} tridiag(-1, 5, -1) S2 and a Jacobi preconditioner, on serial
+   vectors, in the test's own operation order. */
+int oracle_tridiag_atimes(void* data, N_Vector v_vec, N_Vector z_vec)
+{
+  const SUNAMD_TridiagData* P = (const SUNAMD_TridiagData*)data;
+  const sunrealtype* v        = N_VGetArrayPointer(v_vec);
+  sunrealtype* z              = N_VGetArrayPointer(z_vec);
+  const sunrealtype* s1       = N_VGetArrayPointer(P->s1);
+  const sunrealtype* s2       = N_VGetArrayPointer(P->s2);
+  const sunindextype N        = P->N;
+  if (N < 2) { return -1; }
+  z[0] = (5.0 * v[0] * s2[0] - v[1] * s2[1]) / s1[0];
+  for (sunindextype i = 1; i < N - 1; i++)
+  {
+    z[i] = (-v[i - 1] * s2[i - 1] + 5.0 * v[i] * s2[i] - v[i + 1] * s2[i + 1]) / s1[i];
+  }
+  z[N - 1] = (-v[N - 2] * s2[N - 2] + 5.0 * v[N - 1] * s2[N - 1]) / s1[N - 1];
+  return 0;
+}
+
+int oracle_tridiag_psetup(void* data)
+{
+  (void)data;
+  return 0;
+}
+
+int oracle_tridiag_psolve(void* data, N_Vector r_vec, N_Vector z_vec, sunrealtype tol, int lr)
+{
+  (void)tol;
+  (void)lr;
+  const SUNAMD_TridiagData* P = (const SUNAMD_TridiagData*)data;
+  const sunrealtype* r        = N_VGetArrayPointer(r_vec);
+  sunrealtype* z              = N_VGetArrayPointer(z_vec);
+  const sunrealtype* d        = N_VGetArrayPointer(P->d);
+  for (sunindextype i = 0; i < P->N; i++) { z[i] = r[i] / d[i]; }
+  return 0;
 }

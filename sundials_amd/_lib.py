@@ -57,6 +57,11 @@ BS_NAMES = ("nst", "nfe", "nsetups", "netf", "nni", "nnf", "ncfn", "nje", "qlast
 BS_COUNT = len(BS_NAMES)
 
 
+class TridiagData(C.Structure):
+    """Mirror of SUNAMD_TridiagData (include/sunamd/tridiag_mi355.h)."""
+    _fields_ = [("N", C.c_int64), ("d", C.c_void_p), ("s1", C.c_void_p), ("s2", C.c_void_p)]
+
+
 class Batch(C.Structure):
     """Mirror of SUNAMD_Batch (include/sunamd/batched_bdf.h)."""
     _fields_ = [
@@ -146,6 +151,26 @@ def core():
          [P(MassAction), P(c_real), P(c_real), C.c_int64, C.c_int64])
     _sig(lib, "SUNAMD_MassActionJacHost", None,
          [P(MassAction), P(c_real), P(c_real), C.c_int64, C.c_int64])
+    _sig(lib, "SUNLinSolSetATimes", C.c_int, [c_void_p, c_void_p, c_void_p])
+    _sig(lib, "SUNLinSolSetPreconditioner", C.c_int, [c_void_p, c_void_p, c_void_p, c_void_p])
+    _sig(lib, "SUNLinSolSetScalingVectors", C.c_int, [c_void_p, c_void_p, c_void_p])
+    _sig(lib, "SUNLinSolSetZeroGuess", C.c_int, [c_void_p, C.c_int])
+    _sig(lib, "SUNLinSolNumIters", C.c_int, [c_void_p])
+    _sig(lib, "SUNLinSolResNorm", c_real, [c_void_p])
+    _sig(lib, "SUNLinSolGetType", C.c_int, [c_void_p])
+    _sig(lib, "SUNLinSolGetID", C.c_int, [c_void_p])
+    _sig(lib, "SUNLinSolSpace", C.c_int, [c_void_p, P(C.c_long), P(C.c_long)])
+    _sig(lib, "SUNLinSol_SPGMR", c_void_p, [c_void_p, C.c_int, C.c_int, c_void_p])
+    _sig(lib, "SUNLinSol_SPGMRSetPrecType", C.c_int, [c_void_p, C.c_int])
+    _sig(lib, "SUNLinSol_SPGMRSetGSType", C.c_int, [c_void_p, C.c_int])
+    _sig(lib, "SUNLinSol_SPGMRSetMaxRestarts", C.c_int, [c_void_p, C.c_int])
+    _sig(lib, "SUNLinSol_SPGMRSetClassicalGSFn", C.c_int, [c_void_p, c_void_p])
+    _sig(lib, "SUNModifiedGS", C.c_int, [P(c_void_p), P(P(c_real)), C.c_int, C.c_int, P(c_real)])
+    _sig(lib, "SUNClassicalGS", C.c_int, [P(c_void_p), P(P(c_real)), C.c_int, C.c_int, P(c_real),
+                                          P(c_real), P(c_void_p)])
+    _sig(lib, "SUNQRfact", C.c_int, [C.c_int, P(P(c_real)), P(c_real), C.c_int])
+    _sig(lib, "SUNQRsol", C.c_int, [C.c_int, P(P(c_real)), P(c_real), P(c_real)])
+    _sig(lib, "SUNRCompareTol", C.c_int, [c_real, c_real, c_real])
     _sig(lib, "SUNAMD_BatchCreate", P(Batch),
          [C.c_int64, C.c_uint64, C.c_int, P(C.c_int), P(c_real), c_real, c_real])
     _sig(lib, "SUNAMD_BatchFree", None, [P(Batch)])
@@ -199,10 +224,14 @@ def hip():
     _sig(lib, "SUNAMD_ProfilerReset", None, [])
     _sig(lib, "SUNAMD_ProfilerGet", C.c_int, [C.c_int, P(C.c_double), P(C.c_long)])
     _sig(lib, "SUNAMD_BenchVectorOps", C.c_int, [c_index, C.c_int, C.c_int, P(c_real)])
+    _sig(lib, "SUNAMD_TridiagATimes_Mi355", C.c_int, [c_void_p, c_void_p, c_void_p])
+    _sig(lib, "SUNAMD_TridiagPSetup_Mi355", C.c_int, [c_void_p])
+    _sig(lib, "SUNAMD_TridiagPSolve_Mi355", C.c_int, [c_void_p, c_void_p, c_void_p, c_real, C.c_int])
     _sig(lib, "SUNAMD_BatchedBDFCreate", c_void_p, [P(Batch), c_real, c_real, C.c_int])
     _sig(lib, "SUNAMD_BatchedBDFSetMaxNumSteps", C.c_int, [c_void_p, C.c_long])
     _sig(lib, "SUNAMD_BatchedBDFRun", C.c_int, [c_void_p, c_real, P(c_real)])
     _sig(lib, "SUNAMD_BatchedBDFGetResults", C.c_int, [c_void_p, P(c_real), P(C.c_long)])
+    _sig(lib, "SUNAMD_BatchedBDFGetKernelTimes", C.c_int, [c_void_p, P(c_real)])
     _sig(lib, "SUNAMD_BatchedBDFFree", None, [c_void_p])
     _hip = lib
     return lib
@@ -233,6 +262,9 @@ def load_oracle():
          [P(MassAction), c_real, c_real, c_real, P(c_real), P(c_real)])
     _sig(lib, "oracle_massaction_solve_per_cell", C.c_int,
          [P(MassAction), c_real, c_real, c_real, C.c_int, P(c_real), P(c_real), P(c_real)])
+    _sig(lib, "oracle_tridiag_atimes", C.c_int, [c_void_p, c_void_p, c_void_p])
+    _sig(lib, "oracle_tridiag_psetup", C.c_int, [c_void_p])
+    _sig(lib, "oracle_tridiag_psolve", C.c_int, [c_void_p, c_void_p, c_void_p, c_real, C.c_int])
     _sig(lib, "oracle_batch_solve", C.c_int,
          [P(Batch), c_real, c_real, c_real, C.c_int, P(c_real), P(C.c_long), P(c_real)])
     return lib

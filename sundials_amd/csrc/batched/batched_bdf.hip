@@ -12,9 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 
-#include "../core/profiler_internal.h"
 #include "sunamd/batched_bdf.h"
-#include "sunamd/sunamd_profiler.h"
 #include "sunamd/sunmatrix_mi355block.h"
 
 #include "bdf_args.h"
@@ -36,6 +34,8 @@ struct BucketDev
   long* stats;
   unsigned long long* next;
   hipStream_t stream;
+  hipEvent_t ev0, ev1; // bracket the bucket's launch on its stream
+  double last_ms;      // kernel time of the last run
 };
 
 struct SUNAMD_BatchedBDF_
@@ -121,6 +121,8 @@ extern "C" SUNAMD_BatchedBDF* SUNAMD_BatchedBDFCreate(const SUNAMD_Batch* batch,
     if (d.m > 64) { e = hipErrorInvalidValue; break; }
     long nc = d.ncells > 0 ? d.ncells : 1;
     e       = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) { e = hipEventCreate(&d.ev0); }
+    if (e == hipSuccess) { e = hipEventCreate(&d.ev1); }
     if (e == hipSuccess) { e = hipMalloc((void**)&d.idx, sizeof(int) * 4 * d.R); }
     if (e == hipSuccess) { e = hipMalloc((void**)&d.kf, sizeof(double) * nc * d.R); }
     if (e == hipSuccess) { e = hipMalloc((void**)&d.kb, sizeof(double) * nc * d.R); }
@@ -208,10 +210,12 @@ extern "C" int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* w
     long want     = (d.ncells + gpw - 1) / gpw;
     long cap      = (long)B->num_cus * 16;
     int grid      = (int)(want < cap ? want : cap);
-    sunamd_prof_begin(SUNAMD_K_BDF_BATCH, d.stream);
+    e = hipEventRecord(d.ev0, d.stream);
+    if (e != hipSuccess) { return hfail(e, "event record"); }
     bool ok = (B->prec == SUNAMD_LU_FP32) ? launch_bucket<float>(d, B->rtol, B->atol, tout, B->mxstep, grid)
                                           : launch_bucket<double>(d, B->rtol, B->atol, tout, B->mxstep, grid);
-    sunamd_prof_end(SUNAMD_K_BDF_BATCH, d.stream);
+    e = hipEventRecord(d.ev1, d.stream);
+    if (e != hipSuccess) { return hfail(e, "event record"); }
     if (!ok)
     {
       fprintf(stderr, "[sundials_amd batched_bdf] no kernel for m = %d\n", d.m);
@@ -223,6 +227,16 @@ extern "C" int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* w
   hipError_t e = hipDeviceSynchronize();
   if (e != hipSuccess) { return hfail(e, "batched BDF run"); }
   auto t1 = std::chrono::steady_clock::now();
+  for (int k = 0; k < B->nb; k++)
+  {
+    BucketDev& d = B->b[k];
+    d.last_ms    = 0.0;
+    if (d.ncells <= 0) { continue; }
+    float ms = 0.0f;
+    e        = hipEventElapsedTime(&ms, d.ev0, d.ev1);
+    if (e != hipSuccess) { return hfail(e, "event timing"); }
+    d.last_ms = ms;
+  }
   if (wall_s) { *wall_s = std::chrono::duration<double>(t1 - t0).count(); }
   return 0;
 }
@@ -272,6 +286,13 @@ extern "C" int SUNAMD_BatchedBDFGetResults(SUNAMD_BatchedBDF* B, double* y, long
   return rc;
 }
 
+extern "C" int SUNAMD_BatchedBDFGetKernelTimes(SUNAMD_BatchedBDF* B, double* ms)
+{
+  if (!B || !ms) { return -1; }
+  for (int k = 0; k < B->nb; k++) { ms[k] = B->b[k].last_ms; }
+  return B->nb;
+}
+
 extern "C" void SUNAMD_BatchedBDFFree(SUNAMD_BatchedBDF* B)
 {
   if (!B) { return; }
@@ -286,6 +307,8 @@ extern "C" void SUNAMD_BatchedBDFFree(SUNAMD_BatchedBDF* B)
     if (d.yout) { (void)hipFree(d.yout); }
     if (d.stats) { (void)hipFree(d.stats); }
     if (d.next) { (void)hipFree(d.next); }
+    if (d.ev0) { (void)hipEventDestroy(d.ev0); }
+    if (d.ev1) { (void)hipEventDestroy(d.ev1); }
     if (d.stream) { (void)hipStreamDestroy(d.stream); }
   }
   free(B->sys_bucket);
