@@ -80,6 +80,25 @@ int SUNAMD_BatchedBDFSetMaxNumSteps(SUNAMD_BatchedBDF* B, long int mxsteps);
 int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* wall_s);
 /* results in system order: y (ytotal), counters (nsys*SUNAMD_BS_COUNT) */
 int SUNAMD_BatchedBDFGetResults(SUNAMD_BatchedBDF* B, double* y, long int* counters);
+/* Learned initial Newton iterate (no counterpart in the reference, whose
+   Newton solve always starts from a zero correction, cvode.c:3095-3109):
+   weights = SUNAMD_MLPDeviceWeights of a trained network (NULL: off),
+   bit k of bucket_mask enables it for bucket k.  At the first attempt of
+   every step after the first, the network's correction and the zero
+   correction are both evaluated and the one with the smaller weighted
+   residual norm starts the Newton iteration (two RHS evaluations). */
+int SUNAMD_BatchedBDFSetLearnedPredictor(SUNAMD_BatchedBDF* B, const void* weights,
+                                         int bucket_mask);
+/* Training-pair log: device arrays x[capacity x 144], t[capacity x 64],
+   mask[capacity x 64] receive, for every stride-th step of every system, the
+   network inputs at the predictor, the converged correction in error-weight
+   units and the species mask (learned_predictor.h); the row counter restarts
+   here and keeps counting across runs */
+int SUNAMD_BatchedBDFSetTrainingLog(SUNAMD_BatchedBDF* B, float* x, float* t, float* mask,
+                                    long capacity, int stride);
+long int SUNAMD_BatchedBDFGetTrainingLogCount(SUNAMD_BatchedBDF* B);
+/* steps of the last run where the learned guess was tried / kept */
+int SUNAMD_BatchedBDFGetLearnedStats(SUNAMD_BatchedBDF* B, long int* tried, long int* kept);
 /* kernel time of every bucket in the last run, in bucket order (HIP events
    on the bucket's stream around its launch; the buckets run concurrently);
    returns the number of buckets */

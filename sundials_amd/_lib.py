@@ -232,6 +232,21 @@ def hip():
     _sig(lib, "SUNAMD_BatchedBDFRun", C.c_int, [c_void_p, c_real, P(c_real)])
     _sig(lib, "SUNAMD_BatchedBDFGetResults", C.c_int, [c_void_p, P(c_real), P(C.c_long)])
     _sig(lib, "SUNAMD_BatchedBDFGetKernelTimes", C.c_int, [c_void_p, P(c_real)])
+    _sig(lib, "SUNAMD_BatchedBDFSetLearnedPredictor", C.c_int, [c_void_p, c_void_p, C.c_int])
+    _sig(lib, "SUNAMD_BatchedBDFSetTrainingLog", C.c_int,
+         [c_void_p, c_void_p, c_void_p, c_void_p, C.c_long, C.c_int])
+    _sig(lib, "SUNAMD_BatchedBDFGetTrainingLogCount", C.c_long, [c_void_p])
+    _sig(lib, "SUNAMD_BatchedBDFGetLearnedStats", C.c_int, [c_void_p, P(C.c_long), P(C.c_long)])
+    PFl = P(C.c_float)
+    _sig(lib, "SUNAMD_MLPCreate", c_void_p, [C.c_uint64, C.c_int])
+    _sig(lib, "SUNAMD_MLPFree", None, [c_void_p])
+    _sig(lib, "SUNAMD_MLPGetParams", C.c_int, [c_void_p, PFl])
+    _sig(lib, "SUNAMD_MLPSetParams", C.c_int, [c_void_p, PFl])
+    _sig(lib, "SUNAMD_MLPDeviceWeights", c_void_p, [c_void_p])
+    _sig(lib, "SUNAMD_MLPForward", C.c_int, [c_void_p, PFl, C.c_int, PFl])
+    _sig(lib, "SUNAMD_MLPLossGrad", C.c_int, [c_void_p, PFl, PFl, PFl, C.c_int, PFl, PFl])
+    _sig(lib, "SUNAMD_MLPTrain", C.c_int,
+         [c_void_p, c_void_p, c_void_p, c_void_p, C.c_long, C.c_int, C.c_int, C.c_float, PFl])
     _sig(lib, "SUNAMD_BatchedBDFFree", None, [c_void_p])
     _hip = lib
     return lib

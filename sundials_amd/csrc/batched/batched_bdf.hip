@@ -16,6 +16,7 @@
 #include "sunamd/sunmatrix_mi355block.h"
 
 #include "bdf_args.h"
+#include "sunamd/learned_predictor.h"
 
 using sunamd_bdf::BucketArgs;
 using sunamd_bdf::pow2ceil;
@@ -52,6 +53,13 @@ struct SUNAMD_BatchedBDF_
   long* sys_cell;
   long* sys_yoff;
   long ytotal;
+  // learned initial iterate and training log
+  const void* mlp;      // device weight block (SUNAMD_MLPDeviceWeights) or NULL
+  int learned_mask;     // bit k: bucket k uses it
+  float *log_x, *log_t, *log_m;
+  long log_cap;
+  int log_stride;
+  unsigned long long* dcount; // [0] log rows, [1 + 2k], [2 + 2k]: bucket k tried / kept
 };
 
 static int hfail(hipError_t e, const char* what)
@@ -64,10 +72,29 @@ static int hfail(hipError_t e, const char* what)
 #define BDF_SIZES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(16) X(24) X(32) X(48) X(64)
 
 template <typename T>
-static bool launch_bucket(const BucketDev& d, double rtol, double atol, double tout, long mxstep,
-                          int grid)
+static bool launch_bucket(const SUNAMD_BatchedBDF* B, int k, double tout, int grid)
 {
+  const BucketDev& d  = B->b[k];
+  const double rtol   = B->rtol;
+  const double atol   = B->atol;
+  const long mxstep   = B->mxstep;
   BucketArgs A;
+  memset(&A, 0, sizeof(A));
+  if (B->mlp && (B->learned_mask >> k) & 1)
+  {
+    A.mlp_w = (const unsigned short*)B->mlp;
+    A.mlp_b = (const float*)((const unsigned short*)B->mlp + SUNAMD_MLP_WEIGHTS_BF16);
+  }
+  if (B->log_x && B->log_cap > 0)
+  {
+    A.log_x      = B->log_x;
+    A.log_t      = B->log_t;
+    A.log_m      = B->log_m;
+    A.log_next   = B->dcount;
+    A.log_cap    = B->log_cap;
+    A.log_stride = B->log_stride > 0 ? B->log_stride : 1;
+  }
+  A.lstats = B->dcount + 1 + 2 * k;
   A.m      = d.m;
   A.R      = d.R;
   A.ncells = d.ncells;
@@ -103,6 +130,13 @@ extern "C" SUNAMD_BatchedBDF* SUNAMD_BatchedBDFCreate(const SUNAMD_Batch* batch,
   B->atol = atol;
   B->prec = lu_precision;
   B->mxstep = MXSTEP_DEFAULT;
+  if (hipMalloc((void**)&B->dcount, sizeof(unsigned long long) * (1 + 2 * SUNAMD_BATCH_MAXB)) !=
+        hipSuccess ||
+      hipMemset(B->dcount, 0, sizeof(unsigned long long) * (1 + 2 * SUNAMD_BATCH_MAXB)) != hipSuccess)
+  {
+    free(B);
+    return NULL;
+  }
   int dev = 0;
   hipDeviceProp_t prop;
   B->num_cus = 256;
@@ -200,6 +234,11 @@ extern "C" int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* w
       }
     }
   }
+  // learned-guess counters restart with every run (the log keeps filling)
+  {
+    hipError_t e = hipMemset(B->dcount + 1, 0, sizeof(unsigned long long) * 2 * SUNAMD_BATCH_MAXB);
+    if (e != hipSuccess) { return hfail(e, "counter reset"); }
+  }
   for (int n = 0; n < B->nb; n++)
   {
     BucketDev& d = B->b[order[n]];
@@ -212,8 +251,8 @@ extern "C" int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* w
     int grid      = (int)(want < cap ? want : cap);
     e = hipEventRecord(d.ev0, d.stream);
     if (e != hipSuccess) { return hfail(e, "event record"); }
-    bool ok = (B->prec == SUNAMD_LU_FP32) ? launch_bucket<float>(d, B->rtol, B->atol, tout, B->mxstep, grid)
-                                          : launch_bucket<double>(d, B->rtol, B->atol, tout, B->mxstep, grid);
+    bool ok = (B->prec == SUNAMD_LU_FP32) ? launch_bucket<float>(B, order[n], tout, grid)
+                                          : launch_bucket<double>(B, order[n], tout, grid);
     e = hipEventRecord(d.ev1, d.stream);
     if (e != hipSuccess) { return hfail(e, "event record"); }
     if (!ok)
@@ -286,6 +325,52 @@ extern "C" int SUNAMD_BatchedBDFGetResults(SUNAMD_BatchedBDF* B, double* y, long
   return rc;
 }
 
+extern "C" int SUNAMD_BatchedBDFSetLearnedPredictor(SUNAMD_BatchedBDF* B, const void* weights,
+                                                    int bucket_mask)
+{
+  if (!B) { return -1; }
+  B->mlp          = weights;
+  B->learned_mask = bucket_mask;
+  return 0;
+}
+
+extern "C" int SUNAMD_BatchedBDFSetTrainingLog(SUNAMD_BatchedBDF* B, float* x, float* t, float* mask,
+                                               long capacity, int stride)
+{
+  if (!B || capacity < 0 || stride < 1) { return -1; }
+  if (capacity > 0 && (!x || !t || !mask)) { return -1; }
+  B->log_x      = x;
+  B->log_t      = t;
+  B->log_m      = mask;
+  B->log_cap    = capacity;
+  B->log_stride = stride;
+  hipError_t e  = hipMemset(B->dcount, 0, sizeof(unsigned long long));
+  return e == hipSuccess ? 0 : hfail(e, "log reset");
+}
+
+extern "C" long int SUNAMD_BatchedBDFGetTrainingLogCount(SUNAMD_BatchedBDF* B)
+{
+  unsigned long long n = 0;
+  if (!B || hipMemcpy(&n, B->dcount, sizeof(n), hipMemcpyDeviceToHost) != hipSuccess) { return -1; }
+  return (long)(n < (unsigned long long)B->log_cap ? n : (unsigned long long)B->log_cap);
+}
+
+extern "C" int SUNAMD_BatchedBDFGetLearnedStats(SUNAMD_BatchedBDF* B, long int* tried,
+                                                long int* kept)
+{
+  unsigned long long c[1 + 2 * SUNAMD_BATCH_MAXB];
+  if (!B || hipMemcpy(c, B->dcount, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess) { return -1; }
+  long tt = 0, kk = 0;
+  for (int k = 0; k < B->nb; k++)
+  {
+    tt += (long)c[1 + 2 * k];
+    kk += (long)c[2 + 2 * k];
+  }
+  if (tried) { *tried = tt; }
+  if (kept) { *kept = kk; }
+  return 0;
+}
+
 extern "C" int SUNAMD_BatchedBDFGetKernelTimes(SUNAMD_BatchedBDF* B, double* ms)
 {
   if (!B || !ms) { return -1; }
@@ -311,6 +396,7 @@ extern "C" void SUNAMD_BatchedBDFFree(SUNAMD_BatchedBDF* B)
     if (d.ev1) { (void)hipEventDestroy(d.ev1); }
     if (d.stream) { (void)hipStreamDestroy(d.stream); }
   }
+  if (B->dcount) { (void)hipFree(B->dcount); }
   free(B->sys_bucket);
   free(B->sys_cell);
   free(B->sys_yoff);

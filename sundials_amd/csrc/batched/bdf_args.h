@@ -28,7 +28,23 @@ struct BucketArgs
   unsigned long long* next;
   double rtol, atol, tout;
   long mxstep;
+  // learned initial Newton iterate (mlp_w == nullptr: off)
+  const unsigned short* mlp_w; // bf16 W1[128x144] W2[128x128] W3[64x128]
+  const float* mlp_b;          // b1[128] b2[128] b3[64]
+  // training-pair log (log_x == nullptr: off): rows of 144 inputs, 64
+  // targets and 64 mask entries, one row every log_stride-th step
+  float* log_x;
+  float* log_t;
+  float* log_m;
+  unsigned long long* log_next;
+  long log_cap;
+  int log_stride;
+  // learned-guess counters of the bucket: [steps tried, guesses kept]
+  unsigned long long* lstats;
 };
+
+// LDS floats per lane group for the MLP inputs and hidden layers
+constexpr int MLP_LDS_FLOATS = 144 + 128 + 128;
 
 // launches the persistent kernel for systems of M species (T: LU precision);
 // defined in bdf_device.h, instantiated in the bdf_inst_*.hip units

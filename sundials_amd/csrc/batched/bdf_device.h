@@ -138,6 +138,10 @@ struct Sys
   int q, qprime, qwait, L, indx_acor, convfail, qu;
   long nst, nfe, ncfn, netf, nni, nnf, nsetups, nstlp, nje, nstlj;
   bool nls_jcur, acnrmcur;
+  // learned initial iterate: this species' rate feature, and counts of the
+  // steps where a guess was tried / kept
+  float ratef;
+  long ltried, lused;
 };
 
 template <int M, typename T>
@@ -150,6 +154,7 @@ struct Grp
   T* prow;     // G entries of LDS
   int* inv;    // G ints of LDS
   GroupScalars* sc;
+  float* mlp; // MLP_LDS_FLOATS of LDS (xs[144] h1[128] h2[128]) or nullptr
   const BucketArgs* a;
   long cell;
   const double* kf;
@@ -599,7 +604,8 @@ __device__ __forceinline__ void g_lsolve(const Grp<M, T>& g, Sys<M, T>& S)
 }
 
 template <int M, typename T>
-__device__ __forceinline__ int g_ctest(const Grp<M, T>& g, Sys<M, T>& S, int m, double tol)
+__device__ __forceinline__ int g_ctest(const Grp<M, T>& g, Sys<M, T>& S, int m, double tol,
+                                       bool nonzero_start)
 {
   double del = g_wrms(g, S.delta, S.ewt);
   if (m > 0)
@@ -610,7 +616,9 @@ __device__ __forceinline__ int g_ctest(const Grp<M, T>& g, Sys<M, T>& S, int m, 
   double dcon = del * ((1.0 < S.crate) ? 1.0 : S.crate) / tol;
   if (dcon <= 1.0)
   {
-    S.acnrm    = (m == 0) ? del : g_wrms(g, S.acor, S.ewt);
+    // after one iteration from a zero start the correction is the update
+    // itself; from a learned start its norm has to be taken
+    S.acnrm    = (m == 0 && !nonzero_start) ? del : g_wrms(g, S.acor, S.ewt);
     S.acnrmcur = true;
     return CV_SUCCESS;
   }
@@ -619,8 +627,11 @@ __device__ __forceinline__ int g_ctest(const Grp<M, T>& g, Sys<M, T>& S, int m, 
   return NLS_CONTINUE;
 }
 
+// have_res: the residual of the current correction is already in S.delta
+// (and S.y, S.ftemp); nonzero_start: the correction starts from a guess
 template <int M, typename T>
-__device__ __forceinline__ int g_newton(const Grp<M, T>& g, Sys<M, T>& S, bool callLSetup)
+__device__ __forceinline__ int g_newton(const Grp<M, T>& g, Sys<M, T>& S, bool callLSetup,
+                                        bool have_res, bool nonzero_start)
 {
   bool jbad        = false;
   long niters      = 0;
@@ -630,7 +641,8 @@ __device__ __forceinline__ int g_newton(const Grp<M, T>& g, Sys<M, T>& S, bool c
   for (;;)
   {
     int curiter = 0;
-    g_residual(g, S);
+    if (!have_res) { g_residual(g, S); }
+    have_res = false;
     if (callLSetup)
     {
       retval = g_lsetup(g, S, jbad);
@@ -642,7 +654,7 @@ __device__ __forceinline__ int g_newton(const Grp<M, T>& g, Sys<M, T>& S, bool c
       S.delta = -1.0 * S.delta;
       g_lsolve(g, S);
       S.acor = lsum(1.0, S.acor, 1.0, S.delta);
-      retval = g_ctest(g, S, curiter, tol);
+      retval = g_ctest(g, S, curiter, tol, nonzero_start);
       curiter++;
       if (retval == CV_SUCCESS)
       {
@@ -662,9 +674,10 @@ __device__ __forceinline__ int g_newton(const Grp<M, T>& g, Sys<M, T>& S, bool c
     if ((retval > 0) && !S.nls_jcur)
     {
       nconvfails++;
-      callLSetup = true;
-      jbad       = true;
-      S.acor     = 0.0;
+      callLSetup    = true;
+      jbad          = true;
+      S.acor        = 0.0;
+      nonzero_start = false;
       continue;
     }
     break;
@@ -675,15 +688,146 @@ __device__ __forceinline__ int g_newton(const Grp<M, T>& g, Sys<M, T>& S, bool c
   return retval;
 }
 
+// ---- learned initial iterate ------------------------------------------------
+__device__ __forceinline__ float bf16_round(float f)
+{
+  unsigned int u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) { return f; }
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return __uint_as_float(u & 0xffff0000u);
+}
+
+__device__ __forceinline__ float bf16_at(const unsigned short* w, long i)
+{
+  return __uint_as_float(((unsigned int)w[i]) << 16);
+}
+
+// the group's MLP inputs (unrounded) into xs[0..143]; each index has exactly
+// one writing lane
+template <int M, typename T>
+__device__ __forceinline__ void g_features(const Grp<M, T>& g, const Sys<M, T>& S, float* xs)
+{
+  constexpr int G = pow2ceil(M);
+  gsync();
+  if (g.valid)
+  {
+    double a = fabs(S.zn[0]);
+    xs[g.i]      = (float)(log10(a > 1e-30 ? a : 1e-30) * 0.1);
+    xs[65 + g.i] = S.ratef;
+  }
+  for (int j = g.i; j < 144; j += G)
+  {
+    const bool feat = (j < M) || (j >= 65 && j < 65 + M);
+    if (j == 64) { xs[64] = (float)(log10(S.gamma) * 0.1); }
+    else if (!feat) { xs[j] = 0.0f; }
+  }
+  gsync();
+}
+
+// forward pass of the group's MLP on xs; returns this lane's output
+// (species g.i), in error-weight units
+template <int M, typename T>
+__device__ __forceinline__ float g_mlp(const Grp<M, T>& g, float* xs)
+{
+  constexpr int G             = pow2ceil(M);
+  const unsigned short* W1    = g.a->mlp_w;
+  const unsigned short* W2    = W1 + 128 * 144;
+  const unsigned short* W3    = W2 + 128 * 128;
+  const float* b1             = g.a->mlp_b;
+  const float* b2             = b1 + 128;
+  const float* b3             = b2 + 128;
+  float* h1                   = xs + 144;
+  float* h2                   = h1 + 128;
+  for (int j = g.i; j < 144; j += G) { xs[j] = bf16_round(xs[j]); }
+  gsync();
+  for (int o = g.i; o < 128; o += G)
+  {
+    float acc = b1[o];
+    for (int k = 0; k < 129; k++) { acc += bf16_at(W1, (long)o * 144 + k) * xs[k]; }
+    h1[o] = bf16_round(acc / (1.0f + expf(-acc)));
+  }
+  gsync();
+  for (int o = g.i; o < 128; o += G)
+  {
+    float acc = b2[o];
+    for (int k = 0; k < 128; k++) { acc += bf16_at(W2, (long)o * 128 + k) * h1[k]; }
+    h2[o] = bf16_round(acc / (1.0f + expf(-acc)));
+  }
+  gsync();
+  float out = 0.0f;
+  if (g.valid)
+  {
+    out = b3[g.i];
+    for (int k = 0; k < 128; k++) { out += bf16_at(W3, (long)g.i * 128 + k) * h2[k]; }
+  }
+  gsync();
+  return out;
+}
+
+// one training row: inputs, the converged correction in error-weight units,
+// and the species mask
+template <int M, typename T>
+__device__ __forceinline__ void g_log_pair(const Grp<M, T>& g, const Sys<M, T>& S)
+{
+  constexpr int G      = pow2ceil(M);
+  const BucketArgs& A  = *g.a;
+  unsigned long long r = 0;
+  if (g.i == 0) { r = atomicAdd(A.log_next, 1ULL); }
+  r = __shfl(r, g.base, 64);
+  if (r >= (unsigned long long)A.log_cap) { return; }
+  g_features(g, S, g.mlp);
+  float* x = A.log_x + (long)r * 144;
+  for (int j = g.i; j < 144; j += G) { x[j] = g.mlp[j]; }
+  float* t = A.log_t + (long)r * 64;
+  float* w = A.log_m + (long)r * 64;
+  for (int j = g.i; j < 64; j += G)
+  {
+    t[j] = (j == g.i && g.valid) ? (float)(S.acor * S.ewt) : 0.0f;
+    w[j] = (j < M) ? 1.0f : 0.0f;
+  }
+}
+
 template <int M, typename T>
 __device__ __forceinline__ int g_nls(const Grp<M, T>& g, Sys<M, T>& S, int nflag)
 {
   S.convfail = ((nflag == FIRST_CALL) || (nflag == PREV_ERR_FAIL)) ? CV_NO_FAILURES : CV_FAIL_OTHER;
   bool callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (S.nst == 0) ||
                    (S.nst >= S.nstlp + MSBP) || (fabs(S.gamrat - 1.0) > DGMAX_LSETUP);
-  S.acor   = 0.0;
-  int flag = g_newton(g, S, callSetup);
+  S.acor        = 0.0;
+  bool have_res = false, learned = false;
+  if (g.a->mlp_w != nullptr && nflag == FIRST_CALL && S.nst > 0)
+  {
+    // learned start, kept only if its residual is smaller than the zero
+    // start's (the plain predictor is the fallback)
+    g_features(g, S, g.mlp);
+    const double guess = (double)g_mlp(g, g.mlp) / S.ewt;
+    g_residual(g, S);
+    const double r0  = g_wrms(g, S.delta, S.ewt);
+    const double d0 = S.delta, y0 = S.y, f0 = S.ftemp;
+    S.acor           = g.valid ? guess : 0.0;
+    g_residual(g, S);
+    const double r1 = g_wrms(g, S.delta, S.ewt);
+    S.ltried++;
+    if (r1 < r0)
+    {
+      learned = true;
+      S.lused++;
+    }
+    else
+    {
+      S.acor  = 0.0;
+      S.delta = d0;
+      S.y     = y0;
+      S.ftemp = f0;
+    }
+    have_res = true;
+  }
+  int flag = g_newton(g, S, callSetup, have_res, learned);
   if (flag != CV_SUCCESS) { return flag; }
+  if (g.a->log_x != nullptr && nflag == FIRST_CALL && (S.nst % g.a->log_stride) == 0)
+  {
+    g_log_pair(g, S);
+  }
   S.y = lsum(1.0, S.zn[0], 1.0, S.acor);
   if (!S.acnrmcur) { S.acnrm = g_wrms(g, S.acor, S.ewt); }
   return flag;
@@ -982,6 +1126,11 @@ __device__ __forceinline__ void g_store(const Grp<M, T>& g, const Sys<M, T>& S, 
     st[SUNAMD_BS_NJE]     = S.nje;
     st[SUNAMD_BS_QLAST]   = S.qu;
     st[SUNAMD_BS_FLAG]    = flag;
+    if (A.lstats != nullptr && S.ltried > 0)
+    {
+      atomicAdd(A.lstats, (unsigned long long)S.ltried);
+      atomicAdd(A.lstats + 1, (unsigned long long)S.lused);
+    }
   }
 }
 
@@ -1025,6 +1174,22 @@ __device__ __forceinline__ bool g_acquire(Grp<M, T>& g, Sys<M, T>& S, double tou
     S.qprime   = 1;
     S.convfail = CV_NO_FAILURES;
     S.pos      = g.i;
+    S.ltried = S.lused = 0;
+    {
+      // learned-predictor input: log10(1 + sum of k_f + k_b over the
+      // reactions touching this species) / 10
+      double ks      = 0.0;
+      const int* idx = A.idx;
+      for (int r = 0; r < A.R; r++)
+      {
+        if (idx[4 * r] == g.i || idx[4 * r + 1] == g.i || idx[4 * r + 2] == g.i ||
+            idx[4 * r + 3] == g.i)
+        {
+          ks += g.kf[r] + g.kb[r];
+        }
+      }
+      S.ratef = g.valid ? (float)(log10(1.0 + ks) * 0.1) : 0.0f;
+    }
     for (int k = 0; k < QMAX + 2; k++) { g.sc->tau[k] = 0.0; }
 
     S.ewt   = g_ewt(g, S.zn[0]);
@@ -1103,6 +1268,7 @@ __global__ void __launch_bounds__(64) k_batched_bdf(BucketArgs A)
   __shared__ T s_prow[64];
   __shared__ int s_inv[64];
   __shared__ GroupScalars s_sc[GPW];
+  extern __shared__ float s_mlp[]; // GPW * MLP_LDS_FLOATS when learned or logging
 
   const int lane = threadIdx.x;
   const int grp  = lane / G;
@@ -1114,6 +1280,7 @@ __global__ void __launch_bounds__(64) k_batched_bdf(BucketArgs A)
   g.prow  = s_prow + g.base;
   g.inv   = s_inv + g.base;
   g.sc    = &s_sc[grp];
+  g.mlp   = (A.mlp_w != nullptr || A.log_x != nullptr) ? s_mlp + grp * MLP_LDS_FLOATS : nullptr;
   g.a     = &A;
   g.cell  = 0;
   g.kf    = A.kf;
@@ -1138,7 +1305,11 @@ __global__ void __launch_bounds__(64) k_batched_bdf(BucketArgs A)
 template <int M, typename T>
 void launch(const BucketArgs& A, int grid, hipStream_t stream)
 {
-  k_batched_bdf<M, T><<<grid, 64, 0, stream>>>(A);
+  constexpr int GPW = 64 / pow2ceil(M);
+  const size_t lds  = (A.mlp_w != nullptr || A.log_x != nullptr)
+                        ? sizeof(float) * (size_t)GPW * MLP_LDS_FLOATS
+                        : 0;
+  k_batched_bdf<M, T><<<grid, 64, lds, stream>>>(A);
 }
 
 } // namespace sunamd_bdf

@@ -1,0 +1,195 @@
+"""The learned Newton initial iterate on the MI355X.
+
+- MLP forward and gradient from the device kernels (MFMA bf16 products, FP32
+  accumulation) against the float64 CPU reference that rounds to bf16 at the
+  same points (tests/mlp_ref.py, itself pinned by finite differences).
+- Adam training on device-resident data reduces the loss of a teacher
+  regression.
+- End to end: training pairs logged by plain FP64 runs of one batch train the
+  network; on a held-out batch the integrator with the learned start (and
+  the zero start as fallback) completes every system with the final state
+  within the integration tolerance of the plain run, and the effect on
+  Newton iterations per step is measured and printed.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from sundials_amd import _lib
+from mlp_ref import NPARAM, IN, IN_PAD, OUT, forward, loss_grad, unpack
+from batch_util import RTOL, ATOL, make_batch, free_batch, run_device, system_slices, MXSTEP
+
+pytestmark = pytest.mark.gpu
+PF = C.POINTER(C.c_float)
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import torch
+
+    assert torch.cuda.is_available()
+    return _lib.hip()
+
+
+def _params(M):
+    p = np.zeros(NPARAM, dtype=np.float32)
+    assert _lib.hip().SUNAMD_MLPGetParams(M, p.ctypes.data_as(PF)) == 0
+    return p
+
+
+def _inputs(rng, B):
+    x = np.zeros((B, IN_PAD), dtype=np.float32)
+    x[:, :IN] = rng.uniform(-1.5, 0.5, (B, IN))
+    return x
+
+
+def test_mlp_forward_matches_reference(hip):
+    M = hip.SUNAMD_MLPCreate(7, 256)
+    assert M
+    try:
+        rng = np.random.default_rng(1)
+        x = _inputs(rng, 64)
+        y = np.zeros((64, OUT), dtype=np.float32)
+        assert hip.SUNAMD_MLPForward(M, x.ctypes.data_as(PF), 64, y.ctypes.data_as(PF)) == 0
+        y_ref, _ = forward(_params(M).astype(np.float64), x)
+        scale = np.abs(y_ref).max()
+        assert np.abs(y - y_ref).max() <= 2e-3 * scale
+    finally:
+        hip.SUNAMD_MLPFree(M)
+
+
+def test_mlp_gradient_matches_reference(hip):
+    M = hip.SUNAMD_MLPCreate(11, 256)
+    assert M
+    try:
+        rng = np.random.default_rng(2)
+        B = 128
+        x = _inputs(rng, B)
+        t = rng.standard_normal((B, OUT)).astype(np.float32)
+        mask = (rng.random((B, OUT)) < 0.6).astype(np.float32)
+        loss = C.c_float()
+        g = np.zeros(NPARAM, dtype=np.float32)
+        assert hip.SUNAMD_MLPLossGrad(M, x.ctypes.data_as(PF), t.ctypes.data_as(PF),
+                                      mask.ctypes.data_as(PF), B, C.byref(loss),
+                                      g.ctypes.data_as(PF)) == 0
+        l_ref, g_ref = loss_grad(_params(M).astype(np.float64), x, t, mask)
+        assert abs(loss.value - l_ref) <= 1e-4 * abs(l_ref)
+        gd, gr = unpack(g), unpack(g_ref)
+        for name in gr:
+            err = np.abs(gd[name] - gr[name]).max()
+            assert err <= 1e-2 * np.abs(gr[name]).max() + 1e-7, (name, err)
+        # W1 columns of the padding inputs get no gradient
+        assert np.all(gd["W1"][:, IN:] == 0.0)
+    finally:
+        hip.SUNAMD_MLPFree(M)
+
+
+def test_mlp_training_reduces_loss(hip):
+    import torch
+
+    M = hip.SUNAMD_MLPCreate(3, 512)
+    T = hip.SUNAMD_MLPCreate(99, 512)  # teacher network
+    assert M and T
+    try:
+        rng = np.random.default_rng(5)
+        n = 8192
+        x = _inputs(rng, n)
+        t = np.zeros((n, OUT), dtype=np.float32)
+        for s in range(0, n, 512):
+            assert hip.SUNAMD_MLPForward(T, x[s:s + 512].ctypes.data_as(PF), 512,
+                                         t[s:s + 512].ctypes.data_as(PF)) == 0
+        t *= 10.0
+        mask = np.ones((n, OUT), dtype=np.float32)
+        xd, td, md = (torch.from_numpy(a).cuda() for a in (x, t, mask))
+        torch.cuda.synchronize()
+        l0, l1 = C.c_float(), C.c_float()
+        assert hip.SUNAMD_MLPTrain(M, C.c_void_p(xd.data_ptr()), C.c_void_p(td.data_ptr()),
+                                   C.c_void_p(md.data_ptr()), n, 512, 10, 1e-3, C.byref(l0)) == 0
+        assert hip.SUNAMD_MLPTrain(M, C.c_void_p(xd.data_ptr()), C.c_void_p(td.data_ptr()),
+                                   C.c_void_p(md.data_ptr()), n, 512, 1500, 1e-3, C.byref(l1)) == 0
+        print("teacher regression loss: %.4g -> %.4g" % (l0.value, l1.value))
+        assert np.isfinite(l1.value) and l1.value < 0.2 * l0.value
+    finally:
+        hip.SUNAMD_MLPFree(M)
+        hip.SUNAMD_MLPFree(T)
+
+
+def _log_pairs(hip, b, cap, stride):
+    import torch
+
+    x = torch.zeros((cap, IN_PAD), dtype=torch.float32, device="cuda")
+    t = torch.zeros((cap, OUT), dtype=torch.float32, device="cuda")
+    m = torch.zeros((cap, OUT), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    B = hip.SUNAMD_BatchedBDFCreate(b, RTOL, ATOL, 0)
+    assert B
+    try:
+        hip.SUNAMD_BatchedBDFSetMaxNumSteps(B, MXSTEP)
+        assert hip.SUNAMD_BatchedBDFSetTrainingLog(B, C.c_void_p(x.data_ptr()),
+                                                   C.c_void_p(t.data_ptr()),
+                                                   C.c_void_p(m.data_ptr()), cap, stride) == 0
+        wall = C.c_double()
+        assert hip.SUNAMD_BatchedBDFRun(B, 1.0, C.byref(wall)) == 0
+        n = hip.SUNAMD_BatchedBDFGetTrainingLogCount(B)
+    finally:
+        hip.SUNAMD_BatchedBDFFree(B)
+    return x[:n], t[:n], m[:n]
+
+
+def test_learned_predictor_end_to_end(hip):
+    import torch
+
+    mv, wv = (8, 16, 32), (1.0, 1.0, 1.0)
+    train = make_batch(1536, 5005, m_values=mv, weights=wv)
+    test = make_batch(512, 6006, m_values=mv, weights=wv)
+    M = hip.SUNAMD_MLPCreate(17, 1024)
+    assert M
+    try:
+        x, t, m = _log_pairs(hip, train, 400000, 1)
+        n = x.shape[0]
+        assert n > 10000
+        # logged rows are finite, masks match the species counts
+        assert torch.isfinite(x).all() and torch.isfinite(t).all()
+        perm = torch.randperm(n, device="cuda", generator=torch.Generator("cuda").manual_seed(0))
+        x, t, m = x[perm].contiguous(), t[perm].contiguous(), m[perm].contiguous()
+        torch.cuda.synchronize()
+        loss = C.c_float()
+        assert hip.SUNAMD_MLPTrain(M, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
+                                   C.c_void_p(m.data_ptr()), n, 1024, 3000, 1e-3,
+                                   C.byref(loss)) == 0
+        print("training pairs %d, final loss %.4g" % (n, loss.value))
+
+        y64, c64, rc, w64 = run_device(test, 0)
+        assert rc == 0
+        B = hip.SUNAMD_BatchedBDFCreate(test, RTOL, ATOL, 0)
+        try:
+            hip.SUNAMD_BatchedBDFSetMaxNumSteps(B, MXSTEP)
+            assert hip.SUNAMD_BatchedBDFSetLearnedPredictor(B, hip.SUNAMD_MLPDeviceWeights(M),
+                                                            0xff) == 0
+            wall = C.c_double()
+            assert hip.SUNAMD_BatchedBDFRun(B, 1.0, C.byref(wall)) == 0
+            bc = test.contents
+            yl = np.zeros(bc.ytotal)
+            cl = np.zeros((bc.nsys, _lib.BS_COUNT), dtype=np.int64)
+            assert hip.SUNAMD_BatchedBDFGetResults(B, yl.ctypes.data_as(C.POINTER(C.c_double)),
+                                                   cl.ctypes.data_as(C.POINTER(C.c_long))) == 0
+            tried, kept = C.c_long(), C.c_long()
+            assert hip.SUNAMD_BatchedBDFGetLearnedStats(B, C.byref(tried), C.byref(kept)) == 0
+        finally:
+            hip.SUNAMD_BatchedBDFFree(B)
+        assert np.all(cl[:, 9] == 0)
+        for (k, c, mm, off) in system_slices(test):
+            d = yl[off:off + mm] - y64[off:off + mm]
+            w = 1.0 / (RTOL * np.abs(y64[off:off + mm]) + ATOL)
+            assert np.sqrt(np.mean((d * w) ** 2)) < 1.0
+        nni_l = cl[:, 4].sum() / cl[:, 0].sum()
+        nni_p = c64[:, 4].sum() / c64[:, 0].sum()
+        print("held-out: Newton iters/step plain %.4f learned %.4f; guess kept %d of %d steps; "
+              "steps plain %d learned %d" % (nni_p, nni_l, kept.value, tried.value,
+                                             c64[:, 0].sum(), cl[:, 0].sum()))
+        assert tried.value > 0
+    finally:
+        hip.SUNAMD_MLPFree(M)
+        free_batch(train)
+        free_batch(test)
