@@ -31,6 +31,7 @@ struct BucketDev
   int m, R;
   long ncells;
   int* idx;
+  int *soff, *sent; // per-species term lists (m+1 offsets, 4R entries)
   double *kf, *kb, *y0, *yout;
   long* stats;
   unsigned long long* next;
@@ -99,6 +100,8 @@ static bool launch_bucket(const SUNAMD_BatchedBDF* B, int k, double tout, int gr
   A.R      = d.R;
   A.ncells = d.ncells;
   A.idx    = d.idx;
+  A.soff   = d.soff;
+  A.sent   = d.sent;
   A.kf     = d.kf;
   A.kb     = d.kb;
   A.y0     = d.y0;
@@ -167,6 +170,45 @@ extern "C" SUNAMD_BatchedBDF* SUNAMD_BatchedBDFCreate(const SUNAMD_Batch* batch,
     if (e == hipSuccess)
     {
       e = hipMemcpy(d.idx, p->idx, sizeof(int) * 4 * d.R, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess)
+    {
+      // species s's RHS / Jacobian-row terms in reaction order, roles a b c d
+      // within a reaction: the order the host evaluation accumulates them in
+      int* off = (int*)calloc((size_t)d.m + 1, sizeof(int));
+      int* ent = (int*)malloc(sizeof(int) * 4 * (size_t)d.R);
+      if (!off || !ent) { e = hipErrorOutOfMemory; }
+      else
+      {
+        for (int r = 0; r < d.R; r++)
+        {
+          for (int t = 0; t < 4; t++) { off[p->idx[4 * r + t] + 1]++; }
+        }
+        for (int q = 0; q < d.m; q++) { off[q + 1] += off[q]; }
+        int* fill = (int*)malloc(sizeof(int) * (size_t)d.m);
+        if (!fill) { e = hipErrorOutOfMemory; }
+        else
+        {
+          memcpy(fill, off, sizeof(int) * (size_t)d.m);
+          for (int r = 0; r < d.R; r++)
+          {
+            for (int t = 0; t < 4; t++) { ent[fill[p->idx[4 * r + t]]++] = 4 * r + t; }
+          }
+          free(fill);
+          e = hipMalloc((void**)&d.soff, sizeof(int) * ((size_t)d.m + 1));
+          if (e == hipSuccess) { e = hipMalloc((void**)&d.sent, sizeof(int) * 4 * (size_t)d.R); }
+          if (e == hipSuccess)
+          {
+            e = hipMemcpy(d.soff, off, sizeof(int) * ((size_t)d.m + 1), hipMemcpyHostToDevice);
+          }
+          if (e == hipSuccess)
+          {
+            e = hipMemcpy(d.sent, ent, sizeof(int) * 4 * (size_t)d.R, hipMemcpyHostToDevice);
+          }
+        }
+      }
+      free(off);
+      free(ent);
     }
     if (e == hipSuccess && d.ncells > 0)
     {
@@ -386,6 +428,8 @@ extern "C" void SUNAMD_BatchedBDFFree(SUNAMD_BatchedBDF* B)
   {
     BucketDev& d = B->b[k];
     if (d.idx) { (void)hipFree(d.idx); }
+    if (d.soff) { (void)hipFree(d.soff); }
+    if (d.sent) { (void)hipFree(d.sent); }
     if (d.kf) { (void)hipFree(d.kf); }
     if (d.kb) { (void)hipFree(d.kb); }
     if (d.y0) { (void)hipFree(d.y0); }

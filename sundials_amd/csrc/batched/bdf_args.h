@@ -20,6 +20,8 @@ struct BucketArgs
   int m, R;
   long ncells;
   const int* idx;     // 4R
+  const int* soff;    // m+1: species s's terms are sent[soff[s] .. soff[s+1])
+  const int* sent;    // 4R terms, 4*reaction + role (0,1 reactant; 2,3 product)
   const double* kf;   // ncells*R, cell-major
   const double* kb;
   const double* y0;   // ncells*m

@@ -200,20 +200,23 @@ __device__ __forceinline__ double g_rhs(const Grp<M, T>& g, double yi)
   gsync();
   if (g.valid) { g.red[g.i] = yi; }
   gsync();
+  // this species' terms, in reaction order and a, b, c, d role order within a
+  // reaction (the per-species list the host built from the network)
   double f       = 0.0;
   const int* idx = g.a->idx;
-  const int R    = g.a->R;
-  for (int r = 0; r < R; r++)
+  if (g.valid)
   {
-    const int ia = idx[4 * r], ib = idx[4 * r + 1], ic = idx[4 * r + 2], id = idx[4 * r + 3];
-    if (ia != g.i && ib != g.i && ic != g.i && id != g.i) { continue; }
-    double wf = (g.kf[r] * g.red[ia]) * g.red[ib];
-    double wb = (g.kb[r] * g.red[ic]) * g.red[id];
-    double w  = wf - wb;
-    if (ia == g.i) { f = f - w; }
-    if (ib == g.i) { f = f - w; }
-    if (ic == g.i) { f = f + w; }
-    if (id == g.i) { f = f + w; }
+    const int e1 = g.a->soff[g.i + 1];
+    for (int e = g.a->soff[g.i]; e < e1; e++)
+    {
+      const int code = g.a->sent[e];
+      const int r = code >> 2, t = code & 3;
+      const int ia = idx[4 * r], ib = idx[4 * r + 1], ic = idx[4 * r + 2], id = idx[4 * r + 3];
+      double wf = (g.kf[r] * g.red[ia]) * g.red[ib];
+      double wb = (g.kb[r] * g.red[ic]) * g.red[id];
+      double w  = wf - wb;
+      f         = (t < 2) ? f - w : f + w;
+    }
   }
   gsync();
   return f;
@@ -230,20 +233,21 @@ __device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, d
 #pragma unroll
   for (int j = 0; j < M; j++) { a[j] = 0.0; }
   const int* idx = g.a->idx;
-  const int R    = g.a->R;
-  for (int r = 0; r < R; r++)
+  if (g.valid)
   {
-    int ix[4] = {idx[4 * r], idx[4 * r + 1], idx[4 * r + 2], idx[4 * r + 3]};
-    if (ix[0] != g.i && ix[1] != g.i && ix[2] != g.i && ix[3] != g.i) { continue; }
-    double kfv   = g.kf[r], kbv = g.kb[r];
-    double dw[4] = {kfv * g.red[ix[1]], kfv * g.red[ix[0]], -(kbv * g.red[ix[3]]),
-                    -(kbv * g.red[ix[2]])};
-    for (int t = 0; t < 4; t++)
+    const int e1 = g.a->soff[g.i + 1];
+    for (int e = g.a->soff[g.i]; e < e1; e++)
     {
-      if (ix[t] != g.i) { continue; }
+      const int code = g.a->sent[e];
+      const int r = code >> 2, t = code & 3;
+      int ix[4]    = {idx[4 * r], idx[4 * r + 1], idx[4 * r + 2], idx[4 * r + 3]};
+      double kfv   = g.kf[r], kbv = g.kb[r];
+      double dw[4] = {kfv * g.red[ix[1]], kfv * g.red[ix[0]], -(kbv * g.red[ix[3]]),
+                      -(kbv * g.red[ix[2]])};
+#pragma unroll
       for (int q = 0; q < 4; q++)
       {
-        const int col = ix[q];
+        const int col  = ix[q];
         const double v = dw[q];
 #pragma unroll
         for (int j = 0; j < M; j++)
@@ -1178,14 +1182,16 @@ __device__ __forceinline__ bool g_acquire(Grp<M, T>& g, Sys<M, T>& S, double tou
     {
       // learned-predictor input: log10(1 + sum of k_f + k_b over the
       // reactions touching this species) / 10
-      double ks      = 0.0;
-      const int* idx = A.idx;
-      for (int r = 0; r < A.R; r++)
+      // (each touching reaction once, however many roles the species has)
+      double ks = 0.0;
+      if (g.valid)
       {
-        if (idx[4 * r] == g.i || idx[4 * r + 1] == g.i || idx[4 * r + 2] == g.i ||
-            idx[4 * r + 3] == g.i)
+        int last = -1;
+        for (int e = A.soff[g.i]; e < A.soff[g.i + 1]; e++)
         {
-          ks += g.kf[r] + g.kb[r];
+          const int r = A.sent[e] >> 2;
+          if (r != last) { ks += g.kf[r] + g.kb[r]; }
+          last = r;
         }
       }
       S.ratef = g.valid ? (float)(log10(1.0 + ks) * 0.1) : 0.0f;
