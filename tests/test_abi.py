@@ -19,7 +19,16 @@ HEADERS = {
     "nvector_mi355.h": "hip",
     "sunmatrix_mi355block.h": "hip",
     "massaction_mi355.h": "hip",
+    "batched_bdf.h": "hip",
+    "sundials_iterative.h": "core",
+    "sunlinsol_spgmr.h": "core",
+    "tridiag_mi355.h": "hip",
+    "learned_predictor.h": "hip",
+    "sunamd_profiler.h": "hip",
 }
+
+# declared in a header whose functions live in the other library
+SPLIT = {"batched_bdf.h": {"SUNAMD_BatchCreate", "SUNAMD_BatchFree"}}
 
 _DECL = re.compile(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", re.M)
 
@@ -42,9 +51,11 @@ def declared(header):
 @pytest.mark.parametrize("header", sorted(HEADERS))
 def test_header_symbols_exported(header):
     lib = _lib.core() if HEADERS[header] == "core" else _lib.hip()
+    other = _lib.hip() if HEADERS[header] == "core" else _lib.core()
     names = declared(header)
     assert names, header
-    missing = [n for n in names if not hasattr(lib, n)]
+    split = SPLIT.get(header, set())
+    missing = [n for n in names if not hasattr(other if n in split else lib, n)]
     assert not missing, missing
 
 
