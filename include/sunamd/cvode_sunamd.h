@@ -94,6 +94,8 @@ int CVodeSetNonlinearSolver(void* cvode_mem, SUNNonlinearSolver NLS);
 int CVodeSetLinearSolver(void* cvode_mem, SUNLinearSolver LS, SUNMatrix A);
 int CVodeSetJacFn(void* cvode_mem, CVLsJacFn jac);
 int CVodeSetInitialCorrectionFn(void* cvode_mem, CVInitialCorrectionFn fn);
+/* cvode_io.c:1006: fused error-weight and residual kernels (MI355X vectors) */
+int CVodeSetUseIntegratorFusedKernels(void* cvode_mem, sunbooleantype onoff);
 
 int CVode(void* cvode_mem, sunrealtype tout, N_Vector yout, sunrealtype* tret,
           int itask);

@@ -63,6 +63,9 @@ SUNAMD_CoupledRun* SUNAMD_CoupledCreate_Mi355(const SUNAMD_MassAction* p, double
 int SUNAMD_CoupledRun_Mi355(SUNAMD_CoupledRun* r, double tout, SUNAMD_SolveStats* stats);
 int SUNAMD_CoupledGetState_Mi355(SUNAMD_CoupledRun* r, double* yout);
 void SUNAMD_CoupledFree_Mi355(SUNAMD_CoupledRun* r);
+/* switch the integrator onto the fused error-weight / residual kernels
+   (CVodeSetUseIntegratorFusedKernels); results are unchanged bit for bit */
+int SUNAMD_CoupledSetFusedKernels_Mi355(SUNAMD_CoupledRun* r, int onoff);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,22 @@ long int N_VGetNumLaunches_Mi355(N_Vector v);
 #define SUNAMD_REDUCE_SEQUENTIAL 1
 SUNErrCode N_VSetReductionOrder_Mi355(N_Vector v, int order);
 
+/* 1 if v is an MI355X vector of this library, else 0 (no counterpart) */
+int SUNAMD_NVectorIsMi355(N_Vector v);
+
+/* CVODE's fused error-weight and Newton-residual kernels for this vector,
+   with the arguments of cvEwtSetSS_fused / cvEwtSetSV_fused /
+   cvNlsResid_fused (src/cvode/cvode_fused_gpu.cpp:81, 131, 232); results equal
+   the unfused op sequences bit for bit.  They return -1 for a vector that is
+   not an MI355X vector.  CVodeSetUseIntegratorFusedKernels switches the
+   repo's CVODE onto them. */
+int cvEwtSetSS_fused_Mi355(sunbooleantype atolMin0, sunrealtype reltol, sunrealtype Sabstol,
+                           N_Vector ycur, N_Vector tempv, N_Vector weight);
+int cvEwtSetSV_fused_Mi355(sunbooleantype atolMin0, sunrealtype reltol, N_Vector Vabstol,
+                           N_Vector ycur, N_Vector tempv, N_Vector weight);
+int cvNlsResid_fused_Mi355(sunrealtype rl1, sunrealtype ngamma, N_Vector zn1, N_Vector ycor,
+                           N_Vector ftemp, N_Vector res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -219,6 +219,8 @@ def hip():
     _sig(lib, "SUNAMD_CoupledRun_Mi355", C.c_int, [c_void_p, c_real, P(SolveStats)])
     _sig(lib, "SUNAMD_CoupledGetState_Mi355", C.c_int, [c_void_p, P(c_real)])
     _sig(lib, "SUNAMD_CoupledFree_Mi355", None, [c_void_p])
+    _sig(lib, "SUNAMD_CoupledSetFusedKernels_Mi355", C.c_int, [c_void_p, C.c_int])
+    _sig(lib, "SUNAMD_NVectorIsMi355", C.c_int, [c_void_p])
     _sig(lib, "SUNLinSol_Mi355BlockLU_GetPivots", C.c_int, [c_void_p, P(C.c_int)])
     _sig(lib, "SUNAMD_ProfilerEnable", None, [C.c_int])
     _sig(lib, "SUNAMD_ProfilerReset", None, [])

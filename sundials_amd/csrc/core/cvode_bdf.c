@@ -28,6 +28,8 @@
  * Not restated: Adams, rootfinding, projection, constraints, stability limit
  * detection, difference-quotient Jacobians, logging and monitoring.
  * ---------------------------------------------------------------------------*/
+#define _GNU_SOURCE
+#include <dlfcn.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -201,6 +203,12 @@ typedef struct CVodeMemRec
   LsState ls;
 
   CVInitialCorrectionFn initcor;
+
+  /* fused error-weight / residual kernels of the vector (cvode_io.c:1006-1031) */
+  sunbooleantype usefused;
+  int (*fused_ewt_ss)(sunbooleantype, sunrealtype, sunrealtype, N_Vector, N_Vector, N_Vector);
+  int (*fused_ewt_sv)(sunbooleantype, sunrealtype, N_Vector, N_Vector, N_Vector, N_Vector);
+  int (*fused_resid)(sunrealtype, sunrealtype, N_Vector, N_Vector, N_Vector, N_Vector);
 
   sunbooleantype MallocDone;
 } CVodeMemRec;
@@ -514,6 +522,35 @@ int CVodeSetStopTime(void* cvode_mem, sunrealtype tstop)
   return CV_SUCCESS;
 }
 
+/* The reference enables its HIP/CUDA fused kernels only for its own GPU
+   vectors (cvode_io.c:1006-1031).  Here the kernels are looked up in the
+   loaded MI355X library and enabled only when the state vector is one of its
+   vectors; otherwise the call fails and the unfused op sequences stay on. */
+int CVodeSetUseIntegratorFusedKernels(void* cvode_mem, sunbooleantype onoff)
+{
+  CVodeMem cv = (CVodeMem)cvode_mem;
+  if (!cv) { return CV_MEM_NULL; }
+  if (!onoff)
+  {
+    cv->usefused = SUNFALSE;
+    return CV_SUCCESS;
+  }
+  if (!cv->MallocDone) { return CV_ILL_INPUT; }
+  int (*is_mi355)(N_Vector) = (int (*)(N_Vector))dlsym(RTLD_DEFAULT, "SUNAMD_NVectorIsMi355");
+  void* ss = dlsym(RTLD_DEFAULT, "cvEwtSetSS_fused_Mi355");
+  void* sv = dlsym(RTLD_DEFAULT, "cvEwtSetSV_fused_Mi355");
+  void* rs = dlsym(RTLD_DEFAULT, "cvNlsResid_fused_Mi355");
+  if (!is_mi355 || !ss || !sv || !rs || !is_mi355(cv->ewt)) { return CV_ILL_INPUT; }
+  cv->fused_ewt_ss = (int (*)(sunbooleantype, sunrealtype, sunrealtype, N_Vector, N_Vector,
+                              N_Vector))ss;
+  cv->fused_ewt_sv = (int (*)(sunbooleantype, sunrealtype, N_Vector, N_Vector, N_Vector,
+                              N_Vector))sv;
+  cv->fused_resid  = (int (*)(sunrealtype, sunrealtype, N_Vector, N_Vector, N_Vector,
+                              N_Vector))rs;
+  cv->usefused     = SUNTRUE;
+  return CV_SUCCESS;
+}
+
 int CVodeSetInitialCorrectionFn(void* cvode_mem, CVInitialCorrectionFn fn)
 {
   CVodeMem cv = (CVodeMem)cvode_mem;
@@ -802,6 +839,12 @@ static int cvNlsResidual(N_Vector ycor, N_Vector res, void* mem)
   cv->nfe++;
   if (retval < 0) { return CV_RHSFUNC_FAIL; }
   if (retval > 0) { return RHSFUNC_RECVR; }
+  if (cv->usefused)
+  {
+    /* cvode_nls.c:385-391 */
+    cv->fused_resid(cv->rl1, -cv->gamma, cv->zn[1], ycor, cv->ftemp, res);
+    return CV_SUCCESS;
+  }
   N_VLinearSum(cv->rl1, cv->zn[1], ONE, ycor, res);
   N_VLinearSum(-cv->gamma, cv->ftemp, ONE, res, res);
   return CV_SUCCESS;
@@ -813,6 +856,18 @@ static int cvNlsResidual(N_Vector ycor, N_Vector res, void* mem)
 
 static int cvEwtSet(N_Vector ycur, N_Vector weight, CVodeMem cv)
 {
+  if (cv->usefused && (cv->itol == CV_SS || cv->itol == CV_SV))
+  {
+    /* cvode.c:4794-4806, 4834-4846: weights are formed regardless of the
+       component test, which then looks at tempv */
+    if (cv->itol == CV_SS)
+    {
+      cv->fused_ewt_ss(cv->atolmin0, cv->reltol, cv->Sabstol, ycur, cv->tempv, weight);
+    }
+    else { cv->fused_ewt_sv(cv->atolmin0, cv->reltol, cv->Vabstol, ycur, cv->tempv, weight); }
+    if (cv->atolmin0 && N_VMin(cv->tempv) <= ZERO) { return -1; }
+    return 0;
+  }
   switch (cv->itol)
   {
   case CV_SS:

@@ -1,0 +1,133 @@
+// -----------------------------------------------------------------------------
+// CVODE's fused error-weight and Newton-residual kernels on the MI355X vector:
+// the counterparts of cvEwtSetSS_fused / cvEwtSetSV_fused / cvNlsResid_fused
+// (src/cvode/cvode_fused_gpu.cpp:62-255), with the same arguments.  Each
+// element is formed with the separate roundings of the unfused op sequence
+// it replaces (Abs, Scale, AddConst, Inv -- or two LinearSums with the serial
+// vector's three rounding forms), so switching the fused kernels on does not
+// change a single bit of the integration; it saves launches and passes:
+// error weights 64 -> 24 bytes per element, residual 48 -> 32.
+// -----------------------------------------------------------------------------
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "sunamd/nvector_mi355.h"
+#include "sunamd/sundials_core.h"
+
+extern "C" void sunamd_nv_count_launch(N_Vector v);
+
+namespace
+{
+
+__device__ __forceinline__ double lsum3(double a, double x, double b, double y)
+{
+  if (a == b) { return a * (x + y); }
+  if (a == -b) { return a * (x - y); }
+  return (a * x) + (b * y);
+}
+
+__global__ void k_ewt_ss(double rtol, double atol, const double* __restrict__ y,
+                         double* tempv, double* w, long n)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+  {
+    double t = fabs(y[i]);
+    t        = rtol * t;
+    t        = t + atol;
+    tempv[i] = t;
+    w[i]     = 1.0 / t;
+  }
+}
+
+__global__ void k_ewt_sv(double rtol, const double* __restrict__ atol,
+                         const double* __restrict__ y, double* tempv, double* w, long n)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+  {
+    double t = lsum3(rtol, fabs(y[i]), 1.0, atol[i]);
+    tempv[i] = t;
+    w[i]     = 1.0 / t;
+  }
+}
+
+__global__ void k_nls_resid(double rl1, double ngamma, const double* __restrict__ zn1,
+                            const double* __restrict__ ycor, const double* __restrict__ ftemp,
+                            double* res, long n)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+  {
+    double r = lsum3(rl1, zn1[i], 1.0, ycor[i]);
+    res[i]   = lsum3(ngamma, ftemp[i], 1.0, r);
+  }
+}
+
+unsigned grid_for(long n)
+{
+  long g = (n + 255) / 256;
+  return (unsigned)(g < 4096 ? (g > 0 ? g : 1) : 4096);
+}
+
+int done(const char* what)
+{
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+  {
+    fprintf(stderr, "[sundials_amd cvode_fused] %s: %s\n", what, hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+} // namespace
+
+extern "C" int cvEwtSetSS_fused_Mi355(sunbooleantype atolMin0, sunrealtype reltol,
+                                      sunrealtype Sabstol, N_Vector ycur, N_Vector tempv,
+                                      N_Vector weight)
+{
+  (void)atolMin0; // the caller applies the N_VMin test to tempv
+  if (!SUNAMD_NVectorIsMi355(weight)) { return -1; }
+  const long n = (long)N_VGetLength(weight);
+  if (n == 0) { return 0; }
+  hipStream_t s = (hipStream_t)N_VGetStream_Mi355(weight);
+  k_ewt_ss<<<grid_for(n), 256, 0, s>>>(reltol, Sabstol, N_VGetDeviceArrayPointer(ycur),
+                                       N_VGetDeviceArrayPointer(tempv),
+                                       N_VGetDeviceArrayPointer(weight), n);
+  sunamd_nv_count_launch(weight);
+  return done("ewt SS");
+}
+
+extern "C" int cvEwtSetSV_fused_Mi355(sunbooleantype atolMin0, sunrealtype reltol,
+                                      N_Vector Vabstol, N_Vector ycur, N_Vector tempv,
+                                      N_Vector weight)
+{
+  (void)atolMin0;
+  if (!SUNAMD_NVectorIsMi355(weight)) { return -1; }
+  const long n = (long)N_VGetLength(weight);
+  if (n == 0) { return 0; }
+  hipStream_t s = (hipStream_t)N_VGetStream_Mi355(weight);
+  k_ewt_sv<<<grid_for(n), 256, 0, s>>>(reltol, N_VGetDeviceArrayPointer(Vabstol),
+                                       N_VGetDeviceArrayPointer(ycur),
+                                       N_VGetDeviceArrayPointer(tempv),
+                                       N_VGetDeviceArrayPointer(weight), n);
+  sunamd_nv_count_launch(weight);
+  return done("ewt SV");
+}
+
+extern "C" int cvNlsResid_fused_Mi355(sunrealtype rl1, sunrealtype ngamma, N_Vector zn1,
+                                      N_Vector ycor, N_Vector ftemp, N_Vector res)
+{
+  if (!SUNAMD_NVectorIsMi355(res)) { return -1; }
+  const long n = (long)N_VGetLength(res);
+  if (n == 0) { return 0; }
+  hipStream_t s = (hipStream_t)N_VGetStream_Mi355(res);
+  k_nls_resid<<<grid_for(n), 256, 0, s>>>(rl1, ngamma, N_VGetDeviceArrayPointer(zn1),
+                                          N_VGetDeviceArrayPointer(ycor),
+                                          N_VGetDeviceArrayPointer(ftemp),
+                                          N_VGetDeviceArrayPointer(res), n);
+  sunamd_nv_count_launch(res);
+  return done("residual");
+}

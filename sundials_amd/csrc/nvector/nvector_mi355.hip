@@ -757,6 +757,14 @@ static void mi_linearsum(sunrealtype a, N_Vector x, sunrealtype b, N_Vector y, N
   LAUNCHCHK(z);
 }
 
+// launch accounting for kernels issued on a vector's stream from other files
+extern "C" void sunamd_nv_count_launch(N_Vector v) { EXEC(v)->launches++; }
+
+extern "C" int SUNAMD_NVectorIsMi355(N_Vector v)
+{
+  return (v && v->ops && v->ops->nvlinearsum == mi_linearsum) ? 1 : 0;
+}
+
 static void mi_const(sunrealtype c, N_Vector z)
 {
   long n = (long)NVLEN(z);

@@ -261,6 +261,12 @@ struct SUNAMD_CoupledRun_
   void* cv;
 };
 
+extern "C" int SUNAMD_CoupledSetFusedKernels_Mi355(SUNAMD_CoupledRun* r, int onoff)
+{
+  if (!r || !r->cv) { return -1; }
+  return CVodeSetUseIntegratorFusedKernels(r->cv, onoff ? SUNTRUE : SUNFALSE);
+}
+
 extern "C" void SUNAMD_CoupledFree_Mi355(SUNAMD_CoupledRun* r)
 {
   if (!r) { return; }

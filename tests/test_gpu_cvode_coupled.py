@@ -179,3 +179,30 @@ def test_coupled_fp32_lu_accuracy(libs):
     assert st.flag == 0
     e32, e_cpu = _wrms_err(y32, y_ref), _wrms_err(y_cpu, y_ref)
     assert e32 <= 2.0 * e_cpu + 0.1, (e32, e_cpu)
+
+
+@pytest.mark.parametrize("order", [SEQ, TREE])
+def test_coupled_fused_kernels_change_nothing(libs, order):
+    """CVODE's fused error-weight and residual kernels (the counterparts of
+    cvode_fused_gpu.cpp) form every element with the roundings of the op
+    sequences they replace: with them on, the integration is bit-identical
+    to the unfused run (and, in sequential order, to the oracle)."""
+    core, hip, orc = libs
+    p = _prob(core, 8, 16, 257, 1001)
+    n = 257 * 8
+    runs = []
+    for fused in (0, 1):
+        r = hip.SUNAMD_CoupledCreate_Mi355(p, 1e-6, 1e-10, 0, order)
+        assert r
+        assert hip.SUNAMD_CoupledSetFusedKernels_Mi355(r, fused) == 0
+        st = _lib.SolveStats()
+        assert hip.SUNAMD_CoupledRun_Mi355(r, 1.0, C.byref(st)) == 0
+        y = np.zeros(n)
+        assert hip.SUNAMD_CoupledGetState_Mi355(r, dp(y)) == 0
+        runs.append((_counts(st), st.launches, y))
+        hip.SUNAMD_CoupledFree_Mi355(r)
+    core.SUNAMD_MassActionFree(p)
+    (c0, l0, y0), (c1, l1, y1) = runs
+    assert c0 == c1
+    assert np.array_equal(y0, y1)
+    assert l1 < l0  # fewer launches
