@@ -136,12 +136,12 @@ struct Sys
   double rl1, gamma, gammap, gamrat, crate, delp, acnrm;
   double etaqm1, etaq, etaqp1, etamax, saved_tq5;
   int q, qprime, qwait, L, indx_acor, convfail, qu;
-  long nst, nfe, ncfn, netf, nni, nnf, nsetups, nstlp, nje, nstlj;
+  int nst, nfe, ncfn, netf, nni, nnf, nsetups, nstlp, nje, nstlj;
   bool nls_jcur, acnrmcur;
   // learned initial iterate: this species' rate feature, and counts of the
   // steps where a guess was tried / kept
   float ratef;
-  long ltried, lused;
+  int ltried, lused;
 };
 
 template <int M, typename T>
@@ -225,7 +225,8 @@ __device__ __forceinline__ double g_rhs(const Grp<M, T>& g, double yi)
 // ---- Jacobian row i in reaction / target / column order, then the Newton
 // matrix row (I - gamma*J)_i with the dense ScaleAddI rounding --------------
 template <int M, typename T>
-__device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, double gamma, double (&a)[M])
+__device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, double gamma,
+                                                double (&a)[M])
 {
   gsync();
   if (g.valid) { g.red[g.i] = yj; }
@@ -269,11 +270,9 @@ __device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, d
 
 // ---- LU of the group's matrix (rows in lanes, implicit pivoting) ---------
 template <int M, typename T>
-__device__ __forceinline__ int g_getrf(const Grp<M, T>& g, Sys<M, T>& S, const double (&a)[M])
+__device__ __forceinline__ int g_getrf(const Grp<M, T>& g, Sys<M, T>& S)
 {
   constexpr int G = pow2ceil(M);
-#pragma unroll
-  for (int j = 0; j < M; j++) { S.lu[j] = (T)a[j]; }
   int pos  = g.i;
   int fail = 0;
 #pragma unroll
@@ -572,24 +571,30 @@ __device__ __forceinline__ int g_lsetup(const Grp<M, T>& g, Sys<M, T>& S, bool j
                  ((S.convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_JBAD)) ||
                  (S.convfail == CV_FAIL_OTHER);
   bool jcur;
-  double a[M];
-  if (!ls_jbad)
-  {
-    jcur = false;
-    g_newton_matrix(g, S.ysav, S.gamma, a); // saved J rebuilt from its state
-  }
+  if (!ls_jbad) { jcur = false; } // saved J rebuilt from the state it was taken at
   else
   {
     jcur   = true;
     S.ysav = S.y;
+  }
+  if constexpr (sizeof(T) == sizeof(double))
+  {
+    // FP64: the Newton-matrix row is built in the factor registers themselves
+    g_newton_matrix(g, S.ysav, S.gamma, *reinterpret_cast<double(*)[M]>(&S.lu[0]));
+  }
+  else
+  {
+    double a[M];
     g_newton_matrix(g, S.ysav, S.gamma, a);
+#pragma unroll
+    for (int j = 0; j < M; j++) { S.lu[j] = (T)a[j]; }
   }
   if (jcur)
   {
     S.nje++;
     S.nstlj = S.nst;
   }
-  int fail = g_getrf(g, S, a);
+  int fail = g_getrf(g, S);
   S.nsetups++;
   S.nls_jcur = jcur;
   S.gamrat   = 1.0;
