@@ -47,6 +47,8 @@ RTOL, ATOL, TOUT = 1e-6, 1e-10, 1.0
 BATCH_M = (4, 8, 16, 24, 32, 48, 64)
 BATCH_W = (30.0, 25.0, 15.0, 10.0, 10.0, 5.0, 5.0)
 BATCH_SEED = 1004
+TRAIN_SEED = 2004      # independent batch whose plain runs give the training pairs
+CALIB_SEED = 3004      # independent batch for the per-bucket on/off choice
 BATCH_MXSTEP = 100000
 LU_FP64, LU_FP32 = 0, 1
 
@@ -242,6 +244,81 @@ def batch_cpu_baseline(nsys_sample, threads):
                                                  os.cpu_count() or 0)}
 
 
+def _batch(core, nsys, seed):
+    n = len(BATCH_M)
+    return core.SUNAMD_BatchCreate(nsys, seed, n, (C.c_int * n)(*BATCH_M),
+                                   (C.c_double * n)(*BATCH_W), 1.5, 0.2)
+
+
+def train_predictor(args, core, hip):
+    """Training pairs from plain FP64 runs of an independent batch, then Adam
+    on the GPU with the library's MFMA kernels.  Returns (network, info)."""
+    import torch
+
+    t0 = time.perf_counter()
+    bt = _batch(core, args.train_systems, TRAIN_SEED)
+    cap = args.train_pairs
+    x = torch.zeros((cap, 144), dtype=torch.float32, device="cuda")
+    t = torch.zeros((cap, 64), dtype=torch.float32, device="cuda")
+    m = torch.zeros((cap, 64), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    B = hip.SUNAMD_BatchedBDFCreate(bt, RTOL, ATOL, LU_FP64)
+    hip.SUNAMD_BatchedBDFSetMaxNumSteps(B, BATCH_MXSTEP)
+    hip.SUNAMD_BatchedBDFSetTrainingLog(B, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
+                                        C.c_void_p(m.data_ptr()), cap, 1)
+    wall = C.c_double()
+    if hip.SUNAMD_BatchedBDFRun(B, TOUT, C.byref(wall)) != 0:
+        raise RuntimeError("logging run failed")
+    n = int(hip.SUNAMD_BatchedBDFGetTrainingLogCount(B))
+    hip.SUNAMD_BatchedBDFFree(B)
+    core.SUNAMD_BatchFree(bt)
+    perm = torch.randperm(n, device="cuda", generator=torch.Generator("cuda").manual_seed(0))
+    x, t, m = x[:n][perm].contiguous(), t[:n][perm].contiguous(), m[:n][perm].contiguous()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    M = hip.SUNAMD_MLPCreate(17, 1024)
+    loss = C.c_float()
+    if hip.SUNAMD_MLPTrain(M, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
+                           C.c_void_p(m.data_ptr()), n, 1024, args.train_steps, 1e-3,
+                           C.byref(loss)) != 0:
+        raise RuntimeError("training failed")
+    t2 = time.perf_counter()
+    del x, t, m
+    info = {"pairs": n, "train_systems": args.train_systems, "steps": args.train_steps,
+            "batch": 1024, "loss": float(loss.value), "log_s": round(t1 - t0, 2),
+            "train_s": round(t2 - t1, 2)}
+    print("[bench] predictor: %d pairs, loss %.4g, train %.1f s" % (n, loss.value, t2 - t1),
+          file=sys.stderr, flush=True)
+    return M, info
+
+
+def choose_learned_buckets(core, hip, weights):
+    """Per-bucket choice of the learned start: an independent calibration
+    batch is integrated with and without it, and each bucket keeps the
+    faster setting (kernel time of the bucket's launch)."""
+    nb = len(BATCH_M)
+    bcal = _batch(core, 8192, CALIB_SEED)
+    ms = []
+    for mask in (0, (1 << nb) - 1):
+        B = hip.SUNAMD_BatchedBDFCreate(bcal, RTOL, ATOL, LU_FP32)
+        hip.SUNAMD_BatchedBDFSetMaxNumSteps(B, BATCH_MXSTEP)
+        hip.SUNAMD_BatchedBDFSetLearnedPredictor(B, weights, mask)
+        wall = C.c_double()
+        hip.SUNAMD_BatchedBDFRun(B, TOUT, C.byref(wall))
+        hip.SUNAMD_BatchedBDFRun(B, TOUT, C.byref(wall))
+        kms = np.zeros(nb)
+        hip.SUNAMD_BatchedBDFGetKernelTimes(B, kms.ctypes.data_as(C.POINTER(C.c_double)))
+        hip.SUNAMD_BatchedBDFFree(B)
+        ms.append(kms)
+    core.SUNAMD_BatchFree(bcal)
+    mask = 0
+    for k in range(nb):
+        if ms[1][k] < ms[0][k]:
+            mask |= 1 << k
+    return mask, {"plain_ms": [round(v, 3) for v in ms[0]],
+                  "learned_ms": [round(v, 3) for v in ms[1]]}
+
+
 def bench_batched(args):
     import torch
     from sundials_amd import _lib
@@ -293,8 +370,19 @@ def bench_batched(args):
     nst64 = int(c64[:, 0].sum())
     print("[bench] plain fp64: %.3f s, %d cell-steps" % (t64, nst64), file=sys.stderr, flush=True)
 
-    # product form: FP32 Newton-matrix LU
+    # product form: FP32 Newton-matrix LU + learned initial iterate where it pays
+    learned = None
+    mlp = None
+    mask = 0
+    if not args.no_learned:
+        mlp, learned = train_predictor(args, core, hip)
+        mask, calib = choose_learned_buckets(core, hip, hip.SUNAMD_MLPDeviceWeights(mlp))
+        learned["bucket_mask"] = mask
+        learned["buckets_on"] = [int(BATCH_M[k]) for k in range(len(BATCH_M)) if mask >> k & 1]
+        learned["calibration"] = calib
     B = make(LU_FP32)
+    if mlp is not None:
+        hip.SUNAMD_BatchedBDFSetLearnedPredictor(B, hip.SUNAMD_MLPDeviceWeights(mlp), mask)
     for w in range(args.warmup):
         tw = run(B, "warmup")
         print("[bench] warmup %d: %.3f s" % (w, tw), file=sys.stderr, flush=True)
@@ -309,7 +397,13 @@ def bench_batched(args):
     y32, c32 = results(B)
     kms = np.zeros(nb)
     hip.SUNAMD_BatchedBDFGetKernelTimes(B, kms.ctypes.data_as(C.POINTER(C.c_double)))
+    if learned is not None:
+        tried, kept = C.c_long(), C.c_long()
+        hip.SUNAMD_BatchedBDFGetLearnedStats(B, C.byref(tried), C.byref(kept))
+        learned["steps_tried"], learned["guesses_kept"] = tried.value, kept.value
     hip.SUNAMD_BatchedBDFFree(B)
+    if mlp is not None:
+        hip.SUNAMD_MLPFree(mlp)
     nst32 = int(c32[:, 0].sum())
 
     # roofline: the longest bucket kernel of the last timed run
@@ -359,10 +453,12 @@ def bench_batched(args):
         "data": "synthetic",
         "config": {"workload": "uneven batch: %d independent mass-action systems, m in %s "
                                "(weights %s %%), R = 1.5 m, per-system BDF, rtol %g atol %g, "
-                               "t in [0,%g], FP32 Newton-matrix LU, polynomial predictor"
+                               "t in [0,%g], FP32 Newton-matrix LU"
                                % (args.systems, list(BATCH_M), [int(x) for x in BATCH_W],
                                   RTOL, ATOL, TOUT),
-                   "systems": args.systems, "predictor": "polynomial (learned not enabled)"},
+                   "systems": args.systems,
+                   "predictor": "learned where calibrated faster, else polynomial"
+                                if learned is not None else "polynomial"},
         "newton_iters_per_step": c32[:, 4].sum() / max(nst32, 1),
         "cell_steps": nst32,
         "plain_fp64": {"value": v64, "ms": 1e3 * t64, "cell_steps": nst64,
@@ -371,6 +467,7 @@ def bench_batched(args):
         "error_vs_fp64": {"max_weighted_rms": float(werr.max()),
                           "median_weighted_rms": float(np.median(werr)),
                           "max_rel": relmax},
+        "learned": learned,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
@@ -392,6 +489,10 @@ def main():
     ap.add_argument("--cpu-cells", type=int, default=1024)
     ap.add_argument("--no-micro", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train-systems", type=int, default=4096)
+    ap.add_argument("--train-pairs", type=int, default=1500000)
+    ap.add_argument("--train-steps", type=int, default=3000)
     args = ap.parse_args()
 
     if args.gpus != 1:
