@@ -292,31 +292,44 @@ def train_predictor(args, core, hip):
     return M, info
 
 
-def choose_learned_buckets(core, hip, weights):
-    """Per-bucket choice of the learned start: an independent calibration
-    batch is integrated with and without it, and each bucket keeps the
-    faster setting (kernel time of the bucket's launch)."""
+def autotune(core, hip, prec, weights=None):
+    """Per-bucket choices made on an independent calibration batch (8192
+    systems): the kernel's register budget (1, 2 or 4 wavefronts per SIMD,
+    i.e. occupancy against spilling) and, given a trained network, whether
+    the learned start pays off.  Each choice keeps the bucket's shorter
+    kernel time."""
     nb = len(BATCH_M)
     bcal = _batch(core, 8192, CALIB_SEED)
-    ms = []
-    for mask in (0, (1 << nb) - 1):
-        B = hip.SUNAMD_BatchedBDFCreate(bcal, RTOL, ATOL, LU_FP32)
+
+    def kernel_ms(waves, mask):
+        B = hip.SUNAMD_BatchedBDFCreate(bcal, RTOL, ATOL, prec)
         hip.SUNAMD_BatchedBDFSetMaxNumSteps(B, BATCH_MXSTEP)
-        hip.SUNAMD_BatchedBDFSetLearnedPredictor(B, weights, mask)
+        hip.SUNAMD_BatchedBDFSetOccupancy(B, (C.c_int * nb)(*waves))
+        if weights is not None:
+            hip.SUNAMD_BatchedBDFSetLearnedPredictor(B, weights, mask)
         wall = C.c_double()
-        hip.SUNAMD_BatchedBDFRun(B, TOUT, C.byref(wall))
-        hip.SUNAMD_BatchedBDFRun(B, TOUT, C.byref(wall))
+        for _ in range(2):
+            if hip.SUNAMD_BatchedBDFRun(B, TOUT, C.byref(wall)) != 0:
+                raise RuntimeError("calibration run failed")
         kms = np.zeros(nb)
         hip.SUNAMD_BatchedBDFGetKernelTimes(B, kms.ctypes.data_as(C.POINTER(C.c_double)))
         hip.SUNAMD_BatchedBDFFree(B)
-        ms.append(kms)
-    core.SUNAMD_BatchFree(bcal)
+        return kms
+
+    table = {w: kernel_ms([w] * nb, 0) for w in (1, 2, 4)}
+    best_w = [min((1, 2, 4), key=lambda w: table[w][k]) for k in range(nb)]
+    best = np.array([table[best_w[k]][k] for k in range(nb)])
+    info = {"waves": best_w, "ms_by_waves": {str(w): [round(v, 2) for v in table[w]]
+                                            for w in table}}
     mask = 0
-    for k in range(nb):
-        if ms[1][k] < ms[0][k]:
-            mask |= 1 << k
-    return mask, {"plain_ms": [round(v, 3) for v in ms[0]],
-                  "learned_ms": [round(v, 3) for v in ms[1]]}
+    if weights is not None:
+        lms = kernel_ms(best_w, (1 << nb) - 1)
+        for k in range(nb):
+            if lms[k] < best[k]:
+                mask |= 1 << k
+        info["learned_ms"] = [round(v, 2) for v in lms]
+    core.SUNAMD_BatchFree(bcal)
+    return best_w, mask, info
 
 
 def bench_batched(args):
@@ -335,12 +348,13 @@ def bench_batched(args):
     sb = np.ctypeslib.as_array(bc.sys_bucket, shape=(bc.nsys,)).copy()
     msys, rsys = bm[sb], bR[sb]
 
-    def make(prec):
+    def make(prec, waves):
         B = hip.SUNAMD_BatchedBDFCreate(b, RTOL, ATOL, prec)
         if not B:
             print("[bench] device setup failed", file=sys.stderr)
             sys.exit(1)
         hip.SUNAMD_BatchedBDFSetMaxNumSteps(B, BATCH_MXSTEP)
+        hip.SUNAMD_BatchedBDFSetOccupancy(B, (C.c_int * nb)(*waves))
         return B
 
     def results(B):
@@ -361,8 +375,10 @@ def bench_batched(args):
             sys.exit(1)
         return wall.value
 
-    # plain FP64 form: warm up, then time one integration
-    B64 = make(LU_FP64)
+    # plain FP64 form (register budget tuned per bucket): warm up, then time one integration
+    w64, _, tune64 = autotune(core, hip, LU_FP64)
+    print("[bench] fp64 waves per bucket: %s" % w64, file=sys.stderr, flush=True)
+    B64 = make(LU_FP64, w64)
     run(B64, "fp64 warmup")
     t64 = run(B64, "fp64")
     y64, c64 = results(B64)
@@ -374,13 +390,17 @@ def bench_batched(args):
     learned = None
     mlp = None
     mask = 0
+    weights = None
     if not args.no_learned:
         mlp, learned = train_predictor(args, core, hip)
-        mask, calib = choose_learned_buckets(core, hip, hip.SUNAMD_MLPDeviceWeights(mlp))
+        weights = hip.SUNAMD_MLPDeviceWeights(mlp)
+    w32, mask, tune32 = autotune(core, hip, LU_FP32, weights)
+    print("[bench] fp32 waves per bucket: %s, learned mask %d" % (w32, mask), file=sys.stderr,
+          flush=True)
+    if learned is not None:
         learned["bucket_mask"] = mask
         learned["buckets_on"] = [int(BATCH_M[k]) for k in range(len(BATCH_M)) if mask >> k & 1]
-        learned["calibration"] = calib
-    B = make(LU_FP32)
+    B = make(LU_FP32, w32)
     if mlp is not None:
         hip.SUNAMD_BatchedBDFSetLearnedPredictor(B, hip.SUNAMD_MLPDeviceWeights(mlp), mask)
     for w in range(args.warmup):
@@ -416,7 +436,7 @@ def bench_batched(args):
     dom = int(np.argmax(kms))
     dflop = float(fl[sb == dom].sum())
     achieved = dflop / (kms[dom] * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "kernel": "k_batched_bdf<%d, float>" % bm[dom],
+    roofline = {"bound": "mfma", "kernel": "k_batched_bdf<%d, float, %d>" % (bm[dom], w32[dom]),
                 "achieved": round(achieved, 4), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFS, 5), "traffic": None,
                 "flops_per_launch": dflop, "launch_ms": round(float(kms[dom]), 3),
@@ -468,6 +488,7 @@ def bench_batched(args):
                           "median_weighted_rms": float(np.median(werr)),
                           "max_rel": relmax},
         "learned": learned,
+        "autotune": {"fp64": tune64, "fp32": tune32},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

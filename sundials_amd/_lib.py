@@ -235,6 +235,7 @@ def hip():
     _sig(lib, "SUNAMD_BatchedBDFGetResults", C.c_int, [c_void_p, P(c_real), P(C.c_long)])
     _sig(lib, "SUNAMD_BatchedBDFGetKernelTimes", C.c_int, [c_void_p, P(c_real)])
     _sig(lib, "SUNAMD_BatchedBDFSetLearnedPredictor", C.c_int, [c_void_p, c_void_p, C.c_int])
+    _sig(lib, "SUNAMD_BatchedBDFSetOccupancy", C.c_int, [c_void_p, P(C.c_int)])
     _sig(lib, "SUNAMD_BatchedBDFSetTrainingLog", C.c_int,
          [c_void_p, c_void_p, c_void_p, c_void_p, C.c_long, C.c_int])
     _sig(lib, "SUNAMD_BatchedBDFGetTrainingLogCount", C.c_long, [c_void_p])

@@ -57,6 +57,7 @@ struct SUNAMD_BatchedBDF_
   // learned initial iterate and training log
   const void* mlp;      // device weight block (SUNAMD_MLPDeviceWeights) or NULL
   int learned_mask;     // bit k: bucket k uses it
+  int waves[SUNAMD_BATCH_MAXB]; // register budget of bucket k's kernel (1, 2 or 4 waves/SIMD)
   float *log_x, *log_t, *log_m;
   long log_cap;
   int log_stride;
@@ -115,7 +116,7 @@ static bool launch_bucket(const SUNAMD_BatchedBDF* B, int k, double tout, int gr
   switch (d.m)
   {
 #define BDF_CASE(MM)                                                             \
-  case MM: sunamd_bdf::launch<MM, T>(A, grid, d.stream); return true;
+  case MM: sunamd_bdf::launch<MM, T>(A, grid, d.stream, B->waves[k]); return true;
     BDF_SIZES(BDF_CASE)
 #undef BDF_CASE
   default: return false;
@@ -133,6 +134,7 @@ extern "C" SUNAMD_BatchedBDF* SUNAMD_BatchedBDFCreate(const SUNAMD_Batch* batch,
   B->atol = atol;
   B->prec = lu_precision;
   B->mxstep = MXSTEP_DEFAULT;
+  for (int k = 0; k < SUNAMD_BATCH_MAXB; k++) { B->waves[k] = 1; }
   if (hipMalloc((void**)&B->dcount, sizeof(unsigned long long) * (1 + 2 * SUNAMD_BATCH_MAXB)) !=
         hipSuccess ||
       hipMemset(B->dcount, 0, sizeof(unsigned long long) * (1 + 2 * SUNAMD_BATCH_MAXB)) != hipSuccess)
@@ -365,6 +367,17 @@ extern "C" int SUNAMD_BatchedBDFGetResults(SUNAMD_BatchedBDF* B, double* y, long
     free(hs);
   }
   return rc;
+}
+
+extern "C" int SUNAMD_BatchedBDFSetOccupancy(SUNAMD_BatchedBDF* B, const int* waves)
+{
+  if (!B || !waves) { return -1; }
+  for (int k = 0; k < B->nb; k++)
+  {
+    if (waves[k] != 1 && waves[k] != 2 && waves[k] != 4) { return -1; }
+  }
+  for (int k = 0; k < B->nb; k++) { B->waves[k] = waves[k]; }
+  return 0;
 }
 
 extern "C" int SUNAMD_BatchedBDFSetLearnedPredictor(SUNAMD_BatchedBDF* B, const void* weights,

@@ -48,10 +48,11 @@ struct BucketArgs
 // LDS floats per lane group for the MLP inputs and hidden layers
 constexpr int MLP_LDS_FLOATS = 144 + 128 + 128;
 
-// launches the persistent kernel for systems of M species (T: LU precision);
-// defined in bdf_device.h, instantiated in the bdf_inst_*.hip units
+// launches the persistent kernel for systems of M species (T: LU precision)
+// compiled for at least `waves` wavefronts per SIMD (1, 2 or 4); defined in
+// bdf_device.h, instantiated in the bdf_inst_*.hip units
 template <int M, typename T>
-void launch(const BucketArgs& A, int grid, hipStream_t stream);
+void launch(const BucketArgs& A, int grid, hipStream_t stream, int waves);
 
 } // namespace sunamd_bdf
 

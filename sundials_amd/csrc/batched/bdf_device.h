@@ -1270,8 +1270,10 @@ __device__ __forceinline__ bool g_advance(const Grp<M, T>& g, Sys<M, T>& S, doub
 // One wavefront runs 64/G groups.  A group that finishes its system takes
 // the next one from the queue between steps, so the wavefront's groups stay
 // busy while their neighbours' systems need more steps.
-template <int M, typename T>
-__global__ void __launch_bounds__(64) k_batched_bdf(BucketArgs A)
+// W: minimum wavefronts per SIMD the register allocation must allow (1 = no
+// limit; 2 and 4 cap it at 256 and 128 registers, spilling the rest)
+template <int M, typename T, int W>
+__global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
 {
   constexpr int G   = pow2ceil(M);
   constexpr int GPW = 64 / G;
@@ -1314,19 +1316,22 @@ __global__ void __launch_bounds__(64) k_batched_bdf(BucketArgs A)
 
 // launcher, instantiated per (m, precision) in the bdf_inst_*.hip units
 template <int M, typename T>
-void launch(const BucketArgs& A, int grid, hipStream_t stream)
+void launch(const BucketArgs& A, int grid, hipStream_t stream, int waves)
 {
   constexpr int GPW = 64 / pow2ceil(M);
   const size_t lds  = (A.mlp_w != nullptr || A.log_x != nullptr)
                         ? sizeof(float) * (size_t)GPW * MLP_LDS_FLOATS
                         : 0;
-  k_batched_bdf<M, T><<<grid, 64, lds, stream>>>(A);
+  if (waves >= 4) { k_batched_bdf<M, T, 4><<<grid, 64, lds, stream>>>(A); }
+  else if (waves == 2) { k_batched_bdf<M, T, 2><<<grid, 64, lds, stream>>>(A); }
+  else { k_batched_bdf<M, T, 1><<<grid, 64, lds, stream>>>(A); }
 }
 
 } // namespace sunamd_bdf
 
 #define SUNAMD_BDF_INSTANTIATE(MM)                                                              \
-  template void sunamd_bdf::launch<MM, double>(const sunamd_bdf::BucketArgs&, int, hipStream_t); \
-  template void sunamd_bdf::launch<MM, float>(const sunamd_bdf::BucketArgs&, int, hipStream_t);
+  template void sunamd_bdf::launch<MM, double>(const sunamd_bdf::BucketArgs&, int, hipStream_t,  \
+                                               int);                                            \
+  template void sunamd_bdf::launch<MM, float>(const sunamd_bdf::BucketArgs&, int, hipStream_t, int);
 
 #endif
