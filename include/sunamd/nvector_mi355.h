@@ -69,6 +69,19 @@ long int N_VGetNumLaunches_Mi355(N_Vector v);
 #define SUNAMD_REDUCE_SEQUENTIAL 1
 SUNErrCode N_VSetReductionOrder_Mi355(N_Vector v, int order);
 
+/* Classical Gram-Schmidt step on MI355X vectors with SUNClassicalGS's
+   arguments, coefficients and reorthogonalisation rule
+   (include/sundials/sundials_iterative.h:231-233): the k projections and
+   ||v_k||^2 are summed into device memory, the update v_k -= sum h_i v_i
+   reads them there and sums the new norm in the same pass, and one
+   synchronisation returns the Hessenberg column -- 8(2k+3) bytes per element
+   and one host round trip instead of 8(2k+5) and two.  Vectors in
+   sequential reduction order take SUNClassicalGS itself.  Install with
+   SUNLinSol_SPGMRSetClassicalGSFn. */
+SUNErrCode SUNAMD_ClassicalGS_Mi355(N_Vector* v, sunrealtype** h, int k, int p,
+                                    sunrealtype* new_vk_norm, sunrealtype* stemp,
+                                    N_Vector* vtemp);
+
 /* 1 if v is an MI355X vector of this library, else 0 (no counterpart) */
 int SUNAMD_NVectorIsMi355(N_Vector v);
 

@@ -226,6 +226,8 @@ def hip():
     _sig(lib, "SUNAMD_ProfilerReset", None, [])
     _sig(lib, "SUNAMD_ProfilerGet", C.c_int, [C.c_int, P(C.c_double), P(C.c_long)])
     _sig(lib, "SUNAMD_BenchVectorOps", C.c_int, [c_index, C.c_int, C.c_int, P(c_real)])
+    _sig(lib, "SUNAMD_ClassicalGS_Mi355", C.c_int, [P(c_void_p), P(P(c_real)), C.c_int, C.c_int,
+                                                    P(c_real), P(c_real), P(c_void_p)])
     _sig(lib, "SUNAMD_TridiagATimes_Mi355", C.c_int, [c_void_p, c_void_p, c_void_p])
     _sig(lib, "SUNAMD_TridiagPSetup_Mi355", C.c_int, [c_void_p])
     _sig(lib, "SUNAMD_TridiagPSolve_Mi355", C.c_int, [c_void_p, c_void_p, c_void_p, c_real, C.c_int])

@@ -32,6 +32,7 @@
 #include "../core/profiler_internal.h"
 #include "sunamd/nvector_mi355.h"
 #include "sunamd/sunamd_profiler.h"
+#include "sunamd/sundials_iterative.h"
 
 #define NV_MAX_MULTI 32      // vectors per fused call handled in one kernel
 #define NV_BLOCK 256         // threads per workgroup
@@ -51,6 +52,8 @@ struct Mi355Exec
   double* d_partials;     // NV_MAX_RED_BLOCKS * (NV_MAX_MULTI + 1)
   unsigned int* d_ticket; // last-workgroup ticket (kept at zero between calls)
   double* h_result;       // pinned, mapped: NV_MAX_MULTI + 1 scalars
+  double* d_gs;           // Gram-Schmidt coefficients on the device (lazy, GS_MAX + 2)
+  double* h_gs;           // pinned copy of them (lazy)
   long int launches;
   bool own_stream;
   bool is_default;
@@ -168,6 +171,8 @@ static void exec_release(Mi355Exec* ex)
   (void)hipFree(ex->d_partials);
   (void)hipFree(ex->d_ticket);
   (void)hipHostFree(ex->h_result);
+  if (ex->d_gs) { (void)hipFree(ex->d_gs); }
+  if (ex->h_gs) { (void)hipHostFree(ex->h_gs); }
   if (ex->own_stream) { (void)hipStreamDestroy(ex->stream); }
   if (ex->is_default && ex->device >= 0 && ex->device < 64 &&
       g_default_exec[ex->device] == ex)
@@ -1444,6 +1449,151 @@ extern "C" SUNErrCode N_VEnableFusedOps_Mi355(N_Vector v, sunbooleantype tf)
 }
 
 extern "C" long int N_VGetNumLaunches_Mi355(N_Vector v) { return EXEC(v)->launches; }
+
+// ----------------------------------------------------------------------------
+// Device classical Gram-Schmidt step (SUNClassicalGS, sundials_iterative.c:116-170)
+// ----------------------------------------------------------------------------
+#define GS_MAX 128
+
+// v_k <- v_k - sum_j d_j V_j for this chunk's vectors, in order, coefficients
+// read from device memory; the last chunk also sums ||v_k||^2 (tree order)
+template <int NV>
+__global__ void __launch_bounds__(NV_BLOCK) k_gs_update(double* vk, PtrPack V,
+                                                        const double* __restrict__ d, long n,
+                                                        int last, double* partials,
+                                                        unsigned int* ticket, double* out)
+{
+  double nrm[1] = {0.0};
+  double c[NV];
+#pragma unroll
+  for (int j = 0; j < NV; j++) { c[j] = -d[j]; }
+  const long np = n >> 1;
+  GRID_LOOP(np)
+  {
+    dbl2 z = ld2(vk, i);
+#pragma unroll
+    for (int j = 0; j < NV; j++) { z = z + c[j] * ld2(V.p[j], i); }
+    st2(vk, i, z);
+    nrm[0] += z.x * z.x;
+    nrm[0] += z.y * z.y;
+  }
+  TAIL(n)
+  {
+    double z = vk[n - 1];
+#pragma unroll
+    for (int j = 0; j < NV; j++) { z = z + c[j] * V.p[j][n - 1]; }
+    vk[n - 1] = z;
+    nrm[0] += z * z;
+  }
+  if (last) { grid_finish<RSUM, 1>(nrm, partials, ticket, out, 0.0); }
+}
+
+static bool gs_buffers(Mi355Exec* ex)
+{
+  if (ex->d_gs && ex->h_gs) { return true; }
+  if (!ex->d_gs && hipMalloc((void**)&ex->d_gs, sizeof(double) * (GS_MAX + 2)) != hipSuccess)
+  {
+    return false;
+  }
+  if (!ex->h_gs && hipHostMalloc((void**)&ex->h_gs, sizeof(double) * (GS_MAX + 2),
+                                 hipHostMallocDefault) != hipSuccess)
+  {
+    return false;
+  }
+  return true;
+}
+
+// Same arguments, coefficients and reorthogonalisation rule as SUNClassicalGS;
+// the projections are computed into device memory, the update reads them
+// there and sums the new norm in the same pass, so one synchronisation
+// returns the Hessenberg column and both norms.  Vectors in sequential
+// reduction order, or more than GS_MAX basis vectors, take SUNClassicalGS.
+extern "C" SUNErrCode SUNAMD_ClassicalGS_Mi355(N_Vector* v, sunrealtype** h, int k, int p,
+                                               sunrealtype* new_vk_norm, sunrealtype* stemp,
+                                               N_Vector* vtemp)
+{
+  const int k_minus_1 = k - 1;
+  const int i0        = (k - p > 0) ? k - p : 0;
+  const int nv        = k - i0; // vectors projected out
+  N_Vector vk         = v[k];
+  Mi355Exec* ex       = EXEC(vk);
+  const long n        = (long)NVLEN(vk);
+  if (CONTENT(vk)->red_order == SUNAMD_REDUCE_SEQUENTIAL || nv < 1 || nv + 1 > GS_MAX ||
+      n == 0 || !gs_buffers(ex))
+  {
+    return SUNClassicalGS(v, h, k, p, new_vk_norm, stemp, vtemp);
+  }
+  // projections d_j = v_k . v_(i0+j), j < nv, and v_k . v_k into d_gs[0..nv]
+  for (int base = 0; base <= nv; base += 8)
+  {
+    int cnt = nv + 1 - base < 8 ? nv + 1 - base : 8;
+    PtrPack P;
+    for (int j = 0; j < cnt; j++) { P.p[j] = DDATA(v[i0 + base + j]); }
+    int g = red_grid(ex, n >> 1);
+    switch (cnt)
+    {
+#define GD_CASE(NVV)                                                                        \
+  case NVV:                                                                                 \
+    k_dotmulti<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(vk), P, n, ex->d_partials,        \
+                                                    ex->d_ticket, ex->d_gs + base);         \
+    break;
+      NV_CASES(GD_CASE)
+#undef GD_CASE
+    }
+    LAUNCHCHK(vk);
+  }
+  // v_k -= sum_j d_j v_j (the reference's linear combination pairs the
+  // coefficient of v_(i0+j) with v[j]; identical when i0 == 0), new norm^2
+  // into d_gs[nv + 1]
+  for (int base = 0; base < nv; base += 8)
+  {
+    int cnt  = nv - base < 8 ? nv - base : 8;
+    int last = (base + cnt == nv) ? 1 : 0;
+    PtrPack P;
+    for (int j = 0; j < cnt; j++) { P.p[j] = DDATA(v[base + j]); }
+    int g = red_grid(ex, n >> 1);
+    switch (cnt)
+    {
+#define GU_CASE(NVV)                                                                         \
+  case NVV:                                                                                  \
+    k_gs_update<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(vk), P, ex->d_gs + base, n, last, \
+                                                     ex->d_partials, ex->d_ticket,           \
+                                                     ex->d_gs + nv + 1);                     \
+    break;
+      NV_CASES(GU_CASE)
+#undef GU_CASE
+    }
+    LAUNCHCHK(vk);
+  }
+  HIPCHK(vk->sunctx, hipMemcpyAsync(ex->h_gs, ex->d_gs, sizeof(double) * (nv + 2),
+                                    hipMemcpyDeviceToHost, ex->stream));
+  HIPCHK(vk->sunctx, hipStreamSynchronize(ex->stream));
+  const double* d      = ex->h_gs;
+  const double vk_norm = (d[nv] <= 0.0) ? 0.0 : sqrt(d[nv]);
+  for (int i = 0; i < nv; i++) { h[i][k_minus_1] = d[i]; }
+  *new_vk_norm = (d[nv + 1] <= 0.0) ? 0.0 : sqrt(d[nv + 1]);
+
+  // reorthogonalise with the reference's second pass when the projection
+  // removed almost everything
+  if ((1000.0 * (*new_vk_norm)) < vk_norm)
+  {
+    SUNErrCode err = mi_dotprodmulti(nv, vk, v + i0, stemp + 1);
+    if (err != SUN_SUCCESS) { return err; }
+    stemp[0] = 1.0;
+    vtemp[0] = vk;
+    for (int i = i0; i < k; i++)
+    {
+      h[i][k_minus_1] += stemp[i - i0 + 1];
+      stemp[i - i0 + 1] = -stemp[i - i0 + 1];
+      vtemp[i - i0 + 1] = v[i - i0];
+    }
+    err = mi_lincomb(k + 1, stemp, vtemp, vk);
+    if (err != SUN_SUCCESS) { return err; }
+    double nn    = mi_dotprod(vk, vk);
+    *new_vk_norm = (nn <= 0.0) ? 0.0 : sqrt(nn);
+  }
+  return SUN_SUCCESS;
+}
 
 extern "C" SUNErrCode N_VSetReductionOrder_Mi355(N_Vector v, int order)
 {

@@ -70,7 +70,8 @@ def scenario_scalings(n, seed):
             (one, s[2], False), (one, s[3], True)]
 
 
-def run_scenarios(be, gstype, pretype, n=N, maxl=MAXL, tol=TOL, seed=11, max_restarts=0):
+def run_scenarios(be, gstype, pretype, n=N, maxl=MAXL, tol=TOL, seed=11, max_restarts=0,
+                  cgs_fn=None):
     """Returns one record per solve: (scenario, zeroguess, flag, iters, resnorm,
     solution, exact)."""
     core = _lib.core()
@@ -94,6 +95,8 @@ def run_scenarios(be, gstype, pretype, n=N, maxl=MAXL, tol=TOL, seed=11, max_res
         assert core.SUNLinSolInitialize(LS) == 0
         assert core.SUNLinSol_SPGMRSetGSType(LS, gstype) == 0
         assert core.SUNLinSol_SPGMRSetMaxRestarts(LS, max_restarts) == 0
+        if cgs_fn is not None:
+            assert core.SUNLinSol_SPGMRSetClassicalGSFn(LS, cgs_fn) == 0
         for si, (s1, s2, prec) in enumerate(scenario_scalings(n, seed + 1)):
             for v, vals in ((s1v, s1), (s2v, s2), (xv, xhat / s2)):
                 _overwrite(be, v, vals)

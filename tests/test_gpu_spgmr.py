@@ -57,3 +57,58 @@ def test_spgmr_device_restarts(ctx):
     for r, d in zip(ref, dev):
         assert r[:4] == d[:4]
         assert np.array_equal(r[5], d[5])
+
+
+def _device_gs():
+    import ctypes as C
+
+    return C.cast(_lib.hip().SUNAMD_ClassicalGS_Mi355, C.c_void_p)
+
+
+def test_device_gs_step_builds_orthonormal_basis(ctx):
+    """The fused device classical GS step: projections equal V^T w, the new
+    norm equals the residual's, and the basis it builds is orthonormal."""
+    import ctypes as C
+    from helpers import gpu_vec, gpu_get, vec_array
+
+    core, hip = _lib.core(), _lib.hip()
+    rng = np.random.default_rng(4)
+    n, k = 100003, 20
+    A = rng.standard_normal((n, k + 1))
+    vs = [gpu_vec(ctx, A[:, 0] / np.linalg.norm(A[:, 0]))]
+    H = np.zeros((k + 1, k))
+    hp = (C.POINTER(C.c_double) * (k + 1))(
+        *[H[i].ctypes.data_as(C.POINTER(C.c_double)) for i in range(k + 1)])
+    stemp = np.zeros(k + 2)
+    vtemp = (C.c_void_p * (k + 2))()
+    for j in range(1, k + 1):
+        vs.append(gpu_vec(ctx, A[:, j]))
+        Q = np.stack([gpu_get(v) for v in vs[:-1]], axis=1)
+        w = A[:, j]
+        nrm = C.c_double()
+        assert hip.SUNAMD_ClassicalGS_Mi355(vec_array(vs), hp, j, k, C.byref(nrm),
+                                            stemp.ctypes.data_as(C.POINTER(C.c_double)),
+                                            vtemp) == 0
+        assert np.allclose(H[:j, j - 1], Q.T @ w, rtol=1e-11, atol=1e-11)
+        r = w - Q @ (Q.T @ w)
+        assert abs(nrm.value - np.linalg.norm(r)) <= 1e-11 * np.linalg.norm(w)
+        core.N_VScale(1.0 / nrm.value, vs[-1], vs[-1])
+    V = np.stack([gpu_get(v) for v in vs], axis=1)
+    assert np.abs(V.T @ V - np.eye(k + 1)).max() < 1e-12
+    for v in vs:
+        core.N_VDestroy(v)
+
+
+def test_spgmr_with_device_gs_step(ctx):
+    # tree order: the reference test's criterion; sequential order: the step
+    # defers to SUNClassicalGS, so the run equals the serial one bit for bit
+    for (si, zg, flag, iters, res, y, x) in run_scenarios(
+            DeviceBackend(ctx, REDUCE_TREE), CLASSICAL_GS, PREC_LEFT, cgs_fn=_device_gs()):
+        assert flag == 0, (si, zg, flag)
+        assert solution_ok(x, y, 10 * TOL), (si, zg)
+    ref = run_scenarios(SerialBackend(ctx), CLASSICAL_GS, PREC_RIGHT)
+    dev = run_scenarios(DeviceBackend(ctx, REDUCE_SEQUENTIAL), CLASSICAL_GS, PREC_RIGHT,
+                        cgs_fn=_device_gs())
+    for r, d in zip(ref, dev):
+        assert r[:4] == d[:4]
+        assert np.array_equal(r[5], d[5])
