@@ -8,10 +8,11 @@ the oracle's, and the final state is identical bit for bit (stronger than
 the 1e-10 relative bar).
 FP32-LU: the Newton matrix is factored in single precision while residuals,
 updates and norms stay FP64; the modified Newton iteration absorbs the
-factor's rounding, so every system still succeeds and its final state stays
-within the integration tolerance of the FP64 oracle (weighted RMS of the
-difference, weights 1/(rtol |y| + atol), below 1 -- the difference between
-two runs at rtol 1e-6 is of the order of the local error control itself).
+factor's rounding, so every system still succeeds, and its final-state error
+against a reference solution at 10^4x tighter tolerances is of the size of
+the FP64 run's own error (both are integration errors of rtol 1e-6; a
+changed step sequence moves the state by that much, not by the rounding of
+the factors).
 """
 import numpy as np
 import pytest
@@ -36,6 +37,15 @@ def batch_and_oracle():
     free_batch(b)
 
 
+def _wrms_per_system(b, y, y_ref):
+    out = []
+    for (k, c, m, off) in system_slices(b):
+        d = y[off:off + m] - y_ref[off:off + m]
+        w = 1.0 / (RTOL * np.abs(y_ref[off:off + m]) + ATOL)
+        out.append(np.sqrt(np.mean((d * w) ** 2)))
+    return np.array(out)
+
+
 def test_batched_fp64_bitwise_vs_oracle(batch_and_oracle):
     b, y_ref, c_ref = batch_and_oracle
     y, cnt, rc, wall = run_device(b, LU_FP64)
@@ -51,14 +61,22 @@ def test_batched_fp64_bitwise_vs_oracle(batch_and_oracle):
 
 
 def test_batched_fp32_lu_accuracy(batch_and_oracle):
-    b, y_ref, c_ref = batch_and_oracle
+    b, y64, c_ref = batch_and_oracle
     y, cnt, rc, wall = run_device(b, LU_FP32)
     assert rc == 0
     assert np.all(cnt[:, 9] == 0)
-    for (k, c, m, off) in system_slices(b):
-        d = y[off:off + m] - y_ref[off:off + m]
-        w = 1.0 / (RTOL * np.abs(y_ref[off:off + m]) + ATOL)
-        assert np.sqrt(np.mean((d * w) ** 2)) < 1.0, (k, c, m)
+    y_tight, _, rc, _ = run_oracle(b, nthreads=16, rtol=1e-10, atol=1e-14)
+    assert rc == 0
+    e32 = _wrms_per_system(b, y, y_tight)
+    e64 = _wrms_per_system(b, y64, y_tight)
+    print("error vs tight reference (rtol-weighted RMS): fp64 median %.3g max %.3g, "
+          "fp32-LU median %.3g max %.3g" % (np.median(e64), e64.max(), np.median(e32),
+                                             e32.max()))
+    # per system the two errors differ as two step sequences do; over the
+    # batch their distributions agree
+    assert np.median(e32) <= 1.5 * np.median(e64) + 0.05
+    assert np.percentile(e32, 95) <= 1.5 * np.percentile(e64, 95) + 0.1
+    assert e32.max() <= 3.0 * e64.max()
     # Newton iterations per step stay close to the FP64 run
     nni32 = cnt[:, 4].sum() / cnt[:, 0].sum()
     nni64 = c_ref[:, 4].sum() / c_ref[:, 0].sum()
