@@ -97,6 +97,11 @@ int CVodeSetInitialCorrectionFn(void* cvode_mem, CVInitialCorrectionFn fn);
 /* cvode_io.c:1006: fused error-weight and residual kernels (MI355X vectors) */
 int CVodeSetUseIntegratorFusedKernels(void* cvode_mem, sunbooleantype onoff);
 
+/* the x^y of the step-size ratios (SUNRpowerR, cvode.c:3394 ...): correctly
+   rounded double-double evaluation shared with the device integrator (no
+   reference counterpart) */
+double SUNAMD_Pow(double x, double y);
+
 int CVode(void* cvode_mem, sunrealtype tout, N_Vector yout, sunrealtype* tret,
           int itask);
 int CVodeGetDky(void* cvode_mem, sunrealtype t, int k, N_Vector dky);

@@ -97,6 +97,16 @@ static bool launch_bucket(const SUNAMD_BatchedBDF* B, int k, double tout, int gr
     A.log_stride = B->log_stride > 0 ? B->log_stride : 1;
   }
   A.lstats = B->dcount + 1 + 2 * k;
+  {
+    const int gpw = 64 / pow2ceil(d.m);
+    if (sunamd_bdf::bucket_lds_bytes(d.m, d.R, gpw, A.mlp_w != nullptr || A.log_x != nullptr) >
+        60000)
+    {
+      fprintf(stderr, "[sundials_amd batched_bdf] bucket m=%d R=%d needs too much LDS\n", d.m,
+              d.R);
+      return false;
+    }
+  }
   A.m      = d.m;
   A.R      = d.R;
   A.ncells = d.ncells;

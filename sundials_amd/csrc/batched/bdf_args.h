@@ -48,6 +48,16 @@ struct BucketArgs
 // LDS floats per lane group for the MLP inputs and hidden layers
 constexpr int MLP_LDS_FLOATS = 144 + 128 + 128;
 
+// dynamic LDS of one workgroup (GPW lane groups): staged rate constants,
+// network tables, and the MLP area when the learned start or the log is on
+inline size_t bucket_lds_bytes(int m, int R, int gpw, bool mlp)
+{
+  size_t b = sizeof(double) * (size_t)gpw * 2 * R;
+  b += sizeof(int) * (size_t)(8 * R + ((m + 2) & ~1));
+  if (mlp) { b += sizeof(float) * (size_t)gpw * MLP_LDS_FLOATS; }
+  return b;
+}
+
 // launches the persistent kernel for systems of M species (T: LU precision)
 // compiled for at least `waves` wavefronts per SIMD (1, 2 or 4); defined in
 // bdf_device.h, instantiated in the bdf_inst_*.hip units

@@ -156,6 +156,11 @@ struct Grp
   GroupScalars* sc;
   float* mlp; // MLP_LDS_FLOATS of LDS (xs[144] h1[128] h2[128]) or nullptr
   const BucketArgs* a;
+  // the bucket's network, staged in LDS once per workgroup
+  const int* idx;  // 4R reaction indices
+  const int* soff; // M+1 term-list offsets
+  const int* sent; // 4R term codes
+  double* rates;   // this group's 2R rate constants (k_f then k_b), staged per system
   long cell;
   const double* kf;
   const double* kb;
@@ -203,13 +208,13 @@ __device__ __forceinline__ double g_rhs(const Grp<M, T>& g, double yi)
   // this species' terms, in reaction order and a, b, c, d role order within a
   // reaction (the per-species list the host built from the network)
   double f       = 0.0;
-  const int* idx = g.a->idx;
+  const int* idx = g.idx;
   if (g.valid)
   {
-    const int e1 = g.a->soff[g.i + 1];
-    for (int e = g.a->soff[g.i]; e < e1; e++)
+    const int e1 = g.soff[g.i + 1];
+    for (int e = g.soff[g.i]; e < e1; e++)
     {
-      const int code = g.a->sent[e];
+      const int code = g.sent[e];
       const int r = code >> 2, t = code & 3;
       const int ia = idx[4 * r], ib = idx[4 * r + 1], ic = idx[4 * r + 2], id = idx[4 * r + 3];
       double wf = (g.kf[r] * g.red[ia]) * g.red[ib];
@@ -233,13 +238,13 @@ __device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, d
   gsync();
 #pragma unroll
   for (int j = 0; j < M; j++) { a[j] = 0.0; }
-  const int* idx = g.a->idx;
+  const int* idx = g.idx;
   if (g.valid)
   {
-    const int e1 = g.a->soff[g.i + 1];
-    for (int e = g.a->soff[g.i]; e < e1; e++)
+    const int e1 = g.soff[g.i + 1];
+    for (int e = g.soff[g.i]; e < e1; e++)
     {
-      const int code = g.a->sent[e];
+      const int code = g.sent[e];
       const int r = code >> 2, t = code & 3;
       int ix[4]    = {idx[4 * r], idx[4 * r + 1], idx[4 * r + 2], idx[4 * r + 3]};
       double kfv   = g.kf[r], kbv = g.kb[r];
@@ -1157,8 +1162,21 @@ __device__ __forceinline__ bool g_acquire(Grp<M, T>& g, Sys<M, T>& S, double tou
     ticket = __shfl(ticket, g.base, 64);
     if (ticket >= (unsigned long long)A.ncells) { return false; }
     g.cell = (long)ticket;
-    g.kf   = A.kf + g.cell * A.R;
-    g.kb   = A.kb + g.cell * A.R;
+    {
+      // this system's rate constants into the group's LDS slot
+      constexpr int G = pow2ceil(M);
+      gsync();
+      const double* kf = A.kf + g.cell * A.R;
+      const double* kb = A.kb + g.cell * A.R;
+      for (int r = g.i; r < A.R; r += G)
+      {
+        g.rates[r]       = kf[r];
+        g.rates[A.R + r] = kb[r];
+      }
+      gsync();
+      g.kf = g.rates;
+      g.kb = g.rates + A.R;
+    }
 
     S.zn[0] = g.valid ? A.y0[g.cell * M + g.i] : 0.0;
 #pragma unroll
@@ -1192,9 +1210,9 @@ __device__ __forceinline__ bool g_acquire(Grp<M, T>& g, Sys<M, T>& S, double tou
       if (g.valid)
       {
         int last = -1;
-        for (int e = A.soff[g.i]; e < A.soff[g.i + 1]; e++)
+        for (int e = g.soff[g.i]; e < g.soff[g.i + 1]; e++)
         {
-          const int r = A.sent[e] >> 2;
+          const int r = g.sent[e] >> 2;
           if (r != last) { ks += g.kf[r] + g.kb[r]; }
           last = r;
         }
@@ -1281,9 +1299,23 @@ __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
   __shared__ T s_prow[64];
   __shared__ int s_inv[64];
   __shared__ GroupScalars s_sc[GPW];
-  extern __shared__ float s_mlp[]; // GPW * MLP_LDS_FLOATS when learned or logging
+  // dynamic LDS: per-group rate constants (GPW x 2R doubles), the network
+  // tables (4R + 4R + M+1 ints), then GPW x MLP_LDS_FLOATS when learned or logging
+  extern __shared__ double s_dyn[];
+  double* s_rates = s_dyn;
+  int* s_idx      = (int*)(s_dyn + (long)GPW * 2 * A.R);
+  int* s_sent     = s_idx + 4 * A.R;
+  int* s_soff     = s_sent + 4 * A.R;
+  float* s_mlp    = (float*)(s_soff + ((M + 2) & ~1));
 
   const int lane = threadIdx.x;
+  for (int t = lane; t < 4 * A.R; t += 64)
+  {
+    s_idx[t]  = A.idx[t];
+    s_sent[t] = A.sent[t];
+  }
+  for (int t = lane; t <= M; t += 64) { s_soff[t] = A.soff[t]; }
+  __syncthreads();
   const int grp  = lane / G;
   Grp<M, T> g;
   g.i     = lane % G;
@@ -1295,6 +1327,10 @@ __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
   g.sc    = &s_sc[grp];
   g.mlp   = (A.mlp_w != nullptr || A.log_x != nullptr) ? s_mlp + grp * MLP_LDS_FLOATS : nullptr;
   g.a     = &A;
+  g.idx   = s_idx;
+  g.soff  = s_soff;
+  g.sent  = s_sent;
+  g.rates = s_rates + (long)grp * 2 * A.R;
   g.cell  = 0;
   g.kf    = A.kf;
   g.kb    = A.kb;
@@ -1319,9 +1355,7 @@ template <int M, typename T>
 void launch(const BucketArgs& A, int grid, hipStream_t stream, int waves)
 {
   constexpr int GPW = 64 / pow2ceil(M);
-  const size_t lds  = (A.mlp_w != nullptr || A.log_x != nullptr)
-                        ? sizeof(float) * (size_t)GPW * MLP_LDS_FLOATS
-                        : 0;
+  const size_t lds  = bucket_lds_bytes(M, A.R, GPW, A.mlp_w != nullptr || A.log_x != nullptr);
   if (waves >= 4) { k_batched_bdf<M, T, 4><<<grid, 64, lds, stream>>>(A); }
   else if (waves == 2) { k_batched_bdf<M, T, 2><<<grid, 64, lds, stream>>>(A); }
   else { k_batched_bdf<M, T, 1><<<grid, 64, lds, stream>>>(A); }

@@ -1747,3 +1747,7 @@ void CVodeFree(void** cvode_mem)
   free(cv);
   *cvode_mem = NULL;
 }
+
+/* the x^y used by the step-size controller, exported so its rounding can be
+   checked (no reference counterpart) */
+double SUNAMD_Pow(double x, double y) { return sunamd_pow(x, y); }
