@@ -154,14 +154,23 @@ __global__ void k_dsilu(const float* __restrict__ dA, const float* __restrict__ 
   }
 }
 
-// db[j] = sum_b dZ[b][j]; one thread per column
-__global__ void k_colsum(const float* __restrict__ dZ, int B, int N, float* __restrict__ db)
+// db[j] = sum_b dZ[b][j]: one workgroup per column, 256 rows at a time,
+// fixed-order tree over the threads (deterministic)
+__global__ void __launch_bounds__(256) k_colsum(const float* __restrict__ dZ, int B, int N,
+                                                float* __restrict__ db)
 {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) { return; }
-  float s = 0.0f;
-  for (int b = 0; b < B; b++) { s += dZ[(long)b * N + j]; }
-  db[j] = s;
+  __shared__ float part[256];
+  const int j = blockIdx.x;
+  float s     = 0.0f;
+  for (int b = threadIdx.x; b < B; b += 256) { s += dZ[(long)b * N + j]; }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1)
+  {
+    if ((int)threadIdx.x < w) { part[threadIdx.x] += part[threadIdx.x + w]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { db[j] = part[0]; }
 }
 
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -383,17 +392,17 @@ static void backward_dev(SUNAMD_MLP* M, const float* x, const float* t, const fl
                                                    M->acc);
   // layer 3: dW3 = dY^T A2, db3 = colsum dY, dA2 = dY W3
   gemm(s, OUT, HID, B, M->dyb, 1, OUT, M->a2b, HID, 1, M->g + OW3, HID);
-  k_colsum<<<1, 64, 0, s>>>(M->dy, B, OUT, M->g + OB3);
+  k_colsum<<<OUT, 256, 0, s>>>(M->dy, B, OUT, M->g + OB3);
   gemm(s, B, HID, OUT, M->dyb, OUT, 1, M->w3, HID, 1, M->da2, HID);
   k_dsilu<<<blocks_for((long)B * HID), 256, 0, s>>>(M->da2, M->z2, (long)B * HID, M->dz2, M->dz2b);
   // layer 2
   gemm(s, HID, HID, B, M->dz2b, 1, HID, M->a1b, HID, 1, M->g + OW2, HID);
-  k_colsum<<<1, 128, 0, s>>>(M->dz2, B, HID, M->g + OB2);
+  k_colsum<<<HID, 256, 0, s>>>(M->dz2, B, HID, M->g + OB2);
   gemm(s, B, HID, HID, M->dz2b, HID, 1, M->w2, HID, 1, M->da1, HID);
   k_dsilu<<<blocks_for((long)B * HID), 256, 0, s>>>(M->da1, M->z1, (long)B * HID, M->dz1, M->dz1b);
   // layer 1
   gemm(s, HID, IN, B, M->dz1b, 1, HID, M->xb, IN, 1, M->g + OW1, IN);
-  k_colsum<<<1, 128, 0, s>>>(M->dz1, B, HID, M->g + OB1);
+  k_colsum<<<HID, 256, 0, s>>>(M->dz1, B, HID, M->g + OB1);
 }
 
 extern "C" int SUNAMD_MLPForward(SUNAMD_MLP* M, const float* x, int B, float* y)
