@@ -18,7 +18,8 @@ import pytest
 
 from sundials_amd import _lib
 from mlp_ref import NPARAM, IN, IN_PAD, OUT, forward, loss_grad, unpack
-from batch_util import RTOL, ATOL, make_batch, free_batch, run_device, system_slices, MXSTEP
+from batch_util import RTOL, ATOL, make_batch, free_batch, run_device, run_oracle, \
+    system_slices, MXSTEP
 
 pytestmark = pytest.mark.gpu
 PF = C.POINTER(C.c_float)
@@ -179,10 +180,24 @@ def test_learned_predictor_end_to_end(hip):
         finally:
             hip.SUNAMD_BatchedBDFFree(B)
         assert np.all(cl[:, 9] == 0)
-        for (k, c, mm, off) in system_slices(test):
-            d = yl[off:off + mm] - y64[off:off + mm]
-            w = 1.0 / (RTOL * np.abs(y64[off:off + mm]) + ATOL)
-            assert np.sqrt(np.mean((d * w) ** 2)) < 1.0
+        # accuracy: against a reference at 10^4x tighter tolerances the learned
+        # run's errors are distributed like the plain run's
+        y_tight, _, rc, _ = run_oracle(test, nthreads=16, rtol=1e-10, atol=1e-14)
+        assert rc == 0
+
+        def errs(y):
+            out = []
+            for (k, c, mm, off) in system_slices(test):
+                d = y[off:off + mm] - y_tight[off:off + mm]
+                w = 1.0 / (RTOL * np.abs(y_tight[off:off + mm]) + ATOL)
+                out.append(np.sqrt(np.mean((d * w) ** 2)))
+            return np.array(out)
+
+        el, ep = errs(yl), errs(y64)
+        print("error vs tight reference: plain median %.3g p95 %.3g, learned median %.3g p95 %.3g"
+              % (np.median(ep), np.percentile(ep, 95), np.median(el), np.percentile(el, 95)))
+        assert np.median(el) <= 1.5 * np.median(ep) + 0.05
+        assert np.percentile(el, 95) <= 1.5 * np.percentile(ep, 95) + 0.1
         nni_l = cl[:, 4].sum() / cl[:, 0].sum()
         nni_p = c64[:, 4].sum() / c64[:, 0].sum()
         print("held-out: Newton iters/step plain %.4f learned %.4f; guess kept %d of %d steps; "
