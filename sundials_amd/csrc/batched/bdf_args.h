@@ -49,11 +49,13 @@ struct BucketArgs
 constexpr int MLP_LDS_FLOATS = 144 + 128 + 128;
 
 // dynamic LDS of one workgroup (GPW lane groups): staged rate constants,
-// network tables, and the MLP area when the learned start or the log is on
+// network tables, one Jacobian row per lane, and the MLP area when the
+// learned start or the log is on
 inline size_t bucket_lds_bytes(int m, int R, int gpw, bool mlp)
 {
   size_t b = sizeof(double) * (size_t)gpw * 2 * R;
   b += sizeof(int) * (size_t)(8 * R + ((m + 2) & ~1));
+  b += sizeof(double) * (size_t)gpw * m * (m + 1); // Jacobian rows, padded stride m+1
   if (mlp) { b += sizeof(float) * (size_t)gpw * MLP_LDS_FLOATS; }
   return b;
 }

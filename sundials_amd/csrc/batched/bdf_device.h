@@ -161,6 +161,7 @@ struct Grp
   const int* soff; // M+1 term-list offsets
   const int* sent; // 4R term codes
   double* rates;   // this group's 2R rate constants (k_f then k_b), staged per system
+  double* jrow;    // this lane's Jacobian-row accumulator (M doubles, LDS)
   long cell;
   const double* kf;
   const double* kb;
@@ -236,11 +237,14 @@ __device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, d
   gsync();
   if (g.valid) { g.red[g.i] = yj; }
   gsync();
-#pragma unroll
-  for (int j = 0; j < M; j++) { a[j] = 0.0; }
+  // the row is accumulated in LDS so that a term's column is a plain address
+  // (the same additions in the same order as a register row)
+  double* row    = g.jrow;
   const int* idx = g.idx;
   if (g.valid)
   {
+#pragma unroll
+    for (int j = 0; j < M; j++) { row[j] = 0.0; }
     const int e1 = g.soff[g.i + 1];
     for (int e = g.soff[g.i]; e < e1; e++)
     {
@@ -253,13 +257,8 @@ __device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, d
 #pragma unroll
       for (int q = 0; q < 4; q++)
       {
-        const int col  = ix[q];
         const double v = dw[q];
-#pragma unroll
-        for (int j = 0; j < M; j++)
-        {
-          if (j == col) { a[j] = (t < 2) ? (a[j] - v) : (a[j] + v); }
-        }
+        row[ix[q]]     = (t < 2) ? (row[ix[q]] - v) : (row[ix[q]] + v);
       }
     }
   }
@@ -268,8 +267,44 @@ __device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, d
 #pragma unroll
   for (int j = 0; j < M; j++)
   {
-    a[j] = a[j] * c;
+    a[j] = g.valid ? row[j] * c : 0.0;
     if (j == g.i) { a[j] = a[j] + 1.0; }
+  }
+}
+
+// lane l's value of x, for a lane index that is the same in the whole wavefront
+__device__ __forceinline__ double readlane_d(double x, int l)
+{
+  const long long v = __double_as_longlong(x);
+  const int lo      = __builtin_amdgcn_readlane((int)(v & 0xffffffffLL), l);
+  const int hi      = __builtin_amdgcn_readlane((int)(v >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+__device__ __forceinline__ float readlane_d(float x, int l)
+{
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// value of b in the lane that holds physical row k of this lane's group, for
+// groups of 64 or 32 lanes; invr is the lane's entry of the group's inverse
+// row map (lane base+j holds inv[j]); indexes are masked to the group, which
+// only matters for a group without a system (stale map, result unused)
+template <int G>
+__device__ __forceinline__ double g_row_value(double b, int invr, int k)
+{
+  if constexpr (G == 64)
+  {
+    const int lk = __builtin_amdgcn_readlane(invr, k) & 63;
+    return readlane_d(b, lk);
+  }
+  else
+  {
+    const int l0    = __builtin_amdgcn_readlane(invr, k) & 31;
+    const int l1    = __builtin_amdgcn_readlane(invr, 32 + k) & 31;
+    const double b0 = readlane_d(b, l0);
+    const double b1 = readlane_d(b, 32 + l1);
+    return (__lane_id() < 32) ? b0 : b1;
   }
 }
 
@@ -299,23 +334,48 @@ __device__ __forceinline__ int g_getrf(const Grp<M, T>& g, Sys<M, T>& S)
     if (v == (T)0 && fail == 0) { fail = k + 1; }
     if (pos == k) { pos = bp; }
     else if (pos == bp) { pos = k; }
-    gsync();
-    if (g.valid && pos == k)
+    if constexpr (G == 64)
     {
-#pragma unroll
-      for (int j = k; j < M; j++) { g.prow[j] = S.lu[j]; }
-    }
-    gsync();
-    if (g.valid && pos > k && fail == 0)
-    {
-      const T mult = (T)1 / g.prow[k];
-      const T l    = S.lu[k] * mult;
-      S.lu[k]      = l;
+      // one system per wavefront: the pivot row's lane is wavefront-uniform,
+      // its entries are read straight from that lane's registers
+      const unsigned long long holder = __ballot(g.valid && pos == k);
+      const int pl  = __builtin_amdgcn_readfirstlane(__ffsll((long long)holder) - 1) & 63;
+      const bool up = g.valid && pos > k && fail == 0;
+      const T pk    = readlane_d(S.lu[k], pl);
+      T l           = (T)0;
+      if (up)
+      {
+        const T mult = (T)1 / pk;
+        l            = S.lu[k] * mult;
+        S.lu[k]      = l;
+      }
 #pragma unroll
       for (int j = k + 1; j < M; j++)
       {
-        const T pj = g.prow[j];
-        if (pj != (T)0) { S.lu[j] = S.lu[j] - pj * l; }
+        const T pj = readlane_d(S.lu[j], pl);
+        if (up && pj != (T)0) { S.lu[j] = S.lu[j] - pj * l; }
+      }
+    }
+    else
+    {
+      gsync();
+      if (g.valid && pos == k)
+      {
+#pragma unroll
+        for (int j = k; j < M; j++) { g.prow[j] = S.lu[j]; }
+      }
+      gsync();
+      if (g.valid && pos > k && fail == 0)
+      {
+        const T mult = (T)1 / g.prow[k];
+        const T l    = S.lu[k] * mult;
+        S.lu[k]      = l;
+#pragma unroll
+        for (int j = k + 1; j < M; j++)
+        {
+          const T pj = g.prow[j];
+          if (pj != (T)0) { S.lu[j] = S.lu[j] - pj * l; }
+        }
       }
     }
   }
@@ -333,18 +393,42 @@ __device__ __forceinline__ double g_getrs(const Grp<M, T>& g, const Sys<M, T>& S
   // the row swaps carried logical row r to physical row pos_r, and b's
   // element r to the same position: lane r already holds b'[pos_r]
   const int pos = S.pos;
-#pragma unroll
-  for (int k = 0; k < M - 1; k++)
+  constexpr int G = pow2ceil(M);
+  if constexpr (G >= 32)
   {
-    double bk = __shfl(b, g.base + g.inv[k], 64);
-    if (pos > k) { b = b - (double)S.lu[k] * bk; }
+    // one or two systems per wavefront: lane base+j keeps the lane of its
+    // group's physical row j, and rows are fetched by readlane (a uniform lane
+    // index) instead of a data-dependent permute through LDS
+    const int invr = g.inv[g.valid ? g.i : 0];
+#pragma unroll
+    for (int k = 0; k < M - 1; k++)
+    {
+      const double bk = g_row_value<G>(b, invr, k);
+      if (pos > k) { b = b - (double)S.lu[k] * bk; }
+    }
+#pragma unroll
+    for (int k = M - 1; k > 0; k--)
+    {
+      if (pos == k) { b = b / (double)S.lu[k]; }
+      const double bk = g_row_value<G>(b, invr, k);
+      if (pos < k) { b = b - (double)S.lu[k] * bk; }
+    }
   }
-#pragma unroll
-  for (int k = M - 1; k > 0; k--)
+  else
   {
-    if (pos == k) { b = b / (double)S.lu[k]; }
-    double bk = __shfl(b, g.base + g.inv[k], 64);
-    if (pos < k) { b = b - (double)S.lu[k] * bk; }
+#pragma unroll
+    for (int k = 0; k < M - 1; k++)
+    {
+      double bk = __shfl(b, g.base + g.inv[k], 64);
+      if (pos > k) { b = b - (double)S.lu[k] * bk; }
+    }
+#pragma unroll
+    for (int k = M - 1; k > 0; k--)
+    {
+      if (pos == k) { b = b / (double)S.lu[k]; }
+      double bk = __shfl(b, g.base + g.inv[k], 64);
+      if (pos < k) { b = b - (double)S.lu[k] * bk; }
+    }
   }
   if (pos == 0) { b = b / (double)S.lu[0]; }
   // variable i sits in the lane holding physical row i
@@ -1306,7 +1390,8 @@ __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
   int* s_idx      = (int*)(s_dyn + (long)GPW * 2 * A.R);
   int* s_sent     = s_idx + 4 * A.R;
   int* s_soff     = s_sent + 4 * A.R;
-  float* s_mlp    = (float*)(s_soff + ((M + 2) & ~1));
+  double* s_jrow  = (double*)(s_soff + ((M + 2) & ~1));
+  float* s_mlp    = (float*)(s_jrow + (long)GPW * M * (M + 1));
 
   const int lane = threadIdx.x;
   for (int t = lane; t < 4 * A.R; t += 64)
@@ -1331,6 +1416,7 @@ __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
   g.soff  = s_soff;
   g.sent  = s_sent;
   g.rates = s_rates + (long)grp * 2 * A.R;
+  g.jrow  = s_jrow + (long)grp * M * (M + 1) + (long)(g.valid ? g.i : 0) * (M + 1);
   g.cell  = 0;
   g.kf    = A.kf;
   g.kb    = A.kb;
