@@ -98,9 +98,16 @@ int CVodeSetInitialCorrectionFn(void* cvode_mem, CVInitialCorrectionFn fn);
 int CVodeSetUseIntegratorFusedKernels(void* cvode_mem, sunbooleantype onoff);
 
 /* the x^y of the step-size ratios (SUNRpowerR, cvode.c:3394 ...): correctly
-   rounded double-double evaluation shared with the device integrator (no
-   reference counterpart) */
+   rounded evaluation shared with the device integrator (no reference
+   counterpart) */
 double SUNAMD_Pow(double x, double y);
+/* diagnostics of its two paths (no reference counterpart): the fast path's
+   approximation (hi + lo) 2^scale with its error bound bound 2^scale, and
+   whether it decided the rounding (return 1, *res the result); the
+   double-double path alone */
+int SUNAMD_PowFastParts(double x, double y, double* res, double* hi, double* lo, double* bound,
+                        int* scale);
+double SUNAMD_PowExact(double x, double y);
 
 int CVode(void* cvode_mem, sunrealtype tout, N_Vector yout, sunrealtype* tret,
           int itask);

@@ -172,6 +172,8 @@ def core():
     _sig(lib, "SUNQRsol", C.c_int, [C.c_int, P(P(c_real)), P(c_real), P(c_real)])
     _sig(lib, "SUNRCompareTol", C.c_int, [c_real, c_real, c_real])
     _sig(lib, "SUNAMD_Pow", c_real, [c_real, c_real])
+    _sig(lib, "SUNAMD_PowExact", c_real, [c_real, c_real])
+    _sig(lib, "SUNAMD_PowFastParts", C.c_int, [c_real, c_real] + [P(c_real)] * 4 + [P(C.c_int)])
     _sig(lib, "SUNAMD_BatchCreate", P(Batch),
          [C.c_int64, C.c_uint64, C.c_int, P(C.c_int), P(c_real), c_real, c_real])
     _sig(lib, "SUNAMD_BatchFree", None, [P(Batch)])

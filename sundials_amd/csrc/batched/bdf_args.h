@@ -48,14 +48,21 @@ struct BucketArgs
 // LDS floats per lane group for the MLP inputs and hidden layers
 constexpr int MLP_LDS_FLOATS = 144 + 128 + 128;
 
+// doubles of a lane group's LDS rows area: the m Jacobian rows (stride m+1)
+// while the Newton matrix is built, the R reaction rates during an RHS
+__host__ __device__ constexpr long group_rows_stride(int m, int R)
+{
+  return ((long)m * (m + 1) > (long)R) ? (long)m * (m + 1) : (long)R;
+}
+
 // dynamic LDS of one workgroup (GPW lane groups): staged rate constants,
-// network tables, one Jacobian row per lane, and the MLP area when the
-// learned start or the log is on
+// network tables, the groups' rows areas, and the MLP area when the learned
+// start or the log is on
 inline size_t bucket_lds_bytes(int m, int R, int gpw, bool mlp)
 {
   size_t b = sizeof(double) * (size_t)gpw * 2 * R;
   b += sizeof(int) * (size_t)(8 * R + ((m + 2) & ~1));
-  b += sizeof(double) * (size_t)gpw * m * (m + 1); // Jacobian rows, padded stride m+1
+  b += sizeof(double) * (size_t)gpw * group_rows_stride(m, R);
   if (mlp) { b += sizeof(float) * (size_t)gpw * MLP_LDS_FLOATS; }
   return b;
 }

@@ -162,8 +162,9 @@ struct Grp
   const int* sent; // 4R term codes
   double* rates;   // this group's 2R rate constants (k_f then k_b), staged per system
   double* jrow;    // this lane's Jacobian-row accumulator (M doubles, LDS)
+  double* wbuf;    // the group's reaction rates during an RHS (aliases the rows)
   long cell;
-  const double* kf;
+  const double* kf; // this system's k_f, k_b (the LDS slot above)
   const double* kb;
 };
 
@@ -200,28 +201,35 @@ __device__ __forceinline__ double g_maxabs(const Grp<M, T>& g, double x)
 }
 
 // ---- mass-action RHS: f_i accumulated in reaction order ------------------
+// Each reaction's net rate w_r = (k_f y_a) y_b - (k_b y_c) y_d is evaluated
+// once (reactions spread over the group's lanes) into LDS; species i then
+// adds its terms -w (a, b roles) and +w (c, d roles) in reaction and role
+// order, the serial RHS's order of updates of f_i.
 template <int M, typename T>
 __device__ __forceinline__ double g_rhs(const Grp<M, T>& g, double yi)
 {
+  constexpr int G = pow2ceil(M);
   gsync();
   if (g.valid) { g.red[g.i] = yi; }
   gsync();
-  // this species' terms, in reaction order and a, b, c, d role order within a
-  // reaction (the per-species list the host built from the network)
-  double f       = 0.0;
-  const int* idx = g.idx;
+  const int R = g.a->R;
+  for (int r = g.i; r < R; r += G)
+  {
+    const int4 q   = *reinterpret_cast<const int4*>(g.idx + 4 * r);
+    const double wf = (g.kf[r] * g.red[q.x]) * g.red[q.y];
+    const double wb = (g.kb[r] * g.red[q.z]) * g.red[q.w];
+    g.wbuf[r]       = wf - wb;
+  }
+  gsync();
+  double f = 0.0;
   if (g.valid)
   {
     const int e1 = g.soff[g.i + 1];
     for (int e = g.soff[g.i]; e < e1; e++)
     {
       const int code = g.sent[e];
-      const int r = code >> 2, t = code & 3;
-      const int ia = idx[4 * r], ib = idx[4 * r + 1], ic = idx[4 * r + 2], id = idx[4 * r + 3];
-      double wf = (g.kf[r] * g.red[ia]) * g.red[ib];
-      double wb = (g.kb[r] * g.red[ic]) * g.red[id];
-      double w  = wf - wb;
-      f         = (t < 2) ? f - w : f + w;
+      const double w = g.wbuf[code >> 2];
+      f              = ((code & 3) < 2) ? f - w : f + w;
     }
   }
   gsync();
@@ -1258,8 +1266,6 @@ __device__ __forceinline__ bool g_acquire(Grp<M, T>& g, Sys<M, T>& S, double tou
         g.rates[A.R + r] = kb[r];
       }
       gsync();
-      g.kf = g.rates;
-      g.kb = g.rates + A.R;
     }
 
     S.zn[0] = g.valid ? A.y0[g.cell * M + g.i] : 0.0;
@@ -1384,14 +1390,17 @@ __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
   __shared__ int s_inv[64];
   __shared__ GroupScalars s_sc[GPW];
   // dynamic LDS: per-group rate constants (GPW x 2R doubles), the network
-  // tables (4R + 4R + M+1 ints), then GPW x MLP_LDS_FLOATS when learned or logging
-  extern __shared__ double s_dyn[];
+  // tables (4R + 4R + M+1 ints), per-group Jacobian rows / reaction rates
+  // (GPW x rows_stride doubles), then GPW x MLP_LDS_FLOATS when learned or
+  // logging; 16-byte alignment for the int4 reads of the reaction table
+  extern __shared__ __align__(16) double s_dyn[];
+  const long JS   = group_rows_stride(M, A.R);
   double* s_rates = s_dyn;
   int* s_idx      = (int*)(s_dyn + (long)GPW * 2 * A.R);
   int* s_sent     = s_idx + 4 * A.R;
   int* s_soff     = s_sent + 4 * A.R;
   double* s_jrow  = (double*)(s_soff + ((M + 2) & ~1));
-  float* s_mlp    = (float*)(s_jrow + (long)GPW * M * (M + 1));
+  float* s_mlp    = (float*)(s_jrow + (long)GPW * JS);
 
   const int lane = threadIdx.x;
   for (int t = lane; t < 4 * A.R; t += 64)
@@ -1416,10 +1425,13 @@ __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
   g.soff  = s_soff;
   g.sent  = s_sent;
   g.rates = s_rates + (long)grp * 2 * A.R;
-  g.jrow  = s_jrow + (long)grp * M * (M + 1) + (long)(g.valid ? g.i : 0) * (M + 1);
+  g.jrow  = s_jrow + (long)grp * JS + (long)(g.valid ? g.i : 0) * (M + 1);
+  g.wbuf  = s_jrow + (long)grp * JS;
   g.cell  = 0;
-  g.kf    = A.kf;
-  g.kb    = A.kb;
+  // the rate constants are always read from the group's LDS slot (a pointer
+  // with a single, LDS, origin keeps every access a ds_read)
+  g.kf    = g.rates;
+  g.kb    = g.rates + A.R;
   Sys<M, T> S;
 
   bool busy = g_acquire(g, S, A.tout);

@@ -1751,3 +1751,12 @@ void CVodeFree(void** cvode_mem)
 /* the x^y used by the step-size controller, exported so its rounding can be
    checked (no reference counterpart) */
 double SUNAMD_Pow(double x, double y) { return sunamd_pow(x, y); }
+
+int SUNAMD_PowFastParts(double x, double y, double* res, double* hi, double* lo, double* bound,
+                        int* scale)
+{
+  *res = 0.0;
+  return sunamd_pow_fast(x, y, res, hi, lo, bound, scale);
+}
+
+double SUNAMD_PowExact(double x, double y) { return sunamd_pow_exact(x, y); }
