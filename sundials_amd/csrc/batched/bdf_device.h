@@ -294,25 +294,83 @@ __device__ __forceinline__ float readlane_d(float x, int l)
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
-// value of b in the lane that holds physical row k of this lane's group, for
-// groups of 64 or 32 lanes; invr is the lane's entry of the group's inverse
-// row map (lane base+j holds inv[j]); indexes are masked to the group, which
-// only matters for a group without a system (stale map, result unused)
+// b of lane base+k of this lane's group, for groups of 64 or 32 lanes
 template <int G>
-__device__ __forceinline__ double g_row_value(double b, int invr, int k)
+__device__ __forceinline__ double g_lane_value(double b, int k)
 {
-  if constexpr (G == 64)
-  {
-    const int lk = __builtin_amdgcn_readlane(invr, k) & 63;
-    return readlane_d(b, lk);
-  }
+  if constexpr (G == 64) { return readlane_d(b, k); }
   else
   {
-    const int l0    = __builtin_amdgcn_readlane(invr, k) & 31;
-    const int l1    = __builtin_amdgcn_readlane(invr, 32 + k) & 31;
-    const double b0 = readlane_d(b, l0);
-    const double b1 = readlane_d(b, 32 + l1);
+    const double b0 = readlane_d(b, k);
+    const double b1 = readlane_d(b, 32 + k);
     return (__lane_id() < 32) ? b0 : b1;
+  }
+}
+
+// ---- pivot search: largest |a_ik| over the group, ties to the smallest row
+// (the dense GETRF's first maximum), as a reduction on (value, row) pairs:
+// DPP exchanges inside rows of 16 lanes, lane reads across rows --------------
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x)
+{
+  return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_v(double x)
+{
+  const long long u = __double_as_longlong(x);
+  const int lo      = dpp_i<CTRL>((int)(u & 0xffffffffLL));
+  const int hi      = dpp_i<CTRL>((int)(u >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_v(float x)
+{
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(x)));
+}
+
+template <typename V>
+__device__ __forceinline__ void amax_take(V& v, int& p, V ov, int op)
+{
+  if (ov > v || (ov == v && op < p))
+  {
+    v = ov;
+    p = op;
+  }
+}
+
+template <int CTRL, typename V>
+__device__ __forceinline__ void amax_dpp(V& v, int& p)
+{
+  const V ov   = dpp_v<CTRL>(v);
+  const int op = dpp_i<CTRL>(p);
+  amax_take(v, p, ov, op);
+}
+
+// every lane of a G-lane group ends with the group's maximum pair
+template <int G, typename V>
+__device__ __forceinline__ void g_argmax(V& v, int& p)
+{
+  if constexpr (G >= 2) { amax_dpp<0xB1>(v, p); }  // quad_perm [1,0,3,2]
+  if constexpr (G >= 4) { amax_dpp<0x4E>(v, p); }  // quad_perm [2,3,0,1]
+  if constexpr (G >= 8) { amax_dpp<0x141>(v, p); } // row_half_mirror
+  if constexpr (G >= 16) { amax_dpp<0x140>(v, p); } // row_mirror
+  if constexpr (G >= 32)
+  {
+    const V r0 = readlane_d(v, 0), r1 = readlane_d(v, 16);
+    const V r2 = readlane_d(v, 32), r3 = readlane_d(v, 48);
+    const int p0 = __builtin_amdgcn_readlane(p, 0), p1 = __builtin_amdgcn_readlane(p, 16);
+    const int p2 = __builtin_amdgcn_readlane(p, 32), p3 = __builtin_amdgcn_readlane(p, 48);
+    V a = r0, b = r2;
+    int pa = p0, pb = p2;
+    amax_take(a, pa, r1, p1);
+    amax_take(b, pb, r3, p3);
+    if constexpr (G == 64) { amax_take(a, pa, b, pb); }
+    const bool hi_half = (G == 32) && (__lane_id() >= 32);
+    v = hi_half ? b : a;
+    p = hi_half ? pb : pa;
   }
 }
 
@@ -328,17 +386,7 @@ __device__ __forceinline__ int g_getrf(const Grp<M, T>& g, Sys<M, T>& S)
   {
     T v    = (g.valid && pos >= k) ? (T)fabs((double)S.lu[k]) : (T)-1;
     int bp = g.valid ? pos : INT_MAX;
-#pragma unroll
-    for (int off = G / 2; off >= 1; off >>= 1)
-    {
-      T ov   = __shfl_xor(v, off, 64);
-      int op = __shfl_xor(bp, off, 64);
-      if (ov > v || (ov == v && op < bp))
-      {
-        v  = ov;
-        bp = op;
-      }
-    }
+    g_argmax<G>(v, bp);
     if (v == (T)0 && fail == 0) { fail = k + 1; }
     if (pos == k) { pos = bp; }
     else if (pos == bp) { pos = k; }
@@ -391,6 +439,15 @@ __device__ __forceinline__ int g_getrf(const Grp<M, T>& g, Sys<M, T>& S)
   if (g.valid) { g.inv[pos] = g.i; }
   gsync();
   S.pos = pos;
+  if constexpr (G >= 32)
+  {
+    // one or two systems per wavefront: move physical row j to lane base+j,
+    // so the solves read row k's element from a fixed lane
+    const int src = g.base + g.inv[g.valid ? g.i : 0];
+#pragma unroll
+    for (int j = 0; j < M; j++) { S.lu[j] = __shfl(S.lu[j], src, 64); }
+    S.pos = g.i;
+  }
   return fail;
 }
 
@@ -404,23 +461,25 @@ __device__ __forceinline__ double g_getrs(const Grp<M, T>& g, const Sys<M, T>& S
   constexpr int G = pow2ceil(M);
   if constexpr (G >= 32)
   {
-    // one or two systems per wavefront: lane base+j keeps the lane of its
-    // group's physical row j, and rows are fetched by readlane (a uniform lane
-    // index) instead of a data-dependent permute through LDS
-    const int invr = g.inv[g.valid ? g.i : 0];
+    // rows sit in lane order (physical row j in lane base+j): gather b' (b's
+    // element of row j is held by the lane the row came from), then read
+    // row k's element from lane base+k; x_i ends in lane base+i
+    b = __shfl(b, g.base + g.inv[g.valid ? g.i : 0], 64);
 #pragma unroll
     for (int k = 0; k < M - 1; k++)
     {
-      const double bk = g_row_value<G>(b, invr, k);
-      if (pos > k) { b = b - (double)S.lu[k] * bk; }
+      const double bk = g_lane_value<G>(b, k);
+      if (g.i > k) { b = b - (double)S.lu[k] * bk; }
     }
 #pragma unroll
     for (int k = M - 1; k > 0; k--)
     {
-      if (pos == k) { b = b / (double)S.lu[k]; }
-      const double bk = g_row_value<G>(b, invr, k);
-      if (pos < k) { b = b - (double)S.lu[k] * bk; }
+      if (g.i == k) { b = b / (double)S.lu[k]; }
+      const double bk = g_lane_value<G>(b, k);
+      if (g.i < k) { b = b - (double)S.lu[k] * bk; }
     }
+    if (g.i == 0) { b = b / (double)S.lu[0]; }
+    return b;
   }
   else
   {
