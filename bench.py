@@ -376,7 +376,11 @@ def bench_batched(args):
         return wall.value
 
     # plain FP64 form (register budget tuned per bucket): warm up, then time one integration
-    w64, _, tune64 = autotune(core, hip, LU_FP64)
+    fixed = None if args.waves == "auto" else [int(args.waves)] * nb
+    if fixed is None:
+        w64, _, tune64 = autotune(core, hip, LU_FP64)
+    else:
+        w64, tune64 = fixed, {"waves": fixed, "fixed": True}
     print("[bench] fp64 waves per bucket: %s" % w64, file=sys.stderr, flush=True)
     B64 = make(LU_FP64, w64)
     run(B64, "fp64 warmup")
@@ -394,7 +398,10 @@ def bench_batched(args):
     if not args.no_learned:
         mlp, learned = train_predictor(args, core, hip)
         weights = hip.SUNAMD_MLPDeviceWeights(mlp)
-    w32, mask, tune32 = autotune(core, hip, LU_FP32, weights)
+    if fixed is None:
+        w32, mask, tune32 = autotune(core, hip, LU_FP32, weights)
+    else:
+        w32, mask, tune32 = fixed, 0, {"waves": fixed, "fixed": True}
     print("[bench] fp32 waves per bucket: %s, learned mask %d" % (w32, mask), file=sys.stderr,
           flush=True)
     if learned is not None:
@@ -503,7 +510,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["batched", "coupled"], default="batched")
     ap.add_argument("--systems", type=int, default=65536)
-    ap.add_argument("--cpu-systems", type=int, default=8192)
+    ap.add_argument("--cpu-systems", type=int, default=32768)
     ap.add_argument("--cells", type=int, default=1 << 20)
     ap.add_argument("--lu", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--micro-n", type=int, default=1 << 28)
@@ -511,6 +518,9 @@ def main():
     ap.add_argument("--no-micro", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--waves", choices=["auto", "1", "2", "4"], default="auto",
+                    help="wavefronts per SIMD for every bucket (auto: calibrated per bucket; "
+                         "fixed values also keep the learned start off)")
     ap.add_argument("--train-systems", type=int, default=4096)
     ap.add_argument("--train-pairs", type=int, default=1500000)
     ap.add_argument("--train-steps", type=int, default=3000)
