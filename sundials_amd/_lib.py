@@ -62,6 +62,43 @@ class TridiagData(C.Structure):
     _fields_ = [("N", C.c_int64), ("d", C.c_void_p), ("s1", C.c_void_p), ("s2", C.c_void_p)]
 
 
+class Adr(C.Structure):
+    """SUNAMD_Adr (include/sunamd/imex_adr.h)."""
+    _fields_ = [
+        ("n", C.c_int),
+        ("ncells", C.c_int64),
+        ("cx", C.c_double),
+        ("cy", C.c_double),
+        ("cz", C.c_double),
+        ("d", C.c_double),
+        ("A", C.c_double),
+        ("B", C.c_double),
+        ("kappa", C.POINTER(C.c_double)),
+        ("y0", C.POINTER(C.c_double)),
+        ("seed", C.c_uint64),
+    ]
+
+
+class AdrOptions(C.Structure):
+    """SUNAMD_AdrOptions (include/sunamd/imex_adr.h)."""
+    _fields_ = [
+        ("rtol", C.c_double),
+        ("atol", C.c_double),
+        ("tf", C.c_double),
+        ("maxl", C.c_int),
+        ("gstype", C.c_int),
+        ("controller", C.c_int),
+        ("sequential", C.c_int),
+        ("prec_fp32", C.c_int),
+        ("mxsteps", C.c_long),
+    ]
+
+
+ADR_STATS = 16
+ADR_STAT_NAMES = ("nst", "attempts", "netf", "ncfn", "nfe", "nfi", "nni", "nnf", "nsetups",
+                  "nli", "npe", "nps", "njtimes", "ncfl", "wall", "hlast")
+
+
 class Batch(C.Structure):
     """Mirror of SUNAMD_Batch (include/sunamd/batched_bdf.h)."""
     _fields_ = [
@@ -172,6 +209,10 @@ def core():
     _sig(lib, "SUNQRsol", C.c_int, [C.c_int, P(P(c_real)), P(c_real), P(c_real)])
     _sig(lib, "SUNRCompareTol", C.c_int, [c_real, c_real, c_real])
     _sig(lib, "SUNAMD_Pow", c_real, [c_real, c_real])
+    _sig(lib, "SUNAMD_AdrCreate", P(Adr), [C.c_int, C.c_uint64])
+    _sig(lib, "SUNAMD_AdrFree", None, [P(Adr)])
+    _sig(lib, "SUNAMD_ARK324Table", C.c_int, [C.c_int, P(c_real), P(c_real), P(c_real),
+                                                 P(c_real), P(C.c_int), P(C.c_int)])
     _sig(lib, "SUNAMD_PowExact", c_real, [c_real, c_real])
     _sig(lib, "SUNAMD_PowFastParts", C.c_int, [c_real, c_real] + [P(c_real)] * 4 + [P(C.c_int)])
     _sig(lib, "SUNAMD_BatchCreate", P(Batch),
@@ -234,6 +275,7 @@ def hip():
     _sig(lib, "SUNAMD_TridiagATimes_Mi355", C.c_int, [c_void_p, c_void_p, c_void_p])
     _sig(lib, "SUNAMD_TridiagPSetup_Mi355", C.c_int, [c_void_p])
     _sig(lib, "SUNAMD_TridiagPSolve_Mi355", C.c_int, [c_void_p, c_void_p, c_void_p, c_real, C.c_int])
+    _sig(lib, "SUNAMD_AdrSolve_Mi355", C.c_int, [P(Adr), P(AdrOptions), P(c_real), P(c_real)])
     _sig(lib, "SUNAMD_BatchedBDFCreate", c_void_p, [P(Batch), c_real, c_real, C.c_int])
     _sig(lib, "SUNAMD_BatchedBDFSetMaxNumSteps", C.c_int, [c_void_p, C.c_long])
     _sig(lib, "SUNAMD_BatchedBDFRun", C.c_int, [c_void_p, c_real, P(c_real)])
@@ -288,6 +330,9 @@ def load_oracle():
     _sig(lib, "oracle_tridiag_atimes", C.c_int, [c_void_p, c_void_p, c_void_p])
     _sig(lib, "oracle_tridiag_psetup", C.c_int, [c_void_p])
     _sig(lib, "oracle_tridiag_psolve", C.c_int, [c_void_p, c_void_p, c_void_p, c_real, C.c_int])
+    _sig(lib, "oracle_adr_solve", C.c_int, [P(Adr), P(AdrOptions), P(c_real), P(c_real)])
+    _sig(lib, "oracle_ark_dahlquist", C.c_int, [c_real, c_real, c_real, c_real, c_real, c_real,
+                                                C.c_char_p, C.c_int, P(c_real), P(c_real)])
     _sig(lib, "oracle_batch_solve", C.c_int,
          [P(Batch), c_real, c_real, c_real, C.c_int, P(c_real), P(C.c_long), P(c_real)])
     return lib

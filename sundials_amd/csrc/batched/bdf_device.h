@@ -240,7 +240,7 @@ __device__ __forceinline__ double g_rhs(const Grp<M, T>& g, double yi)
 // matrix row (I - gamma*J)_i with the dense ScaleAddI rounding --------------
 template <int M, typename T>
 __device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, double gamma,
-                                                double (&a)[M])
+                                                T (&a)[M])
 {
   gsync();
   if (g.valid) { g.red[g.i] = yj; }
@@ -271,12 +271,14 @@ __device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, d
     }
   }
   gsync();
+  // (I - gamma J)_ij in FP64, rounded once to the factor's precision
   const double c = -gamma;
 #pragma unroll
   for (int j = 0; j < M; j++)
   {
-    a[j] = g.valid ? row[j] * c : 0.0;
-    if (j == g.i) { a[j] = a[j] + 1.0; }
+    double v = g.valid ? row[j] * c : 0.0;
+    if (j == g.i) { v = v + 1.0; }
+    a[j] = (T)v;
   }
 }
 
@@ -733,18 +735,8 @@ __device__ __forceinline__ int g_lsetup(const Grp<M, T>& g, Sys<M, T>& S, bool j
     jcur   = true;
     S.ysav = S.y;
   }
-  if constexpr (sizeof(T) == sizeof(double))
-  {
-    // FP64: the Newton-matrix row is built in the factor registers themselves
-    g_newton_matrix(g, S.ysav, S.gamma, *reinterpret_cast<double(*)[M]>(&S.lu[0]));
-  }
-  else
-  {
-    double a[M];
-    g_newton_matrix(g, S.ysav, S.gamma, a);
-#pragma unroll
-    for (int j = 0; j < M; j++) { S.lu[j] = (T)a[j]; }
-  }
+  // the Newton-matrix row is built in the factor registers themselves
+  g_newton_matrix(g, S.ysav, S.gamma, S.lu);
   if (jcur)
   {
     S.nje++;

@@ -25,10 +25,13 @@ HEADERS = {
     "tridiag_mi355.h": "hip",
     "learned_predictor.h": "hip",
     "sunamd_profiler.h": "hip",
+    "arkode_sunamd.h": "core",
+    "imex_adr.h": "core",
 }
 
 # declared in a header whose functions live in the other library
-SPLIT = {"batched_bdf.h": {"SUNAMD_BatchCreate", "SUNAMD_BatchFree"}}
+SPLIT = {"batched_bdf.h": {"SUNAMD_BatchCreate", "SUNAMD_BatchFree"},
+         "imex_adr.h": {"SUNAMD_AdrSolve_Mi355"}}
 
 _DECL = re.compile(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", re.M)
 

@@ -1,0 +1,139 @@
+"""ARKStep IMEX restatement (core/arkode_ark.c) on the CPU (serial vector):
+
+- the ARK3(2)4L[2]SA tables satisfy the order-3 conditions of both tables
+  and the coupling conditions, the embedding's order-2 conditions, and the
+  row sums c_i (a mistyped coefficient fails one of them);
+- one fixed step of y' = lam_e y + lam_i y equals the pair's stability
+  function R(z_e, z_i) = 1 + (z_e b + z_i b)^T (I - z_e A_e - z_i A_i)^-1 1
+  (stage loop, stage setup and solution combination);
+- fixed-step errors fall by 2^3 per halving (order 3);
+- per-step Newton iterations and fe / fi evaluations equal the reference's
+  Dahlquist unit test for this pair (tests/golden/ark_dahlquist_ark324.json,
+  test/unit_tests/arkode/CXX_serial/ark_test_dahlquist_ark_0_{0,-1}_0.out);
+- on the advection-diffusion-reaction problem: the reference's RHS-count
+  identity nfi - nfe = nni (every attempt evaluates fe and fi once per stage,
+  fi once more per Newton iteration), the stop time is hit exactly, and the
+  adaptive solution's error against a 10^4x tighter run is of the order of
+  the tolerance.
+Parity unpinned: the adaptive controller's step sequence on a problem with
+an output in the reference's own examples (those use a band or dense direct
+solver, which this library does not provide)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from arkode_util import dahlquist, tables, make_problem, options, run_oracle
+from sundials_amd import _lib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_ark324_order_conditions():
+    T = tables()
+    for name in ("E", "I"):
+        A, b, c, d, q, p = T[name]
+        assert (q, p) == (3, 2)
+        assert np.allclose(A.sum(axis=1), c, atol=1e-15)
+        assert abs(b.sum() - 1) < 1e-15
+        assert abs(b @ c - 0.5) < 1e-15
+        assert abs(b @ c ** 2 - 1 / 3) < 1e-15
+        assert abs(b @ (A @ c) - 1 / 6) < 1e-15
+        assert abs(d.sum() - 1) < 1e-15
+        assert abs(d @ c - 0.5) < 1e-15
+        assert abs(b @ c ** 3 - 0.25) > 1e-6  # exactly order 3, not 4
+    AE, AI = T["E"][0], T["I"][0]
+    b, c = T["E"][1], T["E"][2]
+    assert np.array_equal(b, T["I"][1]) and np.array_equal(c, T["I"][2])
+    # coupling conditions of order 3
+    assert abs(b @ (AE @ c) - 1 / 6) < 1e-15 and abs(b @ (AI @ c) - 1 / 6) < 1e-15
+    # explicit table strictly lower triangular; ESDIRK: explicit first stage,
+    # constant diagonal, stiffly accurate
+    assert np.all(np.triu(AE) == 0)
+    assert np.all(np.triu(AI, 1) == 0) and AI[0, 0] == 0
+    g = AI[1, 1]
+    assert AI[2, 2] == g and AI[3, 3] == g and np.array_equal(AI[3], b)
+
+
+def _stability(ze, zi):
+    T = tables()
+    AE, AI, b = T["E"][0], T["I"][0], T["E"][1]
+    one = np.ones(4)
+    K = np.linalg.solve(np.eye(4) - ze * AE - zi * AI, one)
+    return 1.0 + (ze * b + zi * b) @ K
+
+
+@pytest.mark.parametrize("le,li,h,flags", [(-1.0, -1.0, 0.01, 0), (-0.5, -20.0, 0.05, 0),
+                                            (-2.0, -100.0, 0.02, 3), (0.3, -7.0, 0.1, 0)])
+def test_one_step_equals_stability_function(le, li, h, flags):
+    rc, st, y = dahlquist(le, li, h, h, rtol=1e-12, atol=1e-15, flags=flags)
+    assert rc == 0 and st["nst"] == 1
+    R = _stability(le * h, li * h)
+    assert abs(y - R) <= 1e-11 * abs(R)
+
+
+def test_fixed_step_order_three():
+    errs = []
+    for h in (0.1, 0.05, 0.025, 0.0125):
+        rc, st, y = dahlquist(-1.0, -4.0, h, 1.0, rtol=1e-12, atol=1e-15)
+        assert rc == 0
+        errs.append(abs(y - np.exp(-5.0)))
+    ratios = [errs[i] / errs[i + 1] for i in range(3)]
+    assert all(6.5 < r < 9.5 for r in ratios), ratios
+
+
+@pytest.mark.parametrize("interp,flags", [("hermite", 3), ("none", 7)])
+def test_dahlquist_counts_match_reference(interp, flags):
+    g = json.load(open(os.path.join(HERE, "golden", "ark_dahlquist_ark324.json")))
+    assert len(g[interp]) == 4
+    for row in g[interp]:
+        k = row["step"]
+        rc, st, _ = dahlquist(g["lambda_e"], g["lambda_i"], g["h"], g["h"] * k, rtol=g["rtol"],
+                              atol=g["atol"], flags=flags)
+        assert rc == 0 and st["nst"] == k
+        assert (st["nni"], st["nfe"], st["nfi"]) == (row["nni"], row["nfe"], row["nfi"]), \
+            (k, st["nni"], st["nfe"], st["nfi"], row)
+
+
+def test_stop_time_reached_exactly():
+    rc, st, _ = dahlquist(-1.0, -30.0, 0.0, 0.7, rtol=1e-6, atol=1e-10)
+    assert rc == 0 and st["wall"] == 0.7  # stats[14] holds tret for this driver
+    assert st["nst"] > 5
+
+
+@pytest.mark.parametrize("controller", [0, 1, 2])
+def test_adr_rhs_count_identity_and_accuracy(controller):
+    p = make_problem(8)
+    try:
+        o = options(rtol=1e-6, atol=1e-10, tf=0.1, controller=controller)
+        rc, st, y = run_oracle(p, o)
+        assert rc == 0
+        if st["ncfn"] == 0:
+            assert st["nfi"] - st["nfe"] == st["nni"]
+            extra = st["nfe"] - 4 * st["attempts"]
+            assert 1 <= extra <= 2 * 16  # initial-step estimation only
+        ot = options(rtol=1e-10, atol=1e-14, tf=0.1, controller=controller)
+        rc, st_t, yt = run_oracle(p, ot)
+        assert rc == 0
+        w = 1.0 / (1e-6 * np.abs(yt) + 1e-10)
+        err = np.sqrt(np.mean(((y - yt) * w) ** 2))
+        assert err < 5.0, err
+        assert st_t["nst"] > st["nst"]
+    finally:
+        _lib.core().SUNAMD_AdrFree(p)
+
+
+def test_adr_preconditioner_jacobian_reuse():
+    """The preconditioner's Jacobian is refreshed only when the linear-solver
+    interface asks for it (jok false): the first setup, every 51 steps, and
+    after convergence failures (arkode_ls.c:3196-3200)."""
+    p = make_problem(8)
+    try:
+        rc, st, _ = run_oracle(p, options(rtol=1e-6, atol=1e-10, tf=0.1))
+        assert rc == 0
+        assert 1 <= st["npe"] <= 1 + st["nst"] // 51 + st["ncfn"] + st["nnf"]
+        assert st["nsetups"] >= st["npe"]
+        assert st["nps"] >= st["nli"]
+    finally:
+        _lib.core().SUNAMD_AdrFree(p)

@@ -98,3 +98,25 @@ def test_pow_paths_agree():
                 assert r == core.SUNAMD_PowExact(x, y), (x, y)
             n += 1
     assert fast > 0.8 * n
+
+
+def test_pow_special_values_follow_c99():
+    """Zero, negative, infinite and NaN arguments take the C99 special values
+    (Annex F.9.4.4), as the C library's pow (numpy.power) does."""
+    import math
+
+    core = _lib.core()
+    inf, nan = float("inf"), float("nan")
+    xs = [0.0, -0.0, 1.0, -1.0, 2.0, -2.0, 0.5, -0.5, -3.7, inf, -inf, nan, 1e-310, -1e-310]
+    ys = [0.0, -0.0, 1.0, -1.0, 2.0, -2.0, 3.0, -3.0, 0.5, -0.5, 2.5, 1e300, -1e300, inf, -inf,
+          nan, 2.0 ** 60, 2.0 ** 53 + 2]
+    with np.errstate(all="ignore"):
+        for x in xs:
+            for y in ys:
+                got = core.SUNAMD_Pow(x, y)
+                want = float(np.power(np.float64(x), np.float64(y)))
+                if math.isnan(want):
+                    assert math.isnan(got), (x, y, got)
+                else:
+                    assert got == want and math.copysign(1, got) == math.copysign(1, want), \
+                        (x, y, got, want)
