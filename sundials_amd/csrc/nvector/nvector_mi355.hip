@@ -267,44 +267,50 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double* lds /* NV*
   }
 }
 
-// Each workgroup publishes NV partials; the last workgroup (ticket) folds
-// them in workgroup order and writes the results to `out` (pinned host).
+// Each workgroup publishes its NV partials; k_finish, launched next on the
+// same stream, folds them in workgroup order and writes the results to `out`
+// (pinned host memory or device memory).  Two launches instead of a
+// last-workgroup ticket: the kernel boundary orders the partials without a
+// device-scope fence and atomic per workgroup, which serialise at the end of
+// every reduction.  (ticket and out are kept in the kernels' signatures and
+// unused.)
 template <int OP, int NV>
 __device__ __forceinline__ void grid_finish(double (&v)[NV], double* partials,
                                             unsigned int* ticket, double* out,
                                             double identity)
 {
+  (void)ticket;
+  (void)out;
+  (void)identity;
   __shared__ double lds[NV * (NV_BLOCK / 64)];
-  __shared__ bool last;
   block_reduce<OP, NV>(v, lds);
   if (threadIdx.x == 0)
   {
 #pragma unroll
     for (int k = 0; k < NV; k++) { partials[k * gridDim.x + blockIdx.x] = v[k]; }
-    __threadfence();
-    unsigned int t = atomicAdd(ticket, 1u);
-    last           = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) { return; }
-  __threadfence();
+}
+
+template <int OP, int NV>
+__global__ void __launch_bounds__(NV_BLOCK) k_finish(const double* __restrict__ partials,
+                                                     int nblocks, double* out, double identity)
+{
+  __shared__ double lds[NV * (NV_BLOCK / 64)];
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++)
   {
     acc[k] = identity;
-    for (unsigned int b = threadIdx.x; b < gridDim.x; b += NV_BLOCK)
+    for (int b = threadIdx.x; b < nblocks; b += NV_BLOCK)
     {
-      acc[k] = rcomb<OP>(acc[k], __builtin_nontemporal_load(&partials[k * gridDim.x + b]));
+      acc[k] = rcomb<OP>(acc[k], __builtin_nontemporal_load(&partials[k * nblocks + b]));
     }
   }
-  __syncthreads();
   block_reduce<OP, NV>(acc, lds);
   if (threadIdx.x == 0)
   {
 #pragma unroll
     for (int k = 0; k < NV; k++) { out[k] = acc[k]; }
-    *ticket = 0u;
     __threadfence_system();
   }
 }
@@ -830,6 +836,8 @@ static double mi_reduce(N_Vector x, N_Vector y, N_Vector id, double identity)
     k_reduce<K><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), y ? DDATA(y) : DDATA(x),
                                                 id ? DDATA(id) : DDATA(x), n, identity,
                                                 ex->d_partials, ex->d_ticket, ex->h_result);
+    k_finish<RedTraits<K>::op, 1><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g,
+                                                                 ex->h_result, identity);
   }
   if (fam >= 0) { sunamd_prof_end(fam, ex->stream); }
   LAUNCHCHK(x);
@@ -862,8 +870,10 @@ static sunbooleantype mi_invtest(N_Vector x, N_Vector z)
   long n        = (long)NVLEN(x);
   Mi355Exec* ex = EXEC(x);
   if (n == 0) { return SUNTRUE; }
-  k_invtest<<<red_grid(ex, n >> 1), NV_BLOCK, 0, ex->stream>>>(
-    DDATA(x), DDATA(z), n, ex->d_partials, ex->d_ticket, ex->h_result);
+  const int g = red_grid(ex, n >> 1);
+  k_invtest<<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), DDATA(z), n, ex->d_partials,
+                                            ex->d_ticket, ex->h_result);
+  k_finish<RMAX, 1><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g, ex->h_result, 0.0);
   LAUNCHCHK(x);
   HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
   return (ex->h_result[0] > 0.0) ? SUNFALSE : SUNTRUE;
@@ -874,8 +884,10 @@ static sunbooleantype mi_constrmask(N_Vector c, N_Vector x, N_Vector m)
   long n        = (long)NVLEN(x);
   Mi355Exec* ex = EXEC(x);
   if (n == 0) { return SUNTRUE; }
-  k_constrmask<<<red_grid(ex, n >> 1), NV_BLOCK, 0, ex->stream>>>(
-    DDATA(c), DDATA(x), DDATA(m), n, ex->d_partials, ex->d_ticket, ex->h_result);
+  const int g = red_grid(ex, n >> 1);
+  k_constrmask<<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(c), DDATA(x), DDATA(m), n, ex->d_partials,
+                                               ex->d_ticket, ex->h_result);
+  k_finish<RMAX, 1><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g, ex->h_result, 0.0);
   LAUNCHCHK(x);
   HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
   return (ex->h_result[0] > 0.0) ? SUNFALSE : SUNTRUE;
@@ -886,8 +898,10 @@ static sunrealtype mi_minquotient(N_Vector num, N_Vector den)
   long n        = (long)NVLEN(num);
   Mi355Exec* ex = EXEC(num);
   if (n == 0) { return SUN_BIG_REAL; }
-  k_minquotient<<<red_grid(ex, n >> 1), NV_BLOCK, 0, ex->stream>>>(
-    DDATA(num), DDATA(den), n, ex->d_partials, ex->d_ticket, ex->h_result);
+  const int g = red_grid(ex, n >> 1);
+  k_minquotient<<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(num), DDATA(den), n, ex->d_partials,
+                                                ex->d_ticket, ex->h_result);
+  k_finish<RMIN, 1><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g, ex->h_result, DBL_MAX);
   LAUNCHCHK(num);
   HIPCHK(num->sunctx, hipStreamSynchronize(ex->stream));
   return ex->h_result[0];
@@ -938,6 +952,7 @@ static SUNErrCode mi_dotprodmulti(int nvec, N_Vector x, N_Vector* Y, sunrealtype
   case NVV:                                                                                \
     k_dotmulti<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), P, n, ex->d_partials,        \
                                                     ex->d_ticket, slot);                   \
+    k_finish<RSUM, NVV><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g, slot, 0.0);     \
     break;
       NV_CASES(DM_CASE)
 #undef DM_CASE
@@ -1536,6 +1551,8 @@ extern "C" SUNErrCode SUNAMD_ClassicalGS_Mi355(N_Vector* v, sunrealtype** h, int
   case NVV:                                                                                 \
     k_dotmulti<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(vk), P, n, ex->d_partials,        \
                                                     ex->d_ticket, ex->d_gs + base);         \
+    k_finish<RSUM, NVV><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g, ex->d_gs + base,  \
+                                                        0.0);                               \
     break;
       NV_CASES(GD_CASE)
 #undef GD_CASE
@@ -1559,6 +1576,11 @@ extern "C" SUNErrCode SUNAMD_ClassicalGS_Mi355(N_Vector* v, sunrealtype** h, int
     k_gs_update<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(vk), P, ex->d_gs + base, n, last, \
                                                      ex->d_partials, ex->d_ticket,           \
                                                      ex->d_gs + nv + 1);                     \
+    if (last)                                                                                \
+    {                                                                                        \
+      k_finish<RSUM, 1><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g, ex->d_gs + nv + 1, \
+                                                        0.0);                                \
+    }                                                                                        \
     break;
       NV_CASES(GU_CASE)
 #undef GU_CASE
