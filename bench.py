@@ -24,6 +24,8 @@ CPU oracle bit for bit).
 --workload coupled keeps the earlier benchmark: the 2^20-cell, 32-species
 batch as one coupled system through the HIP N_Vector, block-diagonal matrix
 and batched block LU, plus the N_Vector microbench at N = 2^28.
+--workload imex: ARKStep IMEX + SPGMR + block-Jacobi on the periodic
+128^3 x 3 advection-diffusion-reaction problem (the Newton-Krylov path).
 """
 import argparse
 import ctypes as C
@@ -332,6 +334,123 @@ def autotune(core, hip, prec, weights=None):
     return best_w, mask, info
 
 
+IMEX_SEED = 1003
+FAM_LINSUM, FAM_WRMS, FAM_ADR_FE, FAM_SCALE = 2, 3, 7, 8
+
+
+def bench_imex(args):
+    """ARKStep IMEX (ARK3(2)4L[2]SA) + SPGMR (maxl 20, fused device classical
+    Gram-Schmidt) + per-cell block-Jacobi on the advection-diffusion-reaction
+    problem, n^3 cells x 3 species; a step is one integration over [0, tf]."""
+    import torch
+    from sundials_amd import _lib
+
+    core, hip = _lib.core(), _lib.hip()
+    PD = C.POINTER(C.c_double)
+    n = args.grid
+    p = core.SUNAMD_AdrCreate(n, IMEX_SEED)
+    ncells = p.contents.ncells
+    N = 3 * ncells
+
+    def run(prec32=0):
+        o = _lib.AdrOptions(1e-6, 1e-10, args.tf, 20, 2, 0, 0, prec32, 1000000)
+        st = np.zeros(_lib.ADR_STATS)
+        rc = hip.SUNAMD_AdrSolve_Mi355(p, C.byref(o), st.ctypes.data_as(PD), None)
+        if rc != 0:
+            print("[bench] imex solve failed rc=%d" % rc, file=sys.stderr)
+            sys.exit(1)
+        return dict(zip(_lib.ADR_STAT_NAMES, st))
+
+    for w in range(args.warmup):
+        st = run()
+        print("[bench] warmup %d: %.3f s, %d steps" % (w, st["wall"], st["nst"]), file=sys.stderr,
+              flush=True)
+    torch.cuda.synchronize()
+    walls, stats = [], None
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        stats = run()
+        walls.append(stats["wall"])
+        print("[bench] step %d: %.3f s, %d steps, %d Newton, %d Krylov" % (
+            k, stats["wall"], stats["nst"], stats["nni"], stats["nli"]), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = ncells * stats["nst"] * args.steps / elapsed
+
+    # FP32 preconditioner factors, timed once
+    s32 = run(1)
+    # per-family kernel times of one more integration (HIP events on the vector stream)
+    hip.SUNAMD_ProfilerReset()
+    hip.SUNAMD_ProfilerEnable(1)
+    sprof = run()
+    hip.SUNAMD_ProfilerEnable(0)
+    fams = {FAM_LINSUM: ("N_VLinearSum", 24.0 * N), FAM_WRMS: ("N_VWrmsNorm", 16.0 * N),
+            FAM_ADR_FE: ("advection-diffusion RHS (7-point stencil)", 48.0 * ncells),
+            FAM_SCALE: ("N_VScale", 16.0 * N)}
+    kern = {}
+    for f, (name, nbytes) in fams.items():
+        tot, cnt = C.c_double(), C.c_long()
+        hip.SUNAMD_ProfilerGet(f, C.byref(tot), C.byref(cnt))
+        if cnt.value:
+            kern[f] = {"name": name, "total_ms": tot.value, "launches": cnt.value,
+                       "avg_us": 1e3 * tot.value / cnt.value, "bytes_per_launch": nbytes}
+    dom = max(kern, key=lambda f: kern[f]["total_ms"])
+    dk = kern[dom]
+    achieved = dk["bytes_per_launch"] / (dk["avg_us"] * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "kernel": dk["name"], "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "bytes_per_launch": dk["bytes_per_launch"],
+                "avg_launch_us": round(dk["avg_us"], 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        orc = _lib.load_oracle()
+        nc = args.cpu_grid
+        pc = core.SUNAMD_AdrCreate(nc, IMEX_SEED)
+        o = _lib.AdrOptions(1e-6, 1e-10, args.tf, 20, 2, 0, 1, 0, 1000000)
+        st = np.zeros(_lib.ADR_STATS)
+        t = time.perf_counter()
+        rc = orc.oracle_adr_solve(pc, C.byref(o), st.ctypes.data_as(PD), None)
+        wall = time.perf_counter() - t
+        if rc == 0:
+            cs = dict(zip(_lib.ADR_STAT_NAMES, st))
+            cpu = {"value": pc.contents.ncells * cs["nst"] / wall, "unit": "cell-steps/s",
+                   "cores": 1, "kind": "port",
+                   "sample": "CPU oracle (serial vector, the same ARKStep + SPGMR + block-Jacobi "
+                             "restatement) on the %d^3 grid over the same [0, %g] (%d steps, "
+                             "%.1f s, %s)" % (nc, args.tf, cs["nst"], wall, cpu_model())}
+        core.SUNAMD_AdrFree(pc)
+    core.SUNAMD_AdrFree(p)
+    ms = 1e3 * elapsed / args.steps
+    return {
+        "metric": "cell-steps/s (ARKStep IMEX + SPGMR + block-Jacobi, advection-reaction)",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "advection-diffusion-reaction on a periodic %d^3 grid x 3 species, "
+                               "ARKStep IMEX ARK3(2)4L[2]SA, SPGMR maxl 20 (classical GS), "
+                               "block-Jacobi 3x3, rtol 1e-6 atol 1e-10, t in [0, %g]"
+                               % (n, args.tf), "cells": ncells},
+        "integrator": {k: int(stats[k]) for k in ("nst", "attempts", "netf", "ncfn", "nfe", "nfi",
+                                                  "nni", "nli", "nsetups", "npe", "nps")},
+        "fp32_preconditioner": {"ms": round(1e3 * s32["wall"], 3), "nst": int(s32["nst"]),
+                                "nli": int(s32["nli"]),
+                                "value": ncells * s32["nst"] / s32["wall"]},
+        "roofline": roofline,
+        "kernels": {kern[f]["name"]: {k: v for k, v in kern[f].items() if k != "name"}
+                    for f in kern},
+        "cpu_baseline": cpu,
+    }
+
+
 def bench_batched(args):
     import torch
     from sundials_amd import _lib
@@ -508,7 +627,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["batched", "coupled"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "coupled", "imex"], default="batched")
+    ap.add_argument("--grid", type=int, default=128, help="imex: cells per dimension")
+    ap.add_argument("--tf", type=float, default=0.01, help="imex: final time")
+    ap.add_argument("--cpu-grid", type=int, default=64, help="imex: CPU-oracle grid")
     ap.add_argument("--systems", type=int, default=65536)
     ap.add_argument("--cpu-systems", type=int, default=32768)
     ap.add_argument("--cells", type=int, default=1 << 20)
@@ -538,7 +660,12 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
-    line = bench_batched(args) if args.workload == "batched" else bench_coupled(args)
+    if args.workload == "batched":
+        line = bench_batched(args)
+    elif args.workload == "imex":
+        line = bench_imex(args)
+    else:
+        line = bench_coupled(args)
     print(json.dumps(line))
 
 

@@ -26,7 +26,9 @@ extern "C" {
 #define SUNAMD_K_DOTMULTI  4 /* N_VDotProdMulti */
 #define SUNAMD_K_RHS       5 /* mass-action right-hand side */
 #define SUNAMD_K_BDF_BATCH 6 /* batched per-system BDF */
-#define SUNAMD_K_COUNT     8
+#define SUNAMD_K_ADR_FE    7 /* advection-diffusion explicit RHS (IMEX problem) */
+#define SUNAMD_K_SCALE     8 /* N_VScale (also copies and negations) */
+#define SUNAMD_K_COUNT     9
 
 void SUNAMD_ProfilerEnable(int on);
 void SUNAMD_ProfilerReset(void);

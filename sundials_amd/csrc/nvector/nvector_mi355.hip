@@ -801,7 +801,10 @@ static void mi_div(N_Vector x, N_Vector y, N_Vector z) { mi_ew<EW_DIV>(x, y, 0.0
 static void mi_scale(sunrealtype c, N_Vector x, N_Vector z)
 {
   // c == 1 (copy) and c == -1 (negate) give the same bits as c*x
+  hipStream_t s = EXEC(z)->stream;
+  sunamd_prof_begin(SUNAMD_K_SCALE, s);
   mi_ew<EW_SCALE>(x, NULL, c, z);
+  sunamd_prof_end(SUNAMD_K_SCALE, s);
 }
 static void mi_abs(N_Vector x, N_Vector z) { mi_ew<EW_ABS>(x, NULL, 0.0, z); }
 static void mi_inv(N_Vector x, N_Vector z) { mi_ew<EW_INV>(x, NULL, 0.0, z); }

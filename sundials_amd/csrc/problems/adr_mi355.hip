@@ -12,10 +12,12 @@
 #include <cstdio>
 #include <cstring>
 
+#include "../core/profiler_internal.h"
 #include "adr_cell.h"
 #include "sunamd/arkode_sunamd.h"
 #include "sunamd/imex_adr.h"
 #include "sunamd/nvector_mi355.h"
+#include "sunamd/sunamd_profiler.h"
 #include "sunamd/sundials_core.h"
 #include "sunamd/sunlinsol_spgmr.h"
 
@@ -161,8 +163,10 @@ int cb_fe(sunrealtype t, N_Vector y, N_Vector f, void* ud)
   (void)t;
   const AdrDev* P = (const AdrDev*)ud;
   hipStream_t s   = (hipStream_t)N_VGetStream_Mi355(f);
+  sunamd_prof_begin(SUNAMD_K_ADR_FE, s);
   k_fe<<<blocks(P->ncells), TPB, 0, s>>>(*P, N_VGetDeviceArrayPointer(y),
                                          N_VGetDeviceArrayPointer(f));
+  sunamd_prof_end(SUNAMD_K_ADR_FE, s);
   return launch_check("fe");
 }
 
