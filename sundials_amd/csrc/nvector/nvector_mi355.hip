@@ -5,16 +5,17 @@
 // :101-195; per-op formulas from src/nvector/serial/nvector_serial.c:387-1592).
 // Design, replacing the reference HIP vector (src/nvector/hip/nvector_hip.hip.cpp,
 // VectorKernels.hip.hpp, VectorArrayKernels.hip.hpp):
-//  * streaming ops move 128-bit (2 x FP64) per lane per access, non-temporal
-//    for vectors larger than the last-level cache, grid sized to the device;
+//  * streaming ops move 128-bit (2 x FP64) per lane per access, non-temporal,
+//    in contiguous per-workgroup chunks with several loads in flight per
+//    lane before the first use; grid sized to the device;
 //  * LinearSum keeps the serial vector's rounding: its nine special cases
 //    collapse to three per-element formulas, a*x+b*y, a*(x+y), a*(x-y)
 //    (nvector_serial.c:387-480 / :1600-1775), with no fused multiply-add;
 //  * reductions: per-lane sequential accumulation, 64-wide wavefront
-//    shuffle tree, LDS across wavefronts, then the last workgroup to finish
-//    (ticket counter) folds the per-workgroup partials in a fixed order and
-//    writes the scalar(s) straight into pinned host memory -- one kernel and
-//    one stream synchronisation per reduction, deterministic results;
+//    shuffle tree, LDS across wavefronts, per-workgroup partials; a
+//    one-workgroup finish kernel folds them in workgroup order and writes the
+//    scalar(s) straight into pinned host memory -- one stream synchronisation
+//    per reduction, deterministic results;
 //  * DotProdMulti reads x once for all nvec dot products (the serial vector
 //    re-reads x per product, nvector_serial.c:1019-1024) and spreads the work
 //    over the whole device (the reference uses one workgroup per product);
@@ -50,7 +51,6 @@ struct Mi355Exec
   int num_cus;
   hipStream_t stream;
   double* d_partials;     // NV_MAX_RED_BLOCKS * (NV_MAX_MULTI + 1)
-  unsigned int* d_ticket; // last-workgroup ticket (kept at zero between calls)
   double* h_result;       // pinned, mapped: NV_MAX_MULTI + 1 scalars
   double* d_gs;           // Gram-Schmidt coefficients on the device (lazy, GS_MAX + 2)
   double* h_gs;           // pinned copy of them (lazy)
@@ -129,8 +129,6 @@ static Mi355Exec* exec_create(SUNContext ctx, hipStream_t user_stream, bool own_
   }
   e = hipMalloc((void**)&ex->d_partials,
                 sizeof(double) * NV_MAX_RED_BLOCKS * (NV_MAX_MULTI + 1));
-  if (e == hipSuccess) { e = hipMalloc((void**)&ex->d_ticket, sizeof(unsigned int)); }
-  if (e == hipSuccess) { e = hipMemset(ex->d_ticket, 0, sizeof(unsigned int)); }
   if (e == hipSuccess)
   {
     e = hipHostMalloc((void**)&ex->h_result, sizeof(double) * (NV_MAX_MULTI + 1),
@@ -140,7 +138,6 @@ static Mi355Exec* exec_create(SUNContext ctx, hipStream_t user_stream, bool own_
   {
     hip_fail(ctx, e, "exec allocation", __LINE__);
     if (ex->d_partials) { (void)hipFree(ex->d_partials); }
-    if (ex->d_ticket) { (void)hipFree(ex->d_ticket); }
     if (own_stream) { (void)hipStreamDestroy(ex->stream); }
     free(ex);
     return NULL;
@@ -169,7 +166,6 @@ static void exec_release(Mi355Exec* ex)
   if (--ex->refcount > 0) { return; }
   (void)hipStreamSynchronize(ex->stream);
   (void)hipFree(ex->d_partials);
-  (void)hipFree(ex->d_ticket);
   (void)hipHostFree(ex->h_result);
   if (ex->d_gs) { (void)hipFree(ex->d_gs); }
   if (ex->h_gs) { (void)hipHostFree(ex->h_gs); }
@@ -185,23 +181,35 @@ static void exec_release(Mi355Exec* ex)
 // ----------------------------------------------------------------------------
 // launch geometry
 // ----------------------------------------------------------------------------
-// streaming: one 128-bit element pair per lane per iteration, grid capped at
-// 16 workgroups per CU and grid-strided beyond that
-static inline int stream_grid(const Mi355Exec* ex, sunindextype npairs)
+// Element pairs a lane handles per chunk for a kernel that issues `loads`
+// 128-bit loads per pair: about 16 loads in flight per lane (a measured sweep
+// over 6.3M-2^28 elements: 1 pair per lane per iteration leaves ~20 % of the
+// bandwidth unused), fewer pairs when each pair already reads many vectors.
+__host__ __device__ constexpr int unroll_for(int loads)
 {
-  sunindextype g   = (npairs + NV_BLOCK - 1) / NV_BLOCK;
-  sunindextype cap = (sunindextype)ex->num_cus * 16;
+  return loads <= 2 ? 8 : (loads <= 4 ? 4 : (loads <= 8 ? 2 : 1));
+}
+
+// streaming: chunks of U * NV_BLOCK pairs per workgroup, grid capped at 32
+// workgroups per CU and chunk-strided beyond that
+static inline int stream_grid(const Mi355Exec* ex, sunindextype npairs, int U)
+{
+  const sunindextype chunk = (sunindextype)U * NV_BLOCK;
+  sunindextype g           = (npairs + chunk - 1) / chunk;
+  sunindextype cap         = (sunindextype)ex->num_cus * 32;
   if (g > cap) { g = cap; }
   if (g < 1) { g = 1; }
   return (int)g;
 }
 
-// The reduction grid depends only on the vector length (not on the device),
-// so the summation order -- and the rounded result -- is a fixed function of n.
-static inline int red_grid(const Mi355Exec* ex, sunindextype npairs)
+// The reduction grid depends only on the vector length and the kernel (not on
+// the device), so the summation order -- and the rounded result -- is a fixed
+// function of n.
+static inline int red_grid(const Mi355Exec* ex, sunindextype npairs, int U)
 {
   (void)ex;
-  sunindextype g = (npairs + 4 * NV_BLOCK - 1) / (4 * NV_BLOCK);
+  const sunindextype chunk = (sunindextype)U * NV_BLOCK;
+  sunindextype g           = (npairs + chunk - 1) / chunk;
   if (g > NV_MAX_RED_BLOCKS) { g = NV_MAX_RED_BLOCKS; }
   if (g < 1) { g = 1; }
   return (int)g;
@@ -272,16 +280,10 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double* lds /* NV*
 // (pinned host memory or device memory).  Two launches instead of a
 // last-workgroup ticket: the kernel boundary orders the partials without a
 // device-scope fence and atomic per workgroup, which serialise at the end of
-// every reduction.  (ticket and out are kept in the kernels' signatures and
-// unused.)
+// every reduction.
 template <int OP, int NV>
-__device__ __forceinline__ void grid_finish(double (&v)[NV], double* partials,
-                                            unsigned int* ticket, double* out,
-                                            double identity)
+__device__ __forceinline__ void grid_finish(double (&v)[NV], double* partials)
 {
-  (void)ticket;
-  (void)out;
-  (void)identity;
   __shared__ double lds[NV * (NV_BLOCK / 64)];
   block_reduce<OP, NV>(v, lds);
   if (threadIdx.x == 0)
@@ -318,8 +320,41 @@ __global__ void __launch_bounds__(NV_BLOCK) k_finish(const double* __restrict__ 
 // ----------------------------------------------------------------------------
 // streaming kernels: pairs [0, n/2) vectorised, odd tail element by thread 0
 // ----------------------------------------------------------------------------
-#define GRID_LOOP(np) \
-  for (long i = (long)blockIdx.x * NV_BLOCK + threadIdx.x; i < (np); i += (long)gridDim.x * NV_BLOCK)
+// Pairs [0, np) in chunks of U * NV_BLOCK consecutive pairs, chunks strided
+// over the grid; lane t of a chunk starting at c takes pairs c + t + u *
+// NV_BLOCK, u < U.  A full chunk issues all of a lane's loads before the
+// first use (U * loads per pair in flight, a wavefront's loads inside one
+// contiguous chunk); the last, partial chunk goes pair by pair.  Each lane
+// visits its pairs in increasing order, so per-lane accumulations have an
+// order fixed by np and the grid.
+template <int U, typename Load, typename Use>
+__device__ __forceinline__ void chunk_pairs(long np, Load load, Use use)
+{
+  const long chunk = (long)U * NV_BLOCK;
+  for (long c = (long)blockIdx.x * chunk; c < np; c += (long)gridDim.x * chunk)
+  {
+    const long i0 = c + threadIdx.x;
+    if (c + chunk <= np)
+    {
+      decltype(load(0L)) v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) { v[u] = load(i0 + (long)u * NV_BLOCK); }
+#pragma unroll
+      for (int u = 0; u < U; u++) { use(i0 + (long)u * NV_BLOCK, v[u]); }
+    }
+    else
+    {
+      for (long i = i0; i < np; i += NV_BLOCK) { use(i, load(i)); }
+    }
+  }
+}
+
+template <int K>
+struct Pairs
+{
+  dbl2 v[K];
+};
+
 #define TAIL(n) if (((n) & 1) && blockIdx.x == 0 && threadIdx.x == 0)
 
 enum LSMode
@@ -335,14 +370,15 @@ __global__ void __launch_bounds__(NV_BLOCK) k_linearsum(double a, const double* 
                                                         double* __restrict__ z, long n)
 {
   const long np = n >> 1;
-  GRID_LOOP(np)
-  {
-    dbl2 xv = ld2(x, i), yv = ld2(y, i), r;
-    if (MODE == LS_GENERIC) { r = (a * xv) + (b * yv); }
-    else if (MODE == LS_SCALESUM) { r = a * (xv + yv); }
-    else { r = a * (xv - yv); }
-    st2(z, i, r);
-  }
+  chunk_pairs<unroll_for(2)>(
+    np, [&](long i) { return Pairs<2>{{ld2(x, i), ld2(y, i)}}; },
+    [&](long i, const Pairs<2>& p) {
+      dbl2 r;
+      if (MODE == LS_GENERIC) { r = (a * p.v[0]) + (b * p.v[1]); }
+      else if (MODE == LS_SCALESUM) { r = a * (p.v[0] + p.v[1]); }
+      else { r = a * (p.v[0] - p.v[1]); }
+      st2(z, i, r);
+    });
   TAIL(n)
   {
     double xv = x[n - 1], yv = y[n - 1];
@@ -356,7 +392,8 @@ __global__ void __launch_bounds__(NV_BLOCK) k_const(double c, double* __restrict
 {
   const long np = n >> 1;
   dbl2 cv       = {c, c};
-  GRID_LOOP(np) { st2(z, i, cv); }
+  const long stride = (long)gridDim.x * NV_BLOCK;
+  for (long i = (long)blockIdx.x * NV_BLOCK + threadIdx.x; i < np; i += stride) { st2(z, i, cv); }
   TAIL(n) { z[n - 1] = c; }
 }
 
@@ -391,15 +428,21 @@ __global__ void __launch_bounds__(NV_BLOCK) k_ew(const double* __restrict__ x,
 {
   const bool binary = (OP == EW_PROD || OP == EW_DIV);
   const long np     = n >> 1;
-  GRID_LOOP(np)
-  {
-    dbl2 xv = ld2(x, i);
-    dbl2 yv = binary ? ld2(y, i) : xv;
-    dbl2 r;
-    r.x = ew<OP>(xv.x, yv.x, c);
-    r.y = ew<OP>(xv.y, yv.y, c);
-    st2(z, i, r);
-  }
+  constexpr int NL  = binary ? 2 : 1;
+  chunk_pairs<unroll_for(NL)>(
+    np,
+    [&](long i) {
+      Pairs<NL> p;
+      p.v[0] = ld2(x, i);
+      if (binary) { p.v[NL - 1] = ld2(y, i); }
+      return p;
+    },
+    [&](long i, const Pairs<NL>& p) {
+      dbl2 r;
+      r.x = ew<OP>(p.v[0].x, p.v[NL - 1].x, c);
+      r.y = ew<OP>(p.v[0].y, p.v[NL - 1].y, c);
+      st2(z, i, r);
+    });
   TAIL(n) { z[n - 1] = ew<OP>(x[n - 1], binary ? y[n - 1] : 0.0, c); }
 }
 
@@ -468,32 +511,47 @@ __device__ __forceinline__ double red_elem(double x, double y, double id)
   return fabs(x);
 }
 
+// 128-bit loads per element pair of reduction K
+template <int K>
+__host__ __device__ constexpr int red_loads()
+{
+  return K == RK_WSQRMASK ? 3 : ((K == RK_DOT || K == RK_WSQR) ? 2 : 1);
+}
+
 template <int K>
 __global__ void __launch_bounds__(NV_BLOCK) k_reduce(const double* __restrict__ x,
                                                      const double* __restrict__ y,
                                                      const double* __restrict__ id, long n,
-                                                     double identity, double* partials,
-                                                     unsigned int* ticket, double* out)
+                                                     double identity, double* partials)
 {
   const int OP     = RedTraits<K>::op;
   const bool two   = (K == RK_DOT || K == RK_WSQR || K == RK_WSQRMASK);
   const bool three = (K == RK_WSQRMASK);
   double acc[1]    = {identity};
   const long np    = n >> 1;
-  GRID_LOOP(np)
-  {
-    dbl2 xv = ld2(x, i);
-    dbl2 yv = two ? ld2(y, i) : xv;
-    dbl2 iv = three ? ld2(id, i) : xv;
-    acc[0]  = rcomb<OP>(acc[0], red_elem<K>(xv.x, yv.x, iv.x));
-    acc[0]  = rcomb<OP>(acc[0], red_elem<K>(xv.y, yv.y, iv.y));
-  }
+  chunk_pairs<unroll_for(red_loads<K>())>(
+    np,
+    [&](long i) {
+      Pairs<red_loads<K>()> p;
+      p.v[0] = ld2(x, i);
+      if (two) { p.v[1 % red_loads<K>()] = ld2(y, i); }
+      if (three) { p.v[2 % red_loads<K>()] = ld2(id, i); }
+      return p;
+    },
+    [&](long i, const Pairs<red_loads<K>()>& p) {
+      (void)i;
+      const dbl2 xv = p.v[0];
+      const dbl2 yv = p.v[two ? 1 % red_loads<K>() : 0];
+      const dbl2 iv = p.v[three ? 2 % red_loads<K>() : 0];
+      acc[0]        = rcomb<OP>(acc[0], red_elem<K>(xv.x, yv.x, iv.x));
+      acc[0]        = rcomb<OP>(acc[0], red_elem<K>(xv.y, yv.y, iv.y));
+    });
   TAIL(n)
   {
     acc[0] = rcomb<OP>(acc[0], red_elem<K>(x[n - 1], two ? y[n - 1] : 0.0,
                                           three ? id[n - 1] : 0.0));
   }
-  grid_finish<OP, 1>(acc, partials, ticket, out, identity);
+  grid_finish<OP, 1>(acc, partials);
 }
 
 // Left-to-right sums in the serial vector's order (nvector_serial.c:605-761):
@@ -566,30 +624,36 @@ __global__ void __launch_bounds__(64) k_dotmulti_seq(const double* __restrict__ 
 
 template <int NV>
 __global__ void __launch_bounds__(NV_BLOCK) k_dotmulti(const double* __restrict__ x, PtrPack Y,
-                                                       long n, double* partials,
-                                                       unsigned int* ticket, double* out)
+                                                       long n, double* partials)
 {
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) { acc[k] = 0.0; }
   const long np = n >> 1;
-  GRID_LOOP(np)
-  {
-    dbl2 xv = ld2(x, i);
+  chunk_pairs<unroll_for(NV + 1)>(
+    np,
+    [&](long i) {
+      Pairs<NV + 1> p;
+      p.v[0] = ld2(x, i);
 #pragma unroll
-    for (int k = 0; k < NV; k++)
-    {
-      dbl2 yv = ld2(Y.p[k], i);
-      acc[k] += xv.x * yv.x;
-      acc[k] += xv.y * yv.y;
-    }
-  }
+      for (int k = 0; k < NV; k++) { p.v[k + 1] = ld2(Y.p[k], i); }
+      return p;
+    },
+    [&](long i, const Pairs<NV + 1>& p) {
+      (void)i;
+#pragma unroll
+      for (int k = 0; k < NV; k++)
+      {
+        acc[k] += p.v[0].x * p.v[k + 1].x;
+        acc[k] += p.v[0].y * p.v[k + 1].y;
+      }
+    });
   TAIL(n)
   {
 #pragma unroll
     for (int k = 0; k < NV; k++) { acc[k] += x[n - 1] * Y.p[k][n - 1]; }
   }
-  grid_finish<RSUM, NV>(acc, partials, ticket, out, 0.0);
+  grid_finish<RSUM, NV>(acc, partials);
 }
 
 // z = c0*X0; z += c_k*X_k (k = 1..nvec-1): the rounding of the generic-layer
@@ -598,13 +662,20 @@ template <int NV>
 __global__ void __launch_bounds__(NV_BLOCK) k_lincomb(CoefPack c, PtrPack X, double* z, long n)
 {
   const long np = n >> 1;
-  GRID_LOOP(np)
-  {
-    dbl2 acc = c.c[0] * ld2(X.p[0], i);
+  chunk_pairs<unroll_for(NV)>(
+    np,
+    [&](long i) {
+      Pairs<NV> p;
 #pragma unroll
-    for (int k = 1; k < NV; k++) { acc = acc + c.c[k] * ld2(X.p[k], i); }
-    st2(z, i, acc);
-  }
+      for (int k = 0; k < NV; k++) { p.v[k] = ld2(X.p[k], i); }
+      return p;
+    },
+    [&](long i, const Pairs<NV>& p) {
+      dbl2 acc = c.c[0] * p.v[0];
+#pragma unroll
+      for (int k = 1; k < NV; k++) { acc = acc + c.c[k] * p.v[k]; }
+      st2(z, i, acc);
+    });
   TAIL(n)
   {
     double acc = c.c[0] * X.p[0][n - 1];
@@ -620,12 +691,19 @@ __global__ void __launch_bounds__(NV_BLOCK) k_scaleaddmulti(CoefPack a, const do
                                                             PtrPack Y, WPtrPack Z, long n)
 {
   const long np = n >> 1;
-  GRID_LOOP(np)
-  {
-    dbl2 xv = ld2(x, i);
+  chunk_pairs<unroll_for(NV + 1)>(
+    np,
+    [&](long i) {
+      Pairs<NV + 1> p;
+      p.v[0] = ld2(x, i);
 #pragma unroll
-    for (int k = 0; k < NV; k++) { st2(Z.p[k], i, (a.c[k] * xv) + ld2(Y.p[k], i)); }
-  }
+      for (int k = 0; k < NV; k++) { p.v[k + 1] = ld2(Y.p[k], i); }
+      return p;
+    },
+    [&](long i, const Pairs<NV + 1>& p) {
+#pragma unroll
+      for (int k = 0; k < NV; k++) { st2(Z.p[k], i, (a.c[k] * p.v[0]) + p.v[k + 1]); }
+    });
   TAIL(n)
   {
 #pragma unroll
@@ -638,11 +716,18 @@ template <int NV>
 __global__ void __launch_bounds__(NV_BLOCK) k_scalearray(CoefPack c, PtrPack X, WPtrPack Z, long n)
 {
   const long np = n >> 1;
-  GRID_LOOP(np)
-  {
+  chunk_pairs<unroll_for(NV)>(
+    np,
+    [&](long i) {
+      Pairs<NV> p;
 #pragma unroll
-    for (int k = 0; k < NV; k++) { st2(Z.p[k], i, c.c[k] * ld2(X.p[k], i)); }
-  }
+      for (int k = 0; k < NV; k++) { p.v[k] = ld2(X.p[k], i); }
+      return p;
+    },
+    [&](long i, const Pairs<NV>& p) {
+#pragma unroll
+      for (int k = 0; k < NV; k++) { st2(Z.p[k], i, c.c[k] * p.v[k]); }
+    });
   TAIL(n)
   {
 #pragma unroll
@@ -653,8 +738,7 @@ __global__ void __launch_bounds__(NV_BLOCK) k_scalearray(CoefPack c, PtrPack X, 
 // flag/min reductions for InvTest, ConstrMask, MinQuotient
 __global__ void __launch_bounds__(NV_BLOCK) k_invtest(const double* __restrict__ x,
                                                       double* __restrict__ z, long n,
-                                                      double* partials, unsigned int* ticket,
-                                                      double* out)
+                                                      double* partials)
 {
   double acc[1] = {0.0}; // max of "zero found" flags
   for (long i = (long)blockIdx.x * NV_BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * NV_BLOCK)
@@ -663,14 +747,13 @@ __global__ void __launch_bounds__(NV_BLOCK) k_invtest(const double* __restrict__
     if (xv == 0.0) { acc[0] = 1.0; }
     else { z[i] = 1.0 / xv; }
   }
-  grid_finish<RMAX, 1>(acc, partials, ticket, out, 0.0);
+  grid_finish<RMAX, 1>(acc, partials);
 }
 
 __global__ void __launch_bounds__(NV_BLOCK) k_constrmask(const double* __restrict__ c,
                                                          const double* __restrict__ x,
                                                          double* __restrict__ m, long n,
-                                                         double* partials, unsigned int* ticket,
-                                                         double* out)
+                                                         double* partials)
 {
   double acc[1] = {0.0};
   for (long i = (long)blockIdx.x * NV_BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * NV_BLOCK)
@@ -684,13 +767,12 @@ __global__ void __launch_bounds__(NV_BLOCK) k_constrmask(const double* __restric
     }
     m[i] = mi;
   }
-  grid_finish<RMAX, 1>(acc, partials, ticket, out, 0.0);
+  grid_finish<RMAX, 1>(acc, partials);
 }
 
 __global__ void __launch_bounds__(NV_BLOCK) k_minquotient(const double* __restrict__ num,
                                                           const double* __restrict__ den, long n,
-                                                          double* partials, unsigned int* ticket,
-                                                          double* out)
+                                                          double* partials)
 {
   double acc[1] = {DBL_MAX};
   for (long i = (long)blockIdx.x * NV_BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * NV_BLOCK)
@@ -702,7 +784,7 @@ __global__ void __launch_bounds__(NV_BLOCK) k_minquotient(const double* __restri
       acc[0]   = (q < acc[0]) ? q : acc[0];
     }
   }
-  grid_finish<RMIN, 1>(acc, partials, ticket, out, DBL_MAX);
+  grid_finish<RMIN, 1>(acc, partials);
 }
 
 // ----------------------------------------------------------------------------
@@ -750,7 +832,7 @@ static void mi_linearsum(sunrealtype a, N_Vector x, sunrealtype b, N_Vector y, N
   long n = (long)NVLEN(z);
   if (n == 0) { return; }
   Mi355Exec* ex = EXEC(z);
-  int g         = stream_grid(ex, n >> 1);
+  int g         = stream_grid(ex, n >> 1, unroll_for(2));
   sunamd_prof_begin(SUNAMD_K_LINSUM, ex->stream);
   if (a == b)
   {
@@ -781,7 +863,7 @@ static void mi_const(sunrealtype c, N_Vector z)
   long n = (long)NVLEN(z);
   if (n == 0) { return; }
   Mi355Exec* ex = EXEC(z);
-  k_const<<<stream_grid(ex, n >> 1), NV_BLOCK, 0, ex->stream>>>(c, DDATA(z), n);
+  k_const<<<stream_grid(ex, n >> 1, 1), NV_BLOCK, 0, ex->stream>>>(c, DDATA(z), n);
   LAUNCHCHK(z);
 }
 
@@ -791,7 +873,8 @@ static void mi_ew(N_Vector x, N_Vector y, double c, N_Vector z)
   long n = (long)NVLEN(z);
   if (n == 0) { return; }
   Mi355Exec* ex = EXEC(z);
-  k_ew<OP><<<stream_grid(ex, n >> 1), NV_BLOCK, 0, ex->stream>>>(
+  const int nl = (OP == EW_PROD || OP == EW_DIV) ? 2 : 1;
+  k_ew<OP><<<stream_grid(ex, n >> 1, unroll_for(nl)), NV_BLOCK, 0, ex->stream>>>(
     DDATA(x), y ? DDATA(y) : DDATA(x), c, DDATA(z), n);
   LAUNCHCHK(z);
 }
@@ -824,7 +907,7 @@ static double mi_reduce(N_Vector x, N_Vector y, N_Vector id, double identity)
   long n        = (long)NVLEN(x);
   Mi355Exec* ex = EXEC(x);
   if (n == 0) { return identity; }
-  int g = red_grid(ex, n >> 1);
+  int g = red_grid(ex, n >> 1, unroll_for(red_loads<K>()));
   const int fam = (K == RK_WSQR) ? SUNAMD_K_WRMS : -1;
   // min / max are exact in any order; sums follow the vector's reduction order
   const bool seq = (RedTraits<K>::op == RSUM) && CONTENT(x)->red_order == SUNAMD_REDUCE_SEQUENTIAL;
@@ -838,7 +921,7 @@ static double mi_reduce(N_Vector x, N_Vector y, N_Vector id, double identity)
   {
     k_reduce<K><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), y ? DDATA(y) : DDATA(x),
                                                 id ? DDATA(id) : DDATA(x), n, identity,
-                                                ex->d_partials, ex->d_ticket, ex->h_result);
+                                                ex->d_partials);
     k_finish<RedTraits<K>::op, 1><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g,
                                                                  ex->h_result, identity);
   }
@@ -873,9 +956,8 @@ static sunbooleantype mi_invtest(N_Vector x, N_Vector z)
   long n        = (long)NVLEN(x);
   Mi355Exec* ex = EXEC(x);
   if (n == 0) { return SUNTRUE; }
-  const int g = red_grid(ex, n >> 1);
-  k_invtest<<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), DDATA(z), n, ex->d_partials,
-                                            ex->d_ticket, ex->h_result);
+  const int g = red_grid(ex, n >> 1, 4);
+  k_invtest<<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), DDATA(z), n, ex->d_partials);
   k_finish<RMAX, 1><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g, ex->h_result, 0.0);
   LAUNCHCHK(x);
   HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
@@ -887,9 +969,8 @@ static sunbooleantype mi_constrmask(N_Vector c, N_Vector x, N_Vector m)
   long n        = (long)NVLEN(x);
   Mi355Exec* ex = EXEC(x);
   if (n == 0) { return SUNTRUE; }
-  const int g = red_grid(ex, n >> 1);
-  k_constrmask<<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(c), DDATA(x), DDATA(m), n, ex->d_partials,
-                                               ex->d_ticket, ex->h_result);
+  const int g = red_grid(ex, n >> 1, 4);
+  k_constrmask<<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(c), DDATA(x), DDATA(m), n, ex->d_partials);
   k_finish<RMAX, 1><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g, ex->h_result, 0.0);
   LAUNCHCHK(x);
   HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
@@ -901,9 +982,8 @@ static sunrealtype mi_minquotient(N_Vector num, N_Vector den)
   long n        = (long)NVLEN(num);
   Mi355Exec* ex = EXEC(num);
   if (n == 0) { return SUN_BIG_REAL; }
-  const int g = red_grid(ex, n >> 1);
-  k_minquotient<<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(num), DDATA(den), n, ex->d_partials,
-                                                ex->d_ticket, ex->h_result);
+  const int g = red_grid(ex, n >> 1, 4);
+  k_minquotient<<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(num), DDATA(den), n, ex->d_partials);
   k_finish<RMIN, 1><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g, ex->h_result, DBL_MAX);
   LAUNCHCHK(num);
   HIPCHK(num->sunctx, hipStreamSynchronize(ex->stream));
@@ -946,15 +1026,15 @@ static SUNErrCode mi_dotprodmulti(int nvec, N_Vector x, N_Vector* Y, sunrealtype
     int nv = nvec - base < 8 ? nvec - base : 8;
     PtrPack P;
     for (int k = 0; k < nv; k++) { P.p[k] = DDATA(Y[base + k]); }
-    int g        = red_grid(ex, n >> 1);
     double* slot = ex->h_result + (base - group);
+    int g        = 0;
     sunamd_prof_begin(SUNAMD_K_DOTMULTI, ex->stream);
     switch (nv)
     {
 #define DM_CASE(NVV)                                                                       \
   case NVV:                                                                                \
-    k_dotmulti<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), P, n, ex->d_partials,        \
-                                                    ex->d_ticket, slot);                   \
+    g = red_grid(ex, n >> 1, unroll_for(NVV + 1));                                         \
+    k_dotmulti<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(x), P, n, ex->d_partials);       \
     k_finish<RSUM, NVV><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g, slot, 0.0);     \
     break;
       NV_CASES(DM_CASE)
@@ -979,7 +1059,6 @@ static SUNErrCode mi_lincomb(int nvec, sunrealtype* c, N_Vector* X, N_Vector z)
   long n        = (long)NVLEN(z);
   Mi355Exec* ex = EXEC(z);
   if (n == 0) { return SUN_SUCCESS; }
-  int g = stream_grid(ex, n >> 1);
   // first chunk writes z, later chunks accumulate into z (same rounding
   // order as the sequential loop)
   int done = 0;
@@ -1011,7 +1090,10 @@ static SUNErrCode mi_lincomb(int nvec, sunrealtype* c, N_Vector* X, N_Vector z)
     switch (nv)
     {
 #define LC_CASE(NVV)                                                                    \
-  case NVV: k_lincomb<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(C, P, DDATA(z), n); break;
+  case NVV:                                                                             \
+    k_lincomb<NVV><<<stream_grid(ex, n >> 1, unroll_for(NVV)), NV_BLOCK, 0, ex->stream>>>( \
+      C, P, DDATA(z), n);                                                                 \
+    break;
       NV_CASES(LC_CASE)
 #undef LC_CASE
     }
@@ -1028,7 +1110,6 @@ static SUNErrCode mi_scaleaddmulti(int nvec, sunrealtype* a, N_Vector x, N_Vecto
   long n        = (long)NVLEN(x);
   Mi355Exec* ex = EXEC(x);
   if (n == 0) { return SUN_SUCCESS; }
-  int g = stream_grid(ex, n >> 1);
   for (int base = 0; base < nvec; base += 8)
   {
     int nv = nvec - base < 8 ? nvec - base : 8;
@@ -1044,7 +1125,10 @@ static SUNErrCode mi_scaleaddmulti(int nvec, sunrealtype* a, N_Vector x, N_Vecto
     switch (nv)
     {
 #define SAM_CASE(NVV)                                                                        \
-  case NVV: k_scaleaddmulti<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(C, DDATA(x), P, W, n); break;
+  case NVV:                                                                                  \
+    k_scaleaddmulti<NVV><<<stream_grid(ex, n >> 1, unroll_for(NVV + 1)), NV_BLOCK, 0,           \
+                           ex->stream>>>(C, DDATA(x), P, W, n);                                \
+    break;
       NV_CASES(SAM_CASE)
 #undef SAM_CASE
     }
@@ -1059,7 +1143,6 @@ static SUNErrCode mi_scalevectorarray(int nvec, sunrealtype* c, N_Vector* X, N_V
   long n        = (long)NVLEN(Z[0]);
   Mi355Exec* ex = EXEC(Z[0]);
   if (n == 0) { return SUN_SUCCESS; }
-  int g = stream_grid(ex, n >> 1);
   for (int base = 0; base < nvec; base += 8)
   {
     int nv = nvec - base < 8 ? nvec - base : 8;
@@ -1075,7 +1158,10 @@ static SUNErrCode mi_scalevectorarray(int nvec, sunrealtype* c, N_Vector* X, N_V
     switch (nv)
     {
 #define SVA_CASE(NVV)                                                                  \
-  case NVV: k_scalearray<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(C, P, W, n); break;
+  case NVV:                                                                            \
+    k_scalearray<NVV><<<stream_grid(ex, n >> 1, unroll_for(NVV)), NV_BLOCK, 0,            \
+                        ex->stream>>>(C, P, W, n);                                       \
+    break;
       NV_CASES(SVA_CASE)
 #undef SVA_CASE
     }
@@ -1478,23 +1564,30 @@ extern "C" long int N_VGetNumLaunches_Mi355(N_Vector v) { return EXEC(v)->launch
 template <int NV>
 __global__ void __launch_bounds__(NV_BLOCK) k_gs_update(double* vk, PtrPack V,
                                                         const double* __restrict__ d, long n,
-                                                        int last, double* partials,
-                                                        unsigned int* ticket, double* out)
+                                                        int last, double* partials)
 {
   double nrm[1] = {0.0};
   double c[NV];
 #pragma unroll
   for (int j = 0; j < NV; j++) { c[j] = -d[j]; }
   const long np = n >> 1;
-  GRID_LOOP(np)
-  {
-    dbl2 z = ld2(vk, i);
+  chunk_pairs<unroll_for(NV + 1)>(
+    np,
+    [&](long i) {
+      Pairs<NV + 1> p;
+      p.v[0] = ld2(vk, i);
 #pragma unroll
-    for (int j = 0; j < NV; j++) { z = z + c[j] * ld2(V.p[j], i); }
-    st2(vk, i, z);
-    nrm[0] += z.x * z.x;
-    nrm[0] += z.y * z.y;
-  }
+      for (int j = 0; j < NV; j++) { p.v[j + 1] = ld2(V.p[j], i); }
+      return p;
+    },
+    [&](long i, const Pairs<NV + 1>& p) {
+      dbl2 z = p.v[0];
+#pragma unroll
+      for (int j = 0; j < NV; j++) { z = z + c[j] * p.v[j + 1]; }
+      st2(vk, i, z);
+      nrm[0] += z.x * z.x;
+      nrm[0] += z.y * z.y;
+    });
   TAIL(n)
   {
     double z = vk[n - 1];
@@ -1503,7 +1596,7 @@ __global__ void __launch_bounds__(NV_BLOCK) k_gs_update(double* vk, PtrPack V,
     vk[n - 1] = z;
     nrm[0] += z * z;
   }
-  if (last) { grid_finish<RSUM, 1>(nrm, partials, ticket, out, 0.0); }
+  if (last) { grid_finish<RSUM, 1>(nrm, partials); }
 }
 
 static bool gs_buffers(Mi355Exec* ex)
@@ -1547,13 +1640,13 @@ extern "C" SUNErrCode SUNAMD_ClassicalGS_Mi355(N_Vector* v, sunrealtype** h, int
     int cnt = nv + 1 - base < 8 ? nv + 1 - base : 8;
     PtrPack P;
     for (int j = 0; j < cnt; j++) { P.p[j] = DDATA(v[i0 + base + j]); }
-    int g = red_grid(ex, n >> 1);
+    int g = 0;
     switch (cnt)
     {
 #define GD_CASE(NVV)                                                                        \
   case NVV:                                                                                 \
-    k_dotmulti<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(vk), P, n, ex->d_partials,        \
-                                                    ex->d_ticket, ex->d_gs + base);         \
+    g = red_grid(ex, n >> 1, unroll_for(NVV + 1));                                          \
+    k_dotmulti<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(vk), P, n, ex->d_partials);       \
     k_finish<RSUM, NVV><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g, ex->d_gs + base,  \
                                                         0.0);                               \
     break;
@@ -1571,14 +1664,14 @@ extern "C" SUNErrCode SUNAMD_ClassicalGS_Mi355(N_Vector* v, sunrealtype** h, int
     int last = (base + cnt == nv) ? 1 : 0;
     PtrPack P;
     for (int j = 0; j < cnt; j++) { P.p[j] = DDATA(v[base + j]); }
-    int g = red_grid(ex, n >> 1);
+    int g = 0;
     switch (cnt)
     {
 #define GU_CASE(NVV)                                                                         \
   case NVV:                                                                                  \
+    g = red_grid(ex, n >> 1, unroll_for(NVV + 1));                                           \
     k_gs_update<NVV><<<g, NV_BLOCK, 0, ex->stream>>>(DDATA(vk), P, ex->d_gs + base, n, last, \
-                                                     ex->d_partials, ex->d_ticket,           \
-                                                     ex->d_gs + nv + 1);                     \
+                                                     ex->d_partials);                        \
     if (last)                                                                                \
     {                                                                                        \
       k_finish<RSUM, 1><<<1, NV_BLOCK, 0, ex->stream>>>(ex->d_partials, g, ex->d_gs + nv + 1, \

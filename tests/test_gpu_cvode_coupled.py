@@ -137,7 +137,12 @@ def test_coupled_fp64_tree_reductions(libs, m, R, ncells, seed):
 def test_coupled_tiled_matches_base(libs):
     """2^15 cells tiled from 128 base cells: every tile integrates like the
     base set (WRMS norms of the tiled vector equal the base set's up to
-    rounding), checked against the oracle run of the 128 base cells."""
+    rounding), and its global error against a reference solution of the 128
+    base cells at 1000x tighter tolerances stays at the level of the
+    tolerance.  The bound is 20 rtol-weighted units: the oracle's own error on
+    tilings of the same cells ranges from 0.85 to 5.7 units (2 to 256 tiles,
+    sequential sums; the rounding of the norms alone changes its step
+    sequence), so comparing one run's error with another's is not stable."""
     core = libs[0]
     hip, orc = libs[1], libs[2]
     m, R = 8, 16
@@ -151,13 +156,16 @@ def test_coupled_tiled_matches_base(libs):
     y_cpu = np.zeros(128 * m)
     ost = np.zeros(16)
     assert orc.oracle_massaction_solve_coupled(base, 1e-6, 1e-10, 1.0, dp(y_cpu), dp(ost)) == 0
+    y_ref = np.zeros(128 * m)
+    assert orc.oracle_massaction_solve_coupled(base, 1e-9, 1e-13, 1.0, dp(y_ref), None) == 0
     core.SUNAMD_MassActionFree(big)
     core.SUNAMD_MassActionFree(base)
     tiles = y_gpu.reshape(-1, 128 * m)
     # every tile is bit-identical (cells never interact except through norms)
     assert all(np.array_equal(tiles[0], t) for t in tiles[1:])
     assert abs(st.nst - ost[0]) <= 0.1 * ost[0]
-    assert _wrms_err(tiles[0], y_cpu) <= 3.0
+    e_gpu, e_cpu = _wrms_err(tiles[0], y_ref), _wrms_err(y_cpu, y_ref)
+    assert e_cpu <= 20.0 and e_gpu <= 20.0, (e_gpu, e_cpu)
 
 
 def test_coupled_fp32_lu_accuracy(libs):
