@@ -131,6 +131,7 @@ struct Sys
   // Newton-matrix factor row held by this lane, and its physical row
   T lu[M];
   int pos;
+  float rinv; // FP32 factors: 1 / this lane's diagonal entry
   // integrator scalars (uniform across the group)
   double tn, h, hscale, hprime, eta, hu, h0u;
   double rl1, gamma, gammap, gamrat, crate, delp, acnrm;
@@ -309,6 +310,18 @@ __device__ __forceinline__ double g_lane_value(double b, int k)
   }
 }
 
+template <int G>
+__device__ __forceinline__ float g_lane_value(float b, int k)
+{
+  if constexpr (G == 64) { return readlane_d(b, k); }
+  else
+  {
+    const float b0 = readlane_d(b, k);
+    const float b1 = readlane_d(b, 32 + k);
+    return (__lane_id() < 32) ? b0 : b1;
+  }
+}
+
 // ---- pivot search: largest |a_ik| over the group, ties to the smallest row
 // (the dense GETRF's first maximum), as a reduction on (value, row) pairs:
 // DPP exchanges inside rows of 16 lanes, lane reads across rows --------------
@@ -333,14 +346,14 @@ __device__ __forceinline__ float dpp_v(float x)
   return __int_as_float(dpp_i<CTRL>(__float_as_int(x)));
 }
 
+// (ov, op) replaces (v, p) when ov is larger, or equal with a smaller row;
+// evaluated without short-circuit so it compiles to compares and selects
 template <typename V>
 __device__ __forceinline__ void amax_take(V& v, int& p, V ov, int op)
 {
-  if (ov > v || (ov == v && op < p))
-  {
-    v = ov;
-    p = op;
-  }
+  const bool take = (ov > v) | ((ov == v) & (op < p));
+  v               = take ? ov : v;
+  p               = take ? op : p;
 }
 
 template <int CTRL, typename V>
@@ -376,6 +389,12 @@ __device__ __forceinline__ void g_argmax(V& v, int& p)
   }
 }
 
+// Single-precision factors take the fast forms: fused multiply-adds, no zero
+// tests, and (for groups of 32 or 64 lanes) a single-precision solve; the
+// double-precision factorisation keeps the dense GETRF/GETRS operation order.
+template <typename T>
+inline constexpr bool kFastFactor = (sizeof(T) == 4);
+
 // ---- LU of the group's matrix (rows in lanes, implicit pivoting) ---------
 template <int M, typename T>
 __device__ __forceinline__ int g_getrf(const Grp<M, T>& g, Sys<M, T>& S)
@@ -400,18 +419,30 @@ __device__ __forceinline__ int g_getrf(const Grp<M, T>& g, Sys<M, T>& S)
       const int pl  = __builtin_amdgcn_readfirstlane(__ffsll((long long)holder) - 1) & 63;
       const bool up = g.valid && pos > k && fail == 0;
       const T pk    = readlane_d(S.lu[k], pl);
-      T l           = (T)0;
-      if (up)
+      if constexpr (kFastFactor<T>)
       {
-        const T mult = (T)1 / pk;
-        l            = S.lu[k] * mult;
-        S.lu[k]      = l;
-      }
+        // single precision: multiplier 0 on the rows not updated, fused
+        // multiply-add, no zero test
+        const T l = up ? S.lu[k] * ((T)1 / pk) : (T)0;
+        if (up) { S.lu[k] = l; }
 #pragma unroll
-      for (int j = k + 1; j < M; j++)
+        for (int j = k + 1; j < M; j++) { S.lu[j] = __builtin_fmaf(-readlane_d(S.lu[j], pl), l, S.lu[j]); }
+      }
+      else
       {
-        const T pj = readlane_d(S.lu[j], pl);
-        if (up && pj != (T)0) { S.lu[j] = S.lu[j] - pj * l; }
+        T l = (T)0;
+        if (up)
+        {
+          const T mult = (T)1 / pk;
+          l            = S.lu[k] * mult;
+          S.lu[k]      = l;
+        }
+#pragma unroll
+        for (int j = k + 1; j < M; j++)
+        {
+          const T pj = readlane_d(S.lu[j], pl);
+          if (up && pj != (T)0) { S.lu[j] = S.lu[j] - pj * l; }
+        }
       }
     }
     else
@@ -423,7 +454,15 @@ __device__ __forceinline__ int g_getrf(const Grp<M, T>& g, Sys<M, T>& S)
         for (int j = k; j < M; j++) { g.prow[j] = S.lu[j]; }
       }
       gsync();
-      if (g.valid && pos > k && fail == 0)
+      const bool up = g.valid && pos > k && fail == 0;
+      if constexpr (kFastFactor<T>)
+      {
+        const T l = up ? S.lu[k] * ((T)1 / g.prow[k]) : (T)0;
+        if (up) { S.lu[k] = l; }
+#pragma unroll
+        for (int j = k + 1; j < M; j++) { S.lu[j] = __builtin_fmaf(-g.prow[j], l, S.lu[j]); }
+      }
+      else if (up)
       {
         const T mult = (T)1 / g.prow[k];
         const T l    = S.lu[k] * mult;
@@ -449,6 +488,22 @@ __device__ __forceinline__ int g_getrf(const Grp<M, T>& g, Sys<M, T>& S)
 #pragma unroll
     for (int j = 0; j < M; j++) { S.lu[j] = __shfl(S.lu[j], src, 64); }
     S.pos = g.i;
+    if constexpr (kFastFactor<T>)
+    {
+      // single precision: keep 1/u_ii per lane and store the strict upper
+      // part column-scaled, u_ij / u_jj, so the backward solve's chain is a
+      // lane read and a fused multiply-add per column
+      float d = S.lu[0];
+#pragma unroll
+      for (int j = 1; j < M; j++) { d = (j == g.i) ? S.lu[j] : d; }
+      S.rinv = 1.0f / d;
+#pragma unroll
+      for (int j = 1; j < M; j++)
+      {
+        const float rj = g_lane_value<G>(S.rinv, j);
+        S.lu[j]        = (j > g.i) ? S.lu[j] * rj : S.lu[j];
+      }
+    }
   }
   return fail;
 }
@@ -467,6 +522,25 @@ __device__ __forceinline__ double g_getrs(const Grp<M, T>& g, const Sys<M, T>& S
     // element of row j is held by the lane the row came from), then read
     // row k's element from lane base+k; x_i ends in lane base+i
     b = __shfl(b, g.base + g.inv[g.valid ? g.i : 0], 64);
+    if constexpr (kFastFactor<T>)
+    {
+      // single-precision solve: multipliers selected off the dependency
+      // chain (0 on lanes a column does not update), U column-scaled
+      float x = (float)b;
+#pragma unroll
+      for (int k = 0; k < M - 1; k++)
+      {
+        const float lk = (g.i > k) ? S.lu[k] : 0.0f;
+        x              = __builtin_fmaf(-lk, g_lane_value<G>(x, k), x);
+      }
+#pragma unroll
+      for (int k = M - 1; k > 0; k--)
+      {
+        const float uk = (g.i < k) ? S.lu[k] : 0.0f;
+        x              = __builtin_fmaf(-uk, g_lane_value<G>(x, k), x);
+      }
+      return (double)(x * S.rinv);
+    }
 #pragma unroll
     for (int k = 0; k < M - 1; k++)
     {
