@@ -724,49 +724,82 @@ __device__ __forceinline__ void g_restore(Sys<M, T>& S, double saved_t)
   }
 }
 
+// element j of a register array indexed at run time (0 <= j <= N), chosen by
+// value (see opaque) so the array stays in registers
+template <int N>
+__device__ __forceinline__ double rsel(const double (&a)[N + 1], int j)
+{
+  double v = opaque(a[0]);
+#pragma unroll
+  for (int k = 1; k <= N; k++)
+  {
+    const double ak = opaque(a[k]);
+    v               = (j == k) ? ak : v;
+  }
+  return v;
+}
+
+// cvSetBDF + cvSetTqBDF + the gamma update of cvSet.  The step history and
+// the coefficients are computed in registers (loops unrolled to QMAX and
+// predicated on q, the same operations in the same order) and only the
+// results go to the group's LDS slots.
 template <int M, typename T>
 __device__ __forceinline__ void g_set(const Grp<M, T>& g, Sys<M, T>& S, long nst)
 {
-  double* l  = g.sc->l;
-  double* tq = g.sc->tq;
+  double tau[QMAX + 1];
+#pragma unroll
+  for (int k = 0; k <= QMAX; k++) { tau[k] = g.sc->tau[k]; }
+  double l[QMAX + 1];
   double alpha0, alpha0_hat, xi_inv, xistar_inv, hsum;
   l[0] = l[1] = xi_inv = xistar_inv = 1.0;
-  for (int i = 2; i <= S.q; i++) { l[i] = 0.0; }
+#pragma unroll
+  for (int i = 2; i <= QMAX; i++) { l[i] = 0.0; }
   alpha0 = alpha0_hat = -1.0;
   hsum                = S.h;
   if (S.q > 1)
   {
-    for (int j = 2; j < S.q; j++)
+#pragma unroll
+    for (int j = 2; j < QMAX; j++)
     {
-      hsum += g.sc->tau[j - 1];
-      xi_inv = S.h / hsum;
-      alpha0 -= 1.0 / j;
-      for (int i = j; i >= 1; i--) { l[i] += l[i - 1] * xi_inv; }
+      if (j < S.q)
+      {
+        hsum += tau[j - 1];
+        xi_inv = S.h / hsum;
+        alpha0 -= 1.0 / j;
+#pragma unroll
+        for (int i = j; i >= 1; i--) { l[i] += l[i - 1] * xi_inv; }
+      }
     }
     alpha0 -= 1.0 / S.q;
     xistar_inv = -l[1] - alpha0;
-    hsum += g.sc->tau[S.q - 1];
+    hsum += rsel<QMAX>(tau, S.q - 1);
     xi_inv     = S.h / hsum;
     alpha0_hat = -l[1] - xi_inv;
-    for (int i = S.q; i >= 1; i--) { l[i] += l[i - 1] * xistar_inv; }
+#pragma unroll
+    for (int i = QMAX; i >= 1; i--)
+    {
+      if (i <= S.q) { l[i] += l[i - 1] * xistar_inv; }
+    }
   }
   // cvSetTqBDF
+  double* tq = g.sc->tq;
+  const double lq = rsel<QMAX>(l, S.q);
   double A1 = 1.0 - alpha0_hat + alpha0;
   double A2 = 1.0 + S.q * A1;
   tq[2]     = fabs(A1 / (alpha0 * A2));
-  tq[5]     = fabs(A2 * xistar_inv / (l[S.q] * xi_inv));
+  tq[5]     = fabs(A2 * xistar_inv / (lq * xi_inv));
   if (S.qwait == 1)
   {
     if (S.q > 1)
     {
-      double C     = xistar_inv / l[S.q];
+      double C     = xistar_inv / lq;
       double A3    = alpha0 + 1.0 / S.q;
       double A4    = alpha0_hat + xi_inv;
       double Cpinv = (1.0 - A4 + A3) / A3;
       tq[1]        = fabs(C * Cpinv);
     }
     else { tq[1] = 1.0; }
-    hsum += g.sc->tau[S.q];
+    hsum += rsel<QMAX>(tau, S.q);
     xi_inv        = S.h / hsum;
     double A5     = alpha0 - (1.0 / (S.q + 1));
     double A6     = alpha0_hat - xi_inv;
@@ -774,6 +807,8 @@ __device__ __forceinline__ void g_set(const Grp<M, T>& g, Sys<M, T>& S, long nst
     tq[3]         = fabs(Cppinv / (xi_inv * (S.q + 2) * A5));
   }
   tq[4] = CORTES / tq[2];
+#pragma unroll
+  for (int i = 0; i <= QMAX; i++) { g.sc->l[i] = l[i]; }
   // cvSet
   S.rl1   = 1.0 / l[1];
   S.gamma = S.h * S.rl1;
@@ -1132,9 +1167,17 @@ __device__ __forceinline__ void g_complete_step(const Grp<M, T>& g, Sys<M, T>& S
   S.nst++;
   S.hu       = S.h;
   S.qu       = S.q;
+  // step-size history shift from register copies of the old values
   double* tau = g.sc->tau;
-  for (int i = S.q; i >= 2; i--) { tau[i] = tau[i - 1]; }
-  if ((S.q == 1) && (S.nst > 1)) { tau[2] = tau[1]; }
+  double t[QMAX + 1];
+#pragma unroll
+  for (int k = 0; k <= QMAX; k++) { t[k] = tau[k]; }
+#pragma unroll
+  for (int i = QMAX; i >= 2; i--)
+  {
+    if (i <= S.q) { tau[i] = t[i - 1]; }
+  }
+  if ((S.q == 1) && (S.nst > 1)) { tau[2] = t[1]; }
   tau[1] = S.h;
   const double* l = g.sc->l;
 #pragma unroll
