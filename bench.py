@@ -562,7 +562,9 @@ def bench_batched(args):
     dom = int(np.argmax(kms))
     dflop = float(fl[sb == dom].sum())
     achieved = dflop / (kms[dom] * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "kernel": "k_batched_bdf<%d, float, %d>" % (bm[dom], w32[dom]),
+    learn_dom = bool(mask & (1 << dom)) or w32[dom] >= 4
+    roofline = {"bound": "mfma", "kernel": "k_batched_bdf<%d, float, %d, %s>" % (
+                    bm[dom], w32[dom], "true" if learn_dom else "false"),
                 "achieved": round(achieved, 4), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFS, 5), "traffic": None,
                 "flops_per_launch": dflop, "launch_ms": round(float(kms[dom]), 3),

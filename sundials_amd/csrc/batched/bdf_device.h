@@ -156,6 +156,7 @@ struct Grp
   int* inv;    // G ints of LDS
   GroupScalars* sc;
   float* mlp; // MLP_LDS_FLOATS of LDS (xs[144] h1[128] h2[128]) or nullptr
+  bool learn; // kernel built with the learned-start / training-log paths
   const BucketArgs* a;
   // the bucket's network, staged in LDS once per workgroup
   const int* idx;  // 4R reaction indices
@@ -1061,7 +1062,7 @@ __device__ __forceinline__ int g_nls(const Grp<M, T>& g, Sys<M, T>& S, int nflag
                    (S.nst >= S.nstlp + MSBP) || (fabs(S.gamrat - 1.0) > DGMAX_LSETUP);
   S.acor        = 0.0;
   bool have_res = false, learned = false;
-  if (g.a->mlp_w != nullptr && nflag == FIRST_CALL && S.nst > 0)
+  if (g.learn && g.a->mlp_w != nullptr && nflag == FIRST_CALL && S.nst > 0)
   {
     // learned start, kept only if its residual is smaller than the zero
     // start's (the plain predictor is the fallback)
@@ -1090,7 +1091,7 @@ __device__ __forceinline__ int g_nls(const Grp<M, T>& g, Sys<M, T>& S, int nflag
   }
   int flag = g_newton(g, S, callSetup, have_res, learned);
   if (flag != CV_SUCCESS) { return flag; }
-  if (g.a->log_x != nullptr && nflag == FIRST_CALL && (S.nst % g.a->log_stride) == 0)
+  if (g.learn && g.a->log_x != nullptr && nflag == FIRST_CALL && (S.nst % g.a->log_stride) == 0)
   {
     g_log_pair(g, S);
   }
@@ -1547,8 +1548,11 @@ __device__ __forceinline__ bool g_advance(const Grp<M, T>& g, Sys<M, T>& S, doub
 // the next one from the queue between steps, so the wavefront's groups stay
 // busy while their neighbours' systems need more steps.
 // W: minimum wavefronts per SIMD the register allocation must allow (1 = no
-// limit; 2 and 4 cap it at 256 and 128 registers, spilling the rest)
-template <int M, typename T, int W>
+// limit; 2 and 4 cap it at 256 and 128 registers, spilling the rest).
+// LEARN: the learned start and the training log are compiled in; without them
+// the kernel needs fewer registers (no spills at 2 wavefronts per SIMD for
+// most sizes).
+template <int M, typename T, int W, bool LEARN>
 __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
 {
   constexpr int G   = pow2ceil(M);
@@ -1587,7 +1591,9 @@ __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
   g.prow  = s_prow + g.base;
   g.inv   = s_inv + g.base;
   g.sc    = &s_sc[grp];
-  g.mlp   = (A.mlp_w != nullptr || A.log_x != nullptr) ? s_mlp + grp * MLP_LDS_FLOATS : nullptr;
+  g.mlp   = (LEARN && (A.mlp_w != nullptr || A.log_x != nullptr)) ? s_mlp + grp * MLP_LDS_FLOATS
+                                                                   : nullptr;
+  g.learn = LEARN;
   g.a     = &A;
   g.idx   = s_idx;
   g.soff  = s_soff;
@@ -1621,10 +1627,19 @@ template <int M, typename T>
 void launch(const BucketArgs& A, int grid, hipStream_t stream, int waves)
 {
   constexpr int GPW = 64 / pow2ceil(M);
-  const size_t lds  = bucket_lds_bytes(M, A.R, GPW, A.mlp_w != nullptr || A.log_x != nullptr);
-  if (waves >= 4) { k_batched_bdf<M, T, 4><<<grid, 64, lds, stream>>>(A); }
-  else if (waves == 2) { k_batched_bdf<M, T, 2><<<grid, 64, lds, stream>>>(A); }
-  else { k_batched_bdf<M, T, 1><<<grid, 64, lds, stream>>>(A); }
+  const bool learn  = A.mlp_w != nullptr || A.log_x != nullptr;
+  const size_t lds  = bucket_lds_bytes(M, A.R, GPW, learn);
+  if (waves >= 4) { k_batched_bdf<M, T, 4, true><<<grid, 64, lds, stream>>>(A); }
+  else if (waves == 2)
+  {
+    if (learn) { k_batched_bdf<M, T, 2, true><<<grid, 64, lds, stream>>>(A); }
+    else { k_batched_bdf<M, T, 2, false><<<grid, 64, lds, stream>>>(A); }
+  }
+  else
+  {
+    if (learn) { k_batched_bdf<M, T, 1, true><<<grid, 64, lds, stream>>>(A); }
+    else { k_batched_bdf<M, T, 1, false><<<grid, 64, lds, stream>>>(A); }
+  }
 }
 
 } // namespace sunamd_bdf
