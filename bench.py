@@ -294,14 +294,16 @@ def train_predictor(args, core, hip):
     return M, info
 
 
-def autotune(core, hip, prec, weights=None):
-    """Per-bucket choices made on an independent calibration batch (8192
-    systems): the kernel's register budget (1, 2 or 4 wavefronts per SIMD,
+def autotune(core, hip, prec, args_systems, weights=None):
+    """Per-bucket choices made on an independent calibration batch of the
+    measured batch's size: the kernel's register budget (1, 2 or 4 wavefronts per SIMD,
     i.e. occupancy against spilling) and, given a trained network, whether
     the learned start pays off.  Each choice keeps the bucket's shorter
     kernel time."""
     nb = len(BATCH_M)
-    bcal = _batch(core, 8192, CALIB_SEED)
+    # full size: a bucket's occupancy choice only shows its benefit when the
+    # bucket fills the GPU as it does in the measured batch
+    bcal = _batch(core, args_systems, CALIB_SEED)
 
     def kernel_ms(waves, mask):
         B = hip.SUNAMD_BatchedBDFCreate(bcal, RTOL, ATOL, prec)
@@ -316,16 +318,25 @@ def autotune(core, hip, prec, weights=None):
         kms = np.zeros(nb)
         hip.SUNAMD_BatchedBDFGetKernelTimes(B, kms.ctypes.data_as(C.POINTER(C.c_double)))
         hip.SUNAMD_BatchedBDFFree(B)
-        return kms
+        return kms, wall.value
 
-    table = {w: kernel_ms([w] * nb, 0) for w in (1, 2, 4)}
+    runs = {w: kernel_ms([w] * nb, 0) for w in (1, 2, 4)}
+    table = {w: runs[w][0] for w in runs}
     best_w = [min((1, 2, 4), key=lambda w: table[w][k]) for k in range(nb)]
-    best = np.array([table[best_w[k]][k] for k in range(nb)])
+    # buckets share the GPU, so the per-bucket mix is kept only if the whole
+    # batch is faster with it than with the best uniform budget
+    mix_ms, mix_wall = kernel_ms(best_w, 0)
+    w_uni = min(runs, key=lambda w: runs[w][1])
+    if runs[w_uni][1] < mix_wall:
+        best_w, mix_ms = [w_uni] * nb, table[w_uni]
+    best = np.array(mix_ms)
     info = {"waves": best_w, "ms_by_waves": {str(w): [round(v, 2) for v in table[w]]
-                                            for w in table}}
+                                            for w in table},
+            "wall_ms_by_waves": {str(w): round(runs[w][1] * 1e3, 2) for w in runs},
+            "wall_ms_mix": round(mix_wall * 1e3, 2)}
     mask = 0
     if weights is not None:
-        lms = kernel_ms(best_w, (1 << nb) - 1)
+        lms, _ = kernel_ms(best_w, (1 << nb) - 1)
         for k in range(nb):
             if lms[k] < best[k]:
                 mask |= 1 << k
@@ -497,7 +508,7 @@ def bench_batched(args):
     # plain FP64 form (register budget tuned per bucket): warm up, then time one integration
     fixed = None if args.waves == "auto" else [int(args.waves)] * nb
     if fixed is None:
-        w64, _, tune64 = autotune(core, hip, LU_FP64)
+        w64, _, tune64 = autotune(core, hip, LU_FP64, args.systems)
     else:
         w64, tune64 = fixed, {"waves": fixed, "fixed": True}
     print("[bench] fp64 waves per bucket: %s" % w64, file=sys.stderr, flush=True)
@@ -518,7 +529,7 @@ def bench_batched(args):
         mlp, learned = train_predictor(args, core, hip)
         weights = hip.SUNAMD_MLPDeviceWeights(mlp)
     if fixed is None:
-        w32, mask, tune32 = autotune(core, hip, LU_FP32, weights)
+        w32, mask, tune32 = autotune(core, hip, LU_FP32, args.systems, weights)
     else:
         w32, mask, tune32 = fixed, 0, {"waves": fixed, "fixed": True}
     print("[bench] fp32 waves per bucket: %s, learned mask %d" % (w32, mask), file=sys.stderr,
