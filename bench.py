@@ -462,6 +462,110 @@ def bench_imex(args):
     }
 
 
+def bench_kinsol(args):
+    """KINSOL fixed-point iteration with Anderson acceleration (depth 5,
+    CGS2 QR updates) on the replicated analytic map (the IEEE-only variant)
+    over 2^21 cells x 3 unknowns, perturbed initial guesses, tolerance 1e-11;
+    a step is one KINSol call."""
+    import torch
+    from sundials_amd import _lib
+
+    hip = _lib.hip()
+    ncells = args.kin_cells
+    N = 3 * ncells
+
+    def opts(nc, seq=0):
+        o = _lib.KinOptions()
+        o.ncells, o.variant, o.perturb, o.tol, o.maxiter = nc, 1, 1, 1e-11, 100
+        o.m_aa, o.delay_aa, o.orth_aa = 5, 0, 2
+        o.damping_fp, o.damping_aa, o.damping_fn, o.depth_fn, o.sequential = 1.0, 1.0, 0, 0, seq
+        return o
+
+    PD = C.POINTER(C.c_double)
+
+    def run():
+        st = np.zeros(_lib.KIN_STATS)
+        rc = hip.SUNAMD_KinSolve_Mi355(C.byref(opts(ncells)), st.ctypes.data_as(PD), None)
+        if rc != 0:
+            print("[bench] kinsol solve failed rc=%d" % rc, file=sys.stderr)
+            sys.exit(1)
+        return {"nni": int(st[1]), "nfe": int(st[2]), "fnorm": st[3], "wall": st[4]}
+
+    for w in range(args.warmup):
+        st = run()
+        print("[bench] warmup %d: %.4f s, %d iterations" % (w, st["wall"], st["nni"]),
+              file=sys.stderr, flush=True)
+    # the timed region of each step is the KINSol call (its own synchronised
+    # wall clock); creating the problem and the solver's workspace is not
+    torch.cuda.synchronize()
+    iters, elapsed = 0, 0.0
+    for k in range(args.steps):
+        st = run()
+        iters += st["nni"]
+        elapsed += st["wall"]
+        print("[bench] step %d: %.4f s, %d iterations" % (k, st["wall"], st["nni"]),
+              file=sys.stderr, flush=True)
+    value = N * iters / elapsed
+
+    hip.SUNAMD_ProfilerReset()
+    hip.SUNAMD_ProfilerEnable(1)
+    run()
+    hip.SUNAMD_ProfilerEnable(0)
+    fams = {FAM_LINSUM: ("N_VLinearSum", 24.0 * N), FAM_SCALE: ("N_VScale", 16.0 * N)}
+    kern = {}
+    for f, (name, nbytes) in fams.items():
+        tot, cnt = C.c_double(), C.c_long()
+        hip.SUNAMD_ProfilerGet(f, C.byref(tot), C.byref(cnt))
+        if cnt.value:
+            kern[f] = {"name": name, "total_ms": tot.value, "launches": cnt.value,
+                       "avg_us": 1e3 * tot.value / cnt.value, "bytes_per_launch": nbytes}
+    dom = max(kern, key=lambda f: kern[f]["total_ms"])
+    dk = kern[dom]
+    achieved = dk["bytes_per_launch"] / (dk["avg_us"] * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "kernel": dk["name"], "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "bytes_per_launch": dk["bytes_per_launch"],
+                "avg_launch_us": round(dk["avg_us"], 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        orc = _lib.load_oracle()
+        nc = args.kin_cpu_cells
+        so = np.zeros(_lib.KIN_STATS)
+        t = time.perf_counter()
+        rc = orc.oracle_kin_solve(C.byref(opts(nc)), so.ctypes.data_as(PD), None)
+        wall = time.perf_counter() - t
+        if rc == 0:
+            cpu = {"value": 3 * nc * so[1] / wall, "unit": "unknown-iterations/s", "cores": 1,
+                   "kind": "port",
+                   "sample": "CPU oracle (serial vector, the same KINSOL restatement) on %d cells "
+                             "(%d iterations, %.2f s, %s)" % (nc, int(so[1]), wall, cpu_model())}
+    ms = 1e3 * elapsed / args.steps
+    return {
+        "metric": "unknown-iterations/s (KINSOL fixed point + Anderson acceleration)",
+        "value": value,
+        "unit": "unknown-iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "KINSol KIN_FP, Anderson depth 5 (CGS2), the analytic 3-equation "
+                               "map (IEEE-only variant) on %d independent cells, tol 1e-11"
+                               % ncells, "unknowns": N},
+        "solver": {"nni": st["nni"], "nfe": st["nfe"], "fnorm": st["fnorm"]},
+        "timed": "KINSol calls (problem set-up and workspace allocation excluded)",
+        "roofline": roofline,
+        "kernels": {kern[f]["name"]: {k: v for k, v in kern[f].items() if k != "name"}
+                    for f in kern},
+        "cpu_baseline": cpu,
+    }
+
+
 def bench_batched(args):
     import torch
     from sundials_amd import _lib
@@ -640,7 +744,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["batched", "coupled", "imex"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "coupled", "imex", "kinsol"],
+                    default="batched")
+    ap.add_argument("--kin-cells", type=int, default=1 << 21, help="kinsol: cells (3 unknowns)")
+    ap.add_argument("--kin-cpu-cells", type=int, default=1 << 21, help="kinsol: CPU-oracle cells")
     ap.add_argument("--grid", type=int, default=128, help="imex: cells per dimension")
     ap.add_argument("--tf", type=float, default=0.01, help="imex: final time")
     ap.add_argument("--cpu-grid", type=int, default=64, help="imex: CPU-oracle grid")
@@ -677,6 +784,8 @@ def main():
         line = bench_batched(args)
     elif args.workload == "imex":
         line = bench_imex(args)
+    elif args.workload == "kinsol":
+        line = bench_kinsol(args)
     else:
         line = bench_coupled(args)
     print(json.dumps(line))

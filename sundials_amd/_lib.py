@@ -95,6 +95,34 @@ class AdrOptions(C.Structure):
 
 
 ADR_STATS = 16
+
+
+class QRData(C.Structure):
+    """struct _SUNQRData (include/sunamd/sundials_iterative.h)."""
+    _fields_ = [("vtemp", C.c_void_p), ("vtemp2", C.c_void_p),
+                ("temp_array", C.POINTER(C.c_double))]
+
+
+class KinOptions(C.Structure):
+    """SUNAMD_KinOptions (include/sunamd/kin_fp.h)."""
+    _fields_ = [
+        ("ncells", C.c_long),
+        ("variant", C.c_int),
+        ("perturb", C.c_int),
+        ("tol", C.c_double),
+        ("maxiter", C.c_long),
+        ("m_aa", C.c_long),
+        ("delay_aa", C.c_long),
+        ("orth_aa", C.c_int),
+        ("damping_fp", C.c_double),
+        ("damping_aa", C.c_double),
+        ("damping_fn", C.c_int),
+        ("depth_fn", C.c_int),
+        ("sequential", C.c_int),
+    ]
+
+
+KIN_STATS = 5
 ADR_STAT_NAMES = ("nst", "attempts", "netf", "ncfn", "nfe", "nfi", "nni", "nnf", "nsetups",
                   "nli", "npe", "nps", "njtimes", "ncfl", "wall", "hlast")
 
@@ -203,6 +231,8 @@ def core():
     _sig(lib, "SUNLinSol_SPGMRSetMaxRestarts", C.c_int, [c_void_p, C.c_int])
     _sig(lib, "SUNLinSol_SPGMRSetClassicalGSFn", C.c_int, [c_void_p, c_void_p])
     _sig(lib, "SUNModifiedGS", C.c_int, [P(c_void_p), P(P(c_real)), C.c_int, C.c_int, P(c_real)])
+    for name in ("SUNQRAdd_MGS", "SUNQRAdd_ICWY", "SUNQRAdd_CGS2", "SUNQRAdd_DCGS2"):
+        _sig(lib, name, C.c_int, [P(c_void_p), P(c_real), c_void_p, C.c_int, C.c_int, P(QRData)])
     _sig(lib, "SUNClassicalGS", C.c_int, [P(c_void_p), P(P(c_real)), C.c_int, C.c_int, P(c_real),
                                           P(c_real), P(c_void_p)])
     _sig(lib, "SUNQRfact", C.c_int, [C.c_int, P(P(c_real)), P(c_real), C.c_int])
@@ -276,6 +306,7 @@ def hip():
     _sig(lib, "SUNAMD_TridiagPSetup_Mi355", C.c_int, [c_void_p])
     _sig(lib, "SUNAMD_TridiagPSolve_Mi355", C.c_int, [c_void_p, c_void_p, c_void_p, c_real, C.c_int])
     _sig(lib, "SUNAMD_AdrSolve_Mi355", C.c_int, [P(Adr), P(AdrOptions), P(c_real), P(c_real)])
+    _sig(lib, "SUNAMD_KinSolve_Mi355", C.c_int, [P(KinOptions), P(c_real), P(c_real)])
     _sig(lib, "SUNAMD_BatchedBDFCreate", c_void_p, [P(Batch), c_real, c_real, C.c_int])
     _sig(lib, "SUNAMD_BatchedBDFSetMaxNumSteps", C.c_int, [c_void_p, C.c_long])
     _sig(lib, "SUNAMD_BatchedBDFRun", C.c_int, [c_void_p, c_real, P(c_real)])
@@ -331,6 +362,7 @@ def load_oracle():
     _sig(lib, "oracle_tridiag_psetup", C.c_int, [c_void_p])
     _sig(lib, "oracle_tridiag_psolve", C.c_int, [c_void_p, c_void_p, c_void_p, c_real, C.c_int])
     _sig(lib, "oracle_adr_solve", C.c_int, [P(Adr), P(AdrOptions), P(c_real), P(c_real)])
+    _sig(lib, "oracle_kin_solve", C.c_int, [P(KinOptions), P(c_real), P(c_real)])
     _sig(lib, "oracle_ark_dahlquist", C.c_int, [c_real, c_real, c_real, c_real, c_real, c_real,
                                                 C.c_char_p, C.c_int, P(c_real), P(c_real)])
     _sig(lib, "oracle_batch_solve", C.c_int,

@@ -5,6 +5,8 @@
  *   SUNClassicalGS  src/sundials/sundials_iterative.c:116-170
  *   SUNQRfact       src/sundials/sundials_iterative.c:181-286
  *   SUNQRsol        src/sundials/sundials_iterative.c:297-329
+ *   SUNQRAdd_MGS / _ICWY / _CGS2 / _DCGS2
+ *                   src/sundials/sundials_iterative.c:333-606
  * The vector work goes through the N_Vector ops (DotProd / LinearSum for
  * modified GS, DotProdMulti / LinearCombination for classical GS), so on a
  * device vector the orthogonalisation runs on the device.  The
@@ -183,3 +185,112 @@ int SUNQRsol(int n, sunrealtype** h, sunrealtype* q, sunrealtype* b)
   }
   return 0;
 }
+
+/* ---- QR updates of an Anderson-acceleration space ---------------------------
+ * Each appends df as column m of Q R (R column-major, leading dimension mMax):
+ * the projections of df on Q_0..Q_{m-1} go to R(0:m-1, m), the remainder's
+ * norm to R(m, m) and the normalised remainder to Q[m].  Vector work through
+ * the N_Vector ops in the reference's order. */
+#define RC(i, j) R[(j) * mMax + (i)]
+
+SUNErrCode SUNQRAdd_MGS(N_Vector* Q, sunrealtype* R, N_Vector df, int m, int mMax, void* QRdata)
+{
+  SUNQRData d = (SUNQRData)QRdata;
+  N_Vector r  = d->vtemp;
+  N_VScale(1.0, df, r);
+  for (int j = 0; j < m; j++)
+  {
+    RC(j, m) = N_VDotProd(Q[j], r);
+    N_VLinearSum(1.0, r, -RC(j, m), Q[j], r);
+  }
+  RC(m, m) = nonneg_sqrt(N_VDotProd(r, r));
+  N_VScale(1.0 / RC(m, m), r, Q[m]);
+  return SUN_SUCCESS;
+}
+
+/* inverse compact WY: T (lower triangle, row k-1 = Q^T Q_{k-1}) corrects the
+   classical projections for the loss of orthogonality of Q */
+SUNErrCode SUNQRAdd_ICWY(N_Vector* Q, sunrealtype* R, N_Vector df, int m, int mMax,
+                         void* QRdata)
+{
+  SUNQRData d    = (SUNQRData)QRdata;
+  sunrealtype* T = d->temp_array;
+  N_VScale(1.0, df, d->vtemp);
+  if (m > 0)
+  {
+    SUNErrCode e = N_VDotProdMulti(m, Q[m - 1], Q, T + (m - 1) * mMax);
+    if (e) { return e; }
+    T[(m - 1) * mMax + (m - 1)] = 1.0;
+    e = N_VDotProdMulti(m, d->vtemp, Q, &RC(0, m));
+    if (e) { return e; }
+    /* forward substitution with T^T (unit diagonal) */
+    for (int k = 0; k < m; k++)
+    {
+      for (int j = k + 1; j < m; j++) { RC(j, m) -= RC(k, m) * T[j * mMax + k]; }
+    }
+    e = N_VLinearCombination(m, &RC(0, m), Q, d->vtemp2);
+    if (e) { return e; }
+    N_VLinearSum(1.0, d->vtemp, -1.0, d->vtemp2, d->vtemp);
+  }
+  RC(m, m) = nonneg_sqrt(N_VDotProd(d->vtemp, d->vtemp));
+  N_VScale(1.0 / RC(m, m), d->vtemp, Q[m]);
+  return SUN_SUCCESS;
+}
+
+/* classical Gram-Schmidt with one full reorthogonalisation pass */
+SUNErrCode SUNQRAdd_CGS2(N_Vector* Q, sunrealtype* R, N_Vector df, int m, int mMax,
+                         void* QRdata)
+{
+  SUNQRData d    = (SUNQRData)QRdata;
+  sunrealtype* z = d->temp_array;
+  N_VScale(1.0, df, d->vtemp);
+  if (m > 0)
+  {
+    SUNErrCode e = N_VDotProdMulti(m, d->vtemp, Q, &RC(0, m));
+    if (e) { return e; }
+    e = N_VLinearCombination(m, &RC(0, m), Q, d->vtemp2);
+    if (e) { return e; }
+    N_VLinearSum(1.0, d->vtemp, -1.0, d->vtemp2, d->vtemp2);
+    e = N_VDotProdMulti(m, d->vtemp2, Q, z);
+    if (e) { return e; }
+    /* Q[m] is a work vector until it receives the new column */
+    e = N_VLinearCombination(m, z, Q, Q[m]);
+    if (e) { return e; }
+    N_VLinearSum(1.0, d->vtemp2, -1.0, Q[m], d->vtemp);
+    for (int j = 0; j < m; j++) { RC(j, m) = RC(j, m) + z[j]; }
+  }
+  RC(m, m) = nonneg_sqrt(N_VDotProd(d->vtemp, d->vtemp));
+  N_VScale(1.0 / RC(m, m), d->vtemp, Q[m]);
+  return SUN_SUCCESS;
+}
+
+/* classical Gram-Schmidt with the reorthogonalisation of the previous column
+   delayed to this call */
+SUNErrCode SUNQRAdd_DCGS2(N_Vector* Q, sunrealtype* R, N_Vector df, int m, int mMax,
+                          void* QRdata)
+{
+  SUNQRData d    = (SUNQRData)QRdata;
+  sunrealtype* s = d->temp_array;
+  N_VScale(1.0, df, d->vtemp);
+  if (m > 0)
+  {
+    SUNErrCode e = N_VDotProdMulti(m, d->vtemp, Q, &RC(0, m));
+    if (e) { return e; }
+    if (m > 1)
+    {
+      e = N_VDotProdMulti(m - 1, Q[m - 1], Q, s);
+      if (e) { return e; }
+      e = N_VLinearCombination(m - 1, s, Q, d->vtemp2);
+      if (e) { return e; }
+      N_VLinearSum(1.0, Q[m - 1], -1.0, d->vtemp2, Q[m - 1]);
+      for (int j = 0; j < m - 1; j++) { RC(j, m - 1) = RC(j, m - 1) + s[j]; }
+    }
+    e = N_VLinearCombination(m, &RC(0, m), Q, d->vtemp2);
+    if (e) { return e; }
+    N_VLinearSum(1.0, d->vtemp, -1.0, d->vtemp2, d->vtemp);
+  }
+  RC(m, m) = nonneg_sqrt(N_VDotProd(d->vtemp, d->vtemp));
+  N_VScale(1.0 / RC(m, m), d->vtemp, Q[m]);
+  return SUN_SUCCESS;
+}
+#undef RC

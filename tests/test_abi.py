@@ -27,6 +27,8 @@ HEADERS = {
     "sunamd_profiler.h": "hip",
     "arkode_sunamd.h": "core",
     "imex_adr.h": "core",
+    "kinsol_sunamd.h": "core",
+    "kin_fp.h": "hip",
 }
 
 # declared in a header whose functions live in the other library
