@@ -98,6 +98,16 @@ int cvEwtSetSV_fused_Mi355(sunbooleantype atolMin0, sunrealtype reltol, N_Vector
 int cvNlsResid_fused_Mi355(sunrealtype rl1, sunrealtype ngamma, N_Vector zn1, N_Vector ycor,
                            N_Vector ftemp, N_Vector res);
 
+/* No counterpart in the reference (which fuses neither): the Nordsieck
+   predictor and its undo of cvPredict / cvRestore (src/cvode/cvode.c:2748,
+   2791) as one kernel over zn[0..q], q in 1..5 -- zn[j-1] = zn[j-1] +- zn[j]
+   for k = 1..q, j = q..k, element by element in that order, so the result
+   equals the q(q+1)/2 LinearSums bit for bit.  Used by the repo's CVODE under
+   CVodeSetUseIntegratorFusedKernels; -1 for vectors that are not MI355X
+   vectors. */
+int cvPredict_fused_Mi355(int q, N_Vector* zn);
+int cvRestore_fused_Mi355(int q, N_Vector* zn);
+
 #ifdef __cplusplus
 }
 #endif

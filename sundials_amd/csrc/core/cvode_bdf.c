@@ -209,6 +209,8 @@ typedef struct CVodeMemRec
   int (*fused_ewt_ss)(sunbooleantype, sunrealtype, sunrealtype, N_Vector, N_Vector, N_Vector);
   int (*fused_ewt_sv)(sunbooleantype, sunrealtype, N_Vector, N_Vector, N_Vector, N_Vector);
   int (*fused_resid)(sunrealtype, sunrealtype, N_Vector, N_Vector, N_Vector, N_Vector);
+  int (*fused_predict)(int, N_Vector*); /* Nordsieck predictor / its undo in one pass */
+  int (*fused_restore)(int, N_Vector*);
 
   sunbooleantype MallocDone;
 } CVodeMemRec;
@@ -540,7 +542,11 @@ int CVodeSetUseIntegratorFusedKernels(void* cvode_mem, sunbooleantype onoff)
   void* ss = dlsym(RTLD_DEFAULT, "cvEwtSetSS_fused_Mi355");
   void* sv = dlsym(RTLD_DEFAULT, "cvEwtSetSV_fused_Mi355");
   void* rs = dlsym(RTLD_DEFAULT, "cvNlsResid_fused_Mi355");
-  if (!is_mi355 || !ss || !sv || !rs || !is_mi355(cv->ewt)) { return CV_ILL_INPUT; }
+  void* pr = dlsym(RTLD_DEFAULT, "cvPredict_fused_Mi355");
+  void* rr = dlsym(RTLD_DEFAULT, "cvRestore_fused_Mi355");
+  if (!is_mi355 || !ss || !sv || !rs || !pr || !rr || !is_mi355(cv->ewt)) { return CV_ILL_INPUT; }
+  cv->fused_predict = (int (*)(int, N_Vector*))pr;
+  cv->fused_restore = (int (*)(int, N_Vector*))rr;
   cv->fused_ewt_ss = (int (*)(sunbooleantype, sunrealtype, sunrealtype, N_Vector, N_Vector,
                               N_Vector))ss;
   cv->fused_ewt_sv = (int (*)(sunbooleantype, sunrealtype, N_Vector, N_Vector, N_Vector,
@@ -1081,6 +1087,7 @@ static void cvPredict(CVodeMem cv)
   {
     if ((cv->tn - cv->tstop) * cv->h > ZERO) { cv->tn = cv->tstop; }
   }
+  if (cv->usefused && cv->fused_predict(cv->q, cv->zn) == 0) { return; }
   for (int k = 1; k <= cv->q; k++)
   {
     for (int j = cv->q; j >= k; j--)
@@ -1212,6 +1219,7 @@ static int cvNls(CVodeMem cv, int nflag)
 static void cvRestore(CVodeMem cv, sunrealtype saved_t)
 {
   cv->tn = saved_t;
+  if (cv->usefused && cv->fused_restore(cv->q, cv->zn) == 0) { return; }
   for (int k = 1; k <= cv->q; k++)
   {
     for (int j = cv->q; j >= k; j--)

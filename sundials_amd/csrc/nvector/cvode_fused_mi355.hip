@@ -7,6 +7,11 @@
 // vector's three rounding forms), so switching the fused kernels on does not
 // change a single bit of the integration; it saves launches and passes:
 // error weights 64 -> 24 bytes per element, residual 48 -> 32.
+// The same switch also fuses the Nordsieck predictor and its undo (cvPredict /
+// cvRestore, cvode.c:2761-2768, 2791-2803; not fused in the reference): the
+// q(q+1)/2 LinearSums zn[j-1] +- zn[j] run per element in registers, in the
+// same order, so each element rounds as in the op-by-op sequence; at q = 5,
+// 360 -> 88 bytes per element and 15 launches -> 1.
 // -----------------------------------------------------------------------------
 #include <hip/hip_runtime.h>
 
@@ -62,6 +67,56 @@ __global__ void k_nls_resid(double rl1, double ngamma, const double* __restrict_
   {
     double r = lsum3(rl1, zn1[i], 1.0, ycor[i]);
     res[i]   = lsum3(ngamma, ftemp[i], 1.0, r);
+  }
+}
+
+// Pascal-triangle update of the Nordsieck array, element pairs [0, n/2) with
+// 128-bit accesses, the odd tail element by thread 0: zn[j-1] = zn[j-1] + s
+// zn[j] for k = 1..Q, j = Q..k (s = +1 predictor, -1 restore)
+struct ZnPtrs
+{
+  double* p[6];
+};
+
+typedef double dbl2_t __attribute__((ext_vector_type(2)));
+
+template <int Q, int SIGN>
+__global__ void __launch_bounds__(256) k_pascal(ZnPtrs Z, long n)
+{
+  const long np = n >> 1;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < np; i += (long)gridDim.x * 256)
+  {
+    dbl2_t z[Q + 1];
+#pragma unroll
+    for (int j = 0; j <= Q; j++)
+    {
+      z[j] = __builtin_nontemporal_load(reinterpret_cast<const dbl2_t*>(Z.p[j]) + i);
+    }
+#pragma unroll
+    for (int k = 1; k <= Q; k++)
+    {
+#pragma unroll
+      for (int j = Q; j >= k; j--) { z[j - 1] = (SIGN > 0) ? z[j - 1] + z[j] : z[j - 1] - z[j]; }
+    }
+#pragma unroll
+    for (int j = 0; j < Q; j++)
+    {
+      __builtin_nontemporal_store(z[j], reinterpret_cast<dbl2_t*>(Z.p[j]) + i);
+    }
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0)
+  {
+    double z[Q + 1];
+#pragma unroll
+    for (int j = 0; j <= Q; j++) { z[j] = Z.p[j][n - 1]; }
+#pragma unroll
+    for (int k = 1; k <= Q; k++)
+    {
+#pragma unroll
+      for (int j = Q; j >= k; j--) { z[j - 1] = (SIGN > 0) ? z[j - 1] + z[j] : z[j - 1] - z[j]; }
+    }
+#pragma unroll
+    for (int j = 0; j < Q; j++) { Z.p[j][n - 1] = z[j]; }
   }
 }
 
@@ -131,3 +186,35 @@ extern "C" int cvNlsResid_fused_Mi355(sunrealtype rl1, sunrealtype ngamma, N_Vec
   sunamd_nv_count_launch(res);
   return done("residual");
 }
+
+// zn[0..q] on one MI355X stream; q in 1..5
+static int pascal(int q, N_Vector* zn, int sign)
+{
+  if (q < 1 || q > 5 || !SUNAMD_NVectorIsMi355(zn[0])) { return -1; }
+  const long n = (long)N_VGetLength(zn[0]);
+  if (n == 0) { return 0; }
+  ZnPtrs Z = {};
+  for (int j = 0; j <= q; j++)
+  {
+    if (!SUNAMD_NVectorIsMi355(zn[j])) { return -1; }
+    Z.p[j] = N_VGetDeviceArrayPointer(zn[j]);
+  }
+  hipStream_t s     = (hipStream_t)N_VGetStream_Mi355(zn[0]);
+  const unsigned g  = grid_for((n + 1) / 2);
+#define PASCAL_CASE(QQ)                                                              \
+  case QQ:                                                                           \
+    if (sign > 0) { k_pascal<QQ, 1><<<g, 256, 0, s>>>(Z, n); }                       \
+    else { k_pascal<QQ, -1><<<g, 256, 0, s>>>(Z, n); }                               \
+    break;
+  switch (q)
+  {
+    PASCAL_CASE(1) PASCAL_CASE(2) PASCAL_CASE(3) PASCAL_CASE(4) PASCAL_CASE(5)
+  }
+#undef PASCAL_CASE
+  sunamd_nv_count_launch(zn[0]);
+  return done(sign > 0 ? "predict" : "restore");
+}
+
+extern "C" int cvPredict_fused_Mi355(int q, N_Vector* zn) { return pascal(q, zn, 1); }
+
+extern "C" int cvRestore_fused_Mi355(int q, N_Vector* zn) { return pascal(q, zn, -1); }

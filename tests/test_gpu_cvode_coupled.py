@@ -190,14 +190,17 @@ def test_coupled_fp32_lu_accuracy(libs):
 
 
 @pytest.mark.parametrize("order", [SEQ, TREE])
-def test_coupled_fused_kernels_change_nothing(libs, order):
+@pytest.mark.parametrize("m,R,ncells", [(8, 16, 257), (3, 4, 33)])
+def test_coupled_fused_kernels_change_nothing(libs, order, m, R, ncells):
     """CVODE's fused error-weight and residual kernels (the counterparts of
-    cvode_fused_gpu.cpp) form every element with the roundings of the op
-    sequences they replace: with them on, the integration is bit-identical
-    to the unfused run (and, in sequential order, to the oracle)."""
+    cvode_fused_gpu.cpp) and the fused Nordsieck predictor / undo form every
+    element with the roundings of the op sequences they replace: with them
+    on, the integration is bit-identical to the unfused run (and, in
+    sequential order, to the oracle).  3 x 33 = 99 unknowns exercises the odd
+    tail element of the paired kernels."""
     core, hip, orc = libs
-    p = _prob(core, 8, 16, 257, 1001)
-    n = 257 * 8
+    p = _prob(core, m, R, ncells, 1001)
+    n = ncells * m
     runs = []
     for fused in (0, 1):
         r = hip.SUNAMD_CoupledCreate_Mi355(p, 1e-6, 1e-10, 0, order)
