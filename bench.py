@@ -331,9 +331,9 @@ def autotune(core, hip, prec, args_systems, weights=None):
         hip.SUNAMD_BatchedBDFFree(B)
         return kms, wall.value
 
-    runs = {w: kernel_ms([w] * nb, 0) for w in (1, 2, 4)}
+    runs = {w: kernel_ms([w] * nb, 0) for w in (1, 2, 3, 4)}
     table = {w: runs[w][0] for w in runs}
-    best_w = [min((1, 2, 4), key=lambda w: table[w][k]) for k in range(nb)]
+    best_w = [min((1, 2, 3, 4), key=lambda w: table[w][k]) for k in range(nb)]
     # buckets share the GPU, so the per-bucket mix is kept only if the whole
     # batch is faster with it than with the best uniform budget
     mix_ms, mix_wall = kernel_ms(best_w, 0)
@@ -772,7 +772,7 @@ def main():
     ap.add_argument("--no-micro", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--waves", choices=["auto", "1", "2", "4"], default="auto",
+    ap.add_argument("--waves", choices=["auto", "1", "2", "3", "4"], default="auto",
                     help="wavefronts per SIMD for every bucket (auto: calibrated per bucket; "
                          "fixed values also keep the learned start off)")
     ap.add_argument("--train-systems", type=int, default=4096)

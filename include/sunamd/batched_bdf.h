@@ -80,10 +80,11 @@ int SUNAMD_BatchedBDFSetMaxNumSteps(SUNAMD_BatchedBDF* B, long int mxsteps);
 int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* wall_s);
 /* results in system order: y (ytotal), counters (nsys*SUNAMD_BS_COUNT) */
 int SUNAMD_BatchedBDFGetResults(SUNAMD_BatchedBDF* B, double* y, long int* counters);
-/* register budget of each bucket's kernel: waves[k] in {1, 2, 4} is the
+/* register budget of each bucket's kernel: waves[k] in {1, 2, 3, 4} is the
    minimum number of wavefronts per SIMD its compilation allows (1: no limit,
-   default; 2 / 4: at most 256 / 128 registers, the rest spilled) -- the
-   occupancy-versus-spilling choice an autotuner makes per size bucket */
+   default; 2 / 3 / 4: at most 256 / 168 / 128 registers, the rest spilled;
+   3 applies to buckets without the learned start, which otherwise use 2) --
+   the occupancy-versus-spilling choice an autotuner makes per size bucket */
 int SUNAMD_BatchedBDFSetOccupancy(SUNAMD_BatchedBDF* B, const int* waves);
 /* Learned initial Newton iterate (no counterpart in the reference, whose
    Newton solve always starts from a zero correction, cvode.c:3095-3109):

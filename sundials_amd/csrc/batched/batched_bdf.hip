@@ -384,7 +384,7 @@ extern "C" int SUNAMD_BatchedBDFSetOccupancy(SUNAMD_BatchedBDF* B, const int* wa
   if (!B || !waves) { return -1; }
   for (int k = 0; k < B->nb; k++)
   {
-    if (waves[k] != 1 && waves[k] != 2 && waves[k] != 4) { return -1; }
+    if (waves[k] < 1 || waves[k] > 4) { return -1; }
   }
   for (int k = 0; k < B->nb; k++) { B->waves[k] = waves[k]; }
   return 0;

@@ -1548,7 +1548,7 @@ __device__ __forceinline__ bool g_advance(const Grp<M, T>& g, Sys<M, T>& S, doub
 // the next one from the queue between steps, so the wavefront's groups stay
 // busy while their neighbours' systems need more steps.
 // W: minimum wavefronts per SIMD the register allocation must allow (1 = no
-// limit; 2 and 4 cap it at 256 and 128 registers, spilling the rest).
+// limit; 2, 3 and 4 cap it at 256, 168 and 128 registers, spilling the rest).
 // LEARN: the learned start and the training log are compiled in; without them
 // the kernel needs fewer registers (no spills at 2 wavefronts per SIMD for
 // most sizes).
@@ -1630,7 +1630,8 @@ void launch(const BucketArgs& A, int grid, hipStream_t stream, int waves)
   const bool learn  = A.mlp_w != nullptr || A.log_x != nullptr;
   const size_t lds  = bucket_lds_bytes(M, A.R, GPW, learn);
   if (waves >= 4) { k_batched_bdf<M, T, 4, true><<<grid, 64, lds, stream>>>(A); }
-  else if (waves == 2)
+  else if (waves == 3 && !learn) { k_batched_bdf<M, T, 3, false><<<grid, 64, lds, stream>>>(A); }
+  else if (waves >= 2)
   {
     if (learn) { k_batched_bdf<M, T, 2, true><<<grid, 64, lds, stream>>>(A); }
     else { k_batched_bdf<M, T, 2, false><<<grid, 64, lds, stream>>>(A); }

@@ -273,7 +273,9 @@ static int init_aa(KinMem* k)
 
 static int init_orth(KinMem* k)
 {
-  if (k->m_aa > k->orth_alloc)
+  /* (re)allocate also when an earlier solve used a method that needed less */
+  if (k->m_aa > k->orth_alloc || (k->orth_aa != KIN_ORTH_MGS && !k->vtemp3) ||
+      (k->orth_aa == KIN_ORTH_ICWY && !k->T_aa))
   {
     free_orth(k);
     k->orth_alloc = k->m_aa;
