@@ -123,6 +123,28 @@ class KinOptions(C.Structure):
 
 
 KIN_STATS = 5
+
+
+class KinNewtonOptions(C.Structure):
+    """SUNAMD_KinNewtonOptions (include/sunamd/kin_newton.h)."""
+    _fields_ = [
+        ("ncells", C.c_long),
+        ("variant", C.c_int),
+        ("guess", C.c_int),
+        ("perturb", C.c_int),
+        ("strategy", C.c_int),
+        ("msbset", C.c_long),
+        ("fnormtol", C.c_double),
+        ("scsteptol", C.c_double),
+        ("maxiter", C.c_long),
+        ("jac", C.c_int),
+        ("sequential", C.c_int),
+    ]
+
+
+KINN_STATS = 8
+KINN_STAT_NAMES = ("flag", "nni", "nfe", "nje", "nfeD", "nbcf", "nbacktrack", "wall")
+KIN_NONE, KIN_LINESEARCH, KIN_PICARD, KIN_FP = 0, 1, 2, 3
 ADR_STAT_NAMES = ("nst", "attempts", "netf", "ncfn", "nfe", "nfi", "nni", "nnf", "nsetups",
                   "nli", "npe", "nps", "njtimes", "ncfl", "wall", "hlast")
 
@@ -307,6 +329,8 @@ def hip():
     _sig(lib, "SUNAMD_TridiagPSolve_Mi355", C.c_int, [c_void_p, c_void_p, c_void_p, c_real, C.c_int])
     _sig(lib, "SUNAMD_AdrSolve_Mi355", C.c_int, [P(Adr), P(AdrOptions), P(c_real), P(c_real)])
     _sig(lib, "SUNAMD_KinSolve_Mi355", C.c_int, [P(KinOptions), P(c_real), P(c_real)])
+    _sig(lib, "SUNAMD_KinNewtonSolve_Mi355", C.c_int,
+         [P(KinNewtonOptions), P(c_real), P(c_real)])
     _sig(lib, "SUNAMD_BatchedBDFCreate", c_void_p, [P(Batch), c_real, c_real, C.c_int])
     _sig(lib, "SUNAMD_BatchedBDFSetMaxNumSteps", C.c_int, [c_void_p, C.c_long])
     _sig(lib, "SUNAMD_BatchedBDFRun", C.c_int, [c_void_p, c_real, P(c_real)])
@@ -363,6 +387,8 @@ def load_oracle():
     _sig(lib, "oracle_tridiag_psolve", C.c_int, [c_void_p, c_void_p, c_void_p, c_real, C.c_int])
     _sig(lib, "oracle_adr_solve", C.c_int, [P(Adr), P(AdrOptions), P(c_real), P(c_real)])
     _sig(lib, "oracle_kin_solve", C.c_int, [P(KinOptions), P(c_real), P(c_real)])
+    _sig(lib, "oracle_kin_newton_solve", C.c_int, [P(KinNewtonOptions), P(c_real), P(c_real)])
+    _sig(lib, "oracle_kin_fertron_example", C.c_int, [P(c_real), P(c_real)])
     _sig(lib, "oracle_ark_dahlquist", C.c_int, [c_real, c_real, c_real, c_real, c_real, c_real,
                                                 C.c_char_p, C.c_int, P(c_real), P(c_real)])
     _sig(lib, "oracle_batch_solve", C.c_int,

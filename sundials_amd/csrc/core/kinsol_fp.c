@@ -23,48 +23,8 @@
 #include <stdlib.h>
 #include <string.h>
 
-#include "sunamd/kinsol_sunamd.h"
-#include "sunamd/sundials_iterative.h"
+#include "kinsol_impl.h"
 #include "sunamd_pow.h"
-
-#define ZERO 0.0
-#define ONE  1.0
-
-#define MXITER_DEFAULT 200
-#define ONETHIRD       0.3333333333333333
-#define UROUND         2.220446049250313e-16
-
-#define CONTINUE_ITERATIONS (-999)
-
-typedef struct
-{
-  SUNContext ctx;
-  KINSysFn func;
-  void* user_data;
-  int malloc_done;
-  /* options */
-  long int mxiter;
-  sunrealtype fnormtol;
-  sunrealtype beta, beta_aa;
-  int damping, damping_aa;
-  long int m_aa, delay_aa;
-  int orth_aa;
-  int ret_newest;
-  KINDampingFn damping_fn;
-  KINDepthFn depth_fn;
-  /* state */
-  N_Vector uu, fscale;
-  N_Vector unew, fval, vtemp1, vtemp2, vtemp3;
-  long int nni, nfe;
-  sunrealtype fnorm;
-  /* Anderson acceleration workspace */
-  long int m_aa_alloc, orth_alloc, current_depth;
-  sunrealtype *gamma_aa, *R_aa, *T_aa, *cv;
-  N_Vector *q_aa, *df_aa, *dg_aa, *Xv;
-  N_Vector fold_aa, gold_aa;
-  struct _SUNQRData qr;
-  SUNQRAddFn qr_func;
-} KinMem;
 
 void* KINCreate(SUNContext sunctx)
 {
@@ -77,6 +37,14 @@ void* KINCreate(SUNContext sunctx)
   k->beta     = ONE;
   k->beta_aa  = ONE;
   k->orth_aa  = KIN_ORTH_MGS;
+  /* Newton defaults (kinsol.c:262-290) */
+  k->scsteptol    = sunamd_pow(UROUND, TWOTHIRDS);
+  k->sqrt_relfunc = sqrt(UROUND);
+  k->msbset       = MSBSET_DEFAULT;
+  k->msbset_sub   = MSBSET_SUB_DEFAULT;
+  k->mxnbcf       = MXNBCF_DEFAULT;
+  k->omega_min    = OMEGA_MIN_DEFAULT;
+  k->omega_max    = OMEGA_MAX_DEFAULT;
   return k;
 }
 
@@ -95,7 +63,8 @@ int KINInit(void* kinmem, KINSysFn func, N_Vector tmpl)
   k->fval   = N_VClone(tmpl);
   k->vtemp1 = N_VClone(tmpl);
   k->vtemp2 = N_VClone(tmpl);
-  if (!k->unew || !k->fval || !k->vtemp1 || !k->vtemp2) { return KIN_MEM_FAIL; }
+  k->pp     = N_VClone(tmpl);
+  if (!k->unew || !k->fval || !k->vtemp1 || !k->vtemp2 || !k->pp) { return KIN_MEM_FAIL; }
   k->func        = func;
   k->malloc_done = 1;
   return KIN_SUCCESS;
@@ -316,7 +285,7 @@ static int init_orth(KinMem* k)
 }
 
 /* max norm of f_scale * v */
-static sunrealtype sc_fnorm(KinMem* k, N_Vector v, N_Vector scale)
+sunrealtype kin_sc_fnorm(KinMem* k, N_Vector v, N_Vector scale)
 {
   N_VProd(scale, v, k->vtemp1);
   return N_VMaxNorm(k->vtemp1);
@@ -551,7 +520,7 @@ static int kin_fp(KinMem* k)
       tolfac = (iter_aa == 0 && (k->damping_aa || k->damping_fn)) ? k->beta_aa : ONE;
     }
     N_VLinearSum(ONE, k->unew, -ONE, k->uu, delta);
-    k->fnorm = sc_fnorm(k, delta, k->fscale);
+    k->fnorm = kin_sc_fnorm(k, delta, k->fscale);
     if (k->nni >= k->mxiter) { ret = KIN_MAXITER_REACHED; }
     if (k->fnorm <= tolfac * k->fnormtol) { ret = KIN_SUCCESS; }
     if (ret == CONTINUE_ITERATIONS || k->ret_newest) { N_VScale(ONE, k->unew, k->uu); }
@@ -562,12 +531,15 @@ static int kin_fp(KinMem* k)
 int KINSol(void* kinmem, N_Vector uu, int strategy, N_Vector u_scale, N_Vector f_scale)
 {
   KMEM();
-  (void)u_scale;
   if (!k->malloc_done) { return KIN_NO_MALLOC; }
-  if (strategy != KIN_FP) { return KIN_ILL_INPUT; }
-  if (!uu || !f_scale) { return KIN_ILL_INPUT; }
-  k->uu     = uu;
-  k->fscale = f_scale;
+  if (!uu) { return KIN_ILL_INPUT; }
+  k->uu       = uu;
+  k->uscale   = u_scale;
+  k->fscale   = f_scale;
+  k->strategy = strategy;
+  if (strategy == KIN_NONE || strategy == KIN_LINESEARCH) { return kin_newton(k); }
+  /* KIN_PICARD is not provided by this library */
+  if (strategy != KIN_FP || !f_scale || k->constraints) { return KIN_ILL_INPUT; }
   if (k->m_aa != 0)
   {
     int r = init_aa(k);
@@ -589,6 +561,9 @@ void KINFree(void** kinmem)
   N_VDestroy(k->fval);
   N_VDestroy(k->vtemp1);
   N_VDestroy(k->vtemp2);
+  N_VDestroy(k->pp);
+  if (k->constraints) { N_VDestroy(k->constraints); }
+  kin_ls_free(k);
   free(k);
   *kinmem = NULL;
 }

@@ -29,6 +29,7 @@ HEADERS = {
     "imex_adr.h": "core",
     "kinsol_sunamd.h": "core",
     "kin_fp.h": "hip",
+    "kin_newton.h": "hip",
 }
 
 # declared in a header whose functions live in the other library
