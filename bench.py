@@ -357,7 +357,7 @@ def autotune(core, hip, prec, args_systems, weights=None):
 
 
 IMEX_SEED = 1003
-FAM_LINSUM, FAM_WRMS, FAM_ADR_FE, FAM_SCALE = 2, 3, 7, 8
+FAM_GETRS, FAM_GETRF, FAM_LINSUM, FAM_WRMS, FAM_ADR_FE, FAM_SCALE = 0, 1, 2, 3, 7, 8
 
 
 def bench_imex(args):
@@ -578,6 +578,116 @@ def bench_kinsol(args):
     }
 
 
+def bench_kinsol_newton(args):
+    """KINSOL modified Newton with the line search on the constrained
+    Ferraris-Tronconi cell system (6 unknowns per cell, analytic 6 x 6
+    Jacobian blocks, batched block LU in FP64) over 2^20 cells, perturbed
+    initial guesses in the middle of the box, fnormtol 1e-10; a step is one
+    KINSol call."""
+    import torch
+    from sundials_amd import _lib
+
+    hip = _lib.hip()
+    ncells = args.kinn_cells
+    N = 6 * ncells
+    M = 6
+
+    def opts(nc):
+        o = _lib.KinNewtonOptions()
+        o.ncells, o.variant, o.guess, o.perturb = nc, 1, 2, 1
+        o.strategy, o.msbset, o.fnormtol, o.scsteptol = _lib.KIN_LINESEARCH, 0, 1e-10, 1e-12
+        o.maxiter, o.jac, o.sequential = 0, 1, 0
+        return o
+
+    PD = C.POINTER(C.c_double)
+
+    def run():
+        st = np.zeros(_lib.KINN_STATS)
+        rc = hip.SUNAMD_KinNewtonSolve_Mi355(C.byref(opts(ncells)), st.ctypes.data_as(PD), None)
+        if rc != 0:
+            print("[bench] kinsol newton solve failed rc=%d" % rc, file=sys.stderr)
+            sys.exit(1)
+        return dict(zip(_lib.KINN_STAT_NAMES, st))
+
+    for w in range(args.warmup):
+        st = run()
+        print("[bench] warmup %d: %.4f s, %d iterations" % (w, st["wall"], st["nni"]),
+              file=sys.stderr, flush=True)
+    # the timed region of each step is the KINSol call (its own synchronised
+    # wall clock); creating the problem, matrix and solver is not
+    torch.cuda.synchronize()
+    iters, elapsed = 0, 0.0
+    for k in range(args.steps):
+        st = run()
+        iters += int(st["nni"])
+        elapsed += st["wall"]
+        print("[bench] step %d: %.4f s, %d iterations, %d Jacobians"
+              % (k, st["wall"], st["nni"], st["nje"]), file=sys.stderr, flush=True)
+    value = N * iters / elapsed
+
+    hip.SUNAMD_ProfilerReset()
+    hip.SUNAMD_ProfilerEnable(1)
+    run()
+    hip.SUNAMD_ProfilerEnable(0)
+    fams = {FAM_GETRF: ("k_getrf (6x6 blocks)", ncells * (2.0 * M * M * 8 + M * 4)),
+            FAM_GETRS: ("k_getrs (6x6 blocks)", ncells * (M * M * 8 + M * 4 + 2.0 * M * 8)),
+            FAM_LINSUM: ("N_VLinearSum", 24.0 * N), FAM_SCALE: ("N_VScale", 16.0 * N)}
+    kern = {}
+    for f, (name, nbytes) in fams.items():
+        tot, cnt = C.c_double(), C.c_long()
+        hip.SUNAMD_ProfilerGet(f, C.byref(tot), C.byref(cnt))
+        if cnt.value:
+            kern[f] = {"name": name, "total_ms": tot.value, "launches": cnt.value,
+                       "avg_us": 1e3 * tot.value / cnt.value, "bytes_per_launch": nbytes}
+    dom = max(kern, key=lambda f: kern[f]["total_ms"])
+    dk = kern[dom]
+    achieved = dk["bytes_per_launch"] / (dk["avg_us"] * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "kernel": dk["name"], "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "bytes_per_launch": dk["bytes_per_launch"],
+                "avg_launch_us": round(dk["avg_us"], 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        orc = _lib.load_oracle()
+        nc = args.kinn_cpu_cells
+        so = np.zeros(_lib.KINN_STATS)
+        t = time.perf_counter()
+        rc = orc.oracle_kin_newton_solve(C.byref(opts(nc)), so.ctypes.data_as(PD), None)
+        wall = time.perf_counter() - t
+        if rc == 0:
+            cpu = {"value": 6 * nc * so[1] / so[7], "unit": "unknown-iterations/s", "cores": 1,
+                   "kind": "port",
+                   "sample": "CPU oracle (serial vector, per-block LU, the same KINSOL "
+                             "restatement) on %d cells (%d iterations, KINSol %.2f s of %.2f s "
+                             "with set-up, %s)" % (nc, int(so[1]), so[7], wall, cpu_model())}
+    ms = 1e3 * elapsed / args.steps
+    return {
+        "metric": "unknown-iterations/s (KINSOL Newton + line search, block LU)",
+        "value": value,
+        "unit": "unknown-iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "KINSol KIN_LINESEARCH, modified Newton (setup every 10 "
+                               "iterations), constrained Ferraris-Tronconi cells (IEEE-only "
+                               "variant) x %d, analytic 6x6 Jacobian blocks, FP64 block LU, "
+                               "fnormtol 1e-10" % ncells, "unknowns": N},
+        "solver": {k: int(st[k]) for k in ("nni", "nfe", "nje", "nbcf", "nbacktrack")},
+        "timed": "KINSol calls (problem, matrix and solver set-up excluded)",
+        "roofline": roofline,
+        "kernels": {kern[f]["name"]: {k: v for k, v in kern[f].items() if k != "name"}
+                    for f in kern},
+        "cpu_baseline": cpu,
+    }
+
+
 def bench_batched(args):
     import torch
     from sundials_amd import _lib
@@ -756,10 +866,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["batched", "coupled", "imex", "kinsol"],
+    ap.add_argument("--workload", choices=["batched", "coupled", "imex", "kinsol", "kinsol-newton"],
                     default="batched")
     ap.add_argument("--kin-cells", type=int, default=1 << 21, help="kinsol: cells (3 unknowns)")
     ap.add_argument("--kin-cpu-cells", type=int, default=1 << 21, help="kinsol: CPU-oracle cells")
+    ap.add_argument("--kinn-cells", type=int, default=1 << 20,
+                    help="kinsol-newton: cells (6 unknowns)")
+    ap.add_argument("--kinn-cpu-cells", type=int, default=1 << 20,
+                    help="kinsol-newton: CPU-oracle cells")
     ap.add_argument("--grid", type=int, default=128, help="imex: cells per dimension")
     ap.add_argument("--tf", type=float, default=0.01, help="imex: final time")
     ap.add_argument("--cpu-grid", type=int, default=64, help="imex: CPU-oracle grid")
@@ -798,6 +912,8 @@ def main():
         line = bench_imex(args)
     elif args.workload == "kinsol":
         line = bench_kinsol(args)
+    elif args.workload == "kinsol-newton":
+        line = bench_kinsol_newton(args)
     else:
         line = bench_coupled(args)
     print(json.dumps(line))
