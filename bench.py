@@ -592,18 +592,19 @@ def bench_kinsol_newton(args):
     N = 6 * ncells
     M = 6
 
-    def opts(nc):
+    def opts(nc, fused=1):
         o = _lib.KinNewtonOptions()
         o.ncells, o.variant, o.guess, o.perturb = nc, 1, 2, 1
         o.strategy, o.msbset, o.fnormtol, o.scsteptol = _lib.KIN_LINESEARCH, 0, 1e-10, 1e-12
-        o.maxiter, o.jac, o.sequential = 0, 1, 0
+        o.maxiter, o.jac, o.sequential, o.fused = 0, 1, 0, fused
         return o
 
     PD = C.POINTER(C.c_double)
 
-    def run():
+    def run(fused=1):
         st = np.zeros(_lib.KINN_STATS)
-        rc = hip.SUNAMD_KinNewtonSolve_Mi355(C.byref(opts(ncells)), st.ctypes.data_as(PD), None)
+        rc = hip.SUNAMD_KinNewtonSolve_Mi355(C.byref(opts(ncells, fused)), st.ctypes.data_as(PD),
+                                             None)
         if rc != 0:
             print("[bench] kinsol newton solve failed rc=%d" % rc, file=sys.stderr)
             sys.exit(1)
@@ -624,6 +625,11 @@ def bench_kinsol_newton(args):
         print("[bench] step %d: %.4f s, %d iterations, %d Jacobians"
               % (k, st["wall"], st["nni"], st["nje"]), file=sys.stderr, flush=True)
     value = N * iters / elapsed
+    # the same solve with KINSOL's scaled norms as op chains (the reference's
+    # sequence) instead of the fused one-pass kernels; identical iterates
+    unf = run(fused=0)
+    print("[bench] unfused norms: %.4f s, %d iterations" % (unf["wall"], unf["nni"]),
+          file=sys.stderr, flush=True)
 
     hip.SUNAMD_ProfilerReset()
     hip.SUNAMD_ProfilerEnable(1)
@@ -676,11 +682,14 @@ def bench_kinsol_newton(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "KINSol KIN_LINESEARCH, modified Newton (setup every 10 "
-                               "iterations), constrained Ferraris-Tronconi cells (IEEE-only "
+                               "iterations), fused scaled norms, constrained Ferraris-Tronconi cells (IEEE-only "
                                "variant) x %d, analytic 6x6 Jacobian blocks, FP64 block LU, "
                                "fnormtol 1e-10" % ncells, "unknowns": N},
         "solver": {k: int(st[k]) for k in ("nni", "nfe", "nje", "nbcf", "nbacktrack")},
         "timed": "KINSol calls (problem, matrix and solver set-up excluded)",
+        "fused_kernels": {"on_ms": ms, "off_ms": 1e3 * unf["wall"],
+                          "what": "KINSetUseFusedKernels: KINScSNorm / KINScFNorm in one "
+                                  "pass each (bit-identical)"},
         "roofline": roofline,
         "kernels": {kern[f]["name"]: {k: v for k, v in kern[f].items() if k != "name"}
                     for f in kern},

@@ -33,6 +33,7 @@ typedef struct
   long int maxiter;  /* KINSetNumMaxIters (0: default 200) */
   int jac;           /* 1: analytic block Jacobian; 0: dense DQ (CPU only) */
   int sequential;    /* MI355X vector only: reductions in serial order */
+  int fused;         /* MI355X vector only: KINSetUseFusedKernels (scaled norms) */
 } SUNAMD_KinNewtonOptions;
 
 /* stats (doubles): return flag, nni, nfe, nje, nfeDQ, nbcf, nbacktrack,

@@ -114,6 +114,11 @@ int KINSetResMonConstVal(void* kinmem, sunrealtype omegaconst);
    vector (N_VGetArrayPointer / N_VSetArrayPointer). */
 int KINSetLinearSolver(void* kinmem, SUNLinearSolver LS, SUNMatrix A);
 int KINSetJacFn(void* kinmem, KINLsJacFn jac);
+/* No counterpart in the reference: evaluate KINSOL's scaled norms
+   (KINScFNorm, KINScSNorm) as one fused device pass each when the vectors are
+   MI355X vectors (libsundials_amd.so loaded); results are bit-identical to the
+   op chains.  Returns KIN_ILL_INPUT (and leaves the op chains on) otherwise. */
+int KINSetUseFusedKernels(void* kinmem, sunbooleantype onoff);
 /* strategy KIN_NONE, KIN_LINESEARCH or KIN_FP; KIN_FP does not use u_scale,
    and its f_scale weights the max norm of G(u) - u */
 int KINSol(void* kinmem, N_Vector uu, int strategy, N_Vector u_scale, N_Vector f_scale);

@@ -108,6 +108,16 @@ int cvNlsResid_fused_Mi355(sunrealtype rl1, sunrealtype ngamma, N_Vector zn1, N_
 int cvPredict_fused_Mi355(int q, N_Vector* zn);
 int cvRestore_fused_Mi355(int q, N_Vector* zn);
 
+/* No counterpart in the reference (KINSOL has no fused kernels): KINSOL's
+   scaled norms (src/kinsol/kinsol.c:2038-2062) in one pass each,
+   KINScSNorm = max_i |v_i / (1/uscale_i + |u_i|)| and
+   KINScFNorm = max_i |fscale_i v_i|, bit-identical to the op chains.  Used
+   by the repo's KINSOL under KINSetUseFusedKernels; negative for vectors that
+   are not MI355X vectors or on a HIP error.  Scratch is per device and shared
+   by all calls: calls that overlap on one device race. */
+sunrealtype kinScSNorm_fused_Mi355(N_Vector v, N_Vector u, N_Vector uscale);
+sunrealtype kinScFNorm_fused_Mi355(N_Vector v, N_Vector fscale);
+
 #ifdef __cplusplus
 }
 #endif

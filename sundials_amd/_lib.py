@@ -139,6 +139,7 @@ class KinNewtonOptions(C.Structure):
         ("maxiter", C.c_long),
         ("jac", C.c_int),
         ("sequential", C.c_int),
+        ("fused", C.c_int),
     ]
 
 

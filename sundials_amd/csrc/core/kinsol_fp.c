@@ -287,6 +287,11 @@ static int init_orth(KinMem* k)
 /* max norm of f_scale * v */
 sunrealtype kin_sc_fnorm(KinMem* k, N_Vector v, N_Vector scale)
 {
+  if (k->usefused)
+  {
+    const sunrealtype r = k->fused_sfnorm(v, scale);
+    if (r >= ZERO) { return r; }
+  }
   N_VProd(scale, v, k->vtemp1);
   return N_VMaxNorm(k->vtemp1);
 }

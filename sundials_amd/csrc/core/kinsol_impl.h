@@ -81,6 +81,10 @@ struct KinMem
   long int nnilset, nnilset_sub, nbcf, nbktrk, ncscmx;
   int jac_current, update_fnorm_sub, retry_nni, eval_omega;
   KinLsMem* ls;
+  /* fused scaled norms of the MI355X vector (KINSetUseFusedKernels) */
+  int usefused;
+  sunrealtype (*fused_ssnorm)(N_Vector v, N_Vector u, N_Vector uscale);
+  sunrealtype (*fused_sfnorm)(N_Vector v, N_Vector fscale);
   /* Anderson acceleration workspace */
   long int m_aa_alloc, orth_alloc, current_depth;
   sunrealtype *gamma_aa, *R_aa, *T_aa, *cv;

@@ -26,6 +26,8 @@
  * iteration runs on the device.  Only iterative (matrix-free) linear solvers
  * and the Picard strategy are outside this library.
  * ---------------------------------------------------------------------------*/
+#define _GNU_SOURCE
+#include <dlfcn.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -151,6 +153,29 @@ int KINSetResMonConstVal(void* kinmem, sunrealtype omegaconst)
   KMEM();
   if (omegaconst < ZERO) { return KIN_ILL_INPUT; }
   k->omega = omegaconst;
+  return KIN_SUCCESS;
+}
+
+/* The scaled norms as one device pass each when the vectors are the MI355X
+   library's (looked up in the loaded library, like CVODE's fused kernels);
+   bit-identical to the op chains.  Fails, leaving the op chains on, when the
+   library or its vector is absent. */
+int KINSetUseFusedKernels(void* kinmem, sunbooleantype onoff)
+{
+  KMEM();
+  if (!onoff)
+  {
+    k->usefused = 0;
+    return KIN_SUCCESS;
+  }
+  if (!k->malloc_done) { return KIN_ILL_INPUT; }
+  int (*is_mi355)(N_Vector) = (int (*)(N_Vector))dlsym(RTLD_DEFAULT, "SUNAMD_NVectorIsMi355");
+  void* ss = dlsym(RTLD_DEFAULT, "kinScSNorm_fused_Mi355");
+  void* sf = dlsym(RTLD_DEFAULT, "kinScFNorm_fused_Mi355");
+  if (!is_mi355 || !ss || !sf || !is_mi355(k->vtemp1)) { return KIN_ILL_INPUT; }
+  k->fused_ssnorm = (sunrealtype(*)(N_Vector, N_Vector, N_Vector))ss;
+  k->fused_sfnorm = (sunrealtype(*)(N_Vector, N_Vector))sf;
+  k->usefused     = 1;
   return KIN_SUCCESS;
 }
 
@@ -381,6 +406,11 @@ static int ls_solve(KinMem* k, N_Vector x, N_Vector b)
 /* kinsol.c:2051-2062: max_i |v_i| / (1/uscale_i + |u_i|) */
 static sunrealtype sc_snorm(KinMem* k, N_Vector v, N_Vector u)
 {
+  if (k->usefused)
+  {
+    const sunrealtype r = k->fused_ssnorm(v, u, k->uscale);
+    if (r >= ZERO) { return r; }
+  }
   N_VInv(k->uscale, k->vtemp1);
   N_VAbs(u, k->vtemp2);
   N_VLinearSum(ONE, k->vtemp1, ONE, k->vtemp2, k->vtemp1);

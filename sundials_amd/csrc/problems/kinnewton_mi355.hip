@@ -142,6 +142,7 @@ extern "C" int SUNAMD_KinNewtonSolve_Mi355(const SUNAMD_KinNewtonOptions* o, dou
     r |= KINInit(kmem, cb_f, u);
     r |= KINSetLinearSolver(kmem, LS, J);
     r |= KINSetJacFn(kmem, cb_jac);
+    if (o->fused) { r |= KINSetUseFusedKernels(kmem, SUNTRUE); }
     ok = r == 0;
   }
   if (ok)

@@ -6,10 +6,12 @@ oracle (the same restatement over the serial vector, per-block LU).
 - one cell, the reference example's two initial guesses: the device retraces
   the published kinFerTron_dns counts (nni / nfe / nje) and printed solution;
 - 1, 257 and 4096 perturbed cells, both guesses, exact and modified Newton,
-  with and without the line search, reductions in serial order: flag, every
-  counter and every final value identical to the oracle bit for bit;
+  with and without the line search, with and without the fused scaled norms
+  (KINSetUseFusedKernels), reductions in serial order: flag, every counter
+  and every final value identical to the oracle bit for bit;
 - device-wide tree reductions on 2^18 cells: converged, counts within one of
-  the oracle's, every cell at its root, the constraints satisfied.
+  the oracle's, every cell at its root, the constraints satisfied; the fused
+  norms give the identical run.
 """
 import ctypes as C
 import json
@@ -37,6 +39,7 @@ def _opts(**kw):
     o = _lib.KinNewtonOptions()
     o.ncells, o.variant, o.guess, o.perturb, o.strategy, o.msbset = 1, 1, 2, 0, 0, 1
     o.fnormtol, o.scsteptol, o.maxiter, o.jac, o.sequential = 1e-5, 1e-5, 0, 1, 1
+    o.fused = 0
     for k, v in kw.items():
         setattr(o, k, v)
     return o
@@ -77,11 +80,13 @@ def test_fertron_on_device_retraces_published_run(ref_run):
 @pytest.mark.parametrize("ncells", [1, 257, 4096])
 @pytest.mark.parametrize("guess", [1, 2])
 @pytest.mark.parametrize("strategy,msbset", [(0, 1), (1, 1), (0, 0), (1, 0)])
-def test_device_equals_oracle_bitwise(ncells, guess, strategy, msbset):
+@pytest.mark.parametrize("fused", [0, 1])
+def test_device_equals_oracle_bitwise(ncells, guess, strategy, msbset, fused):
     o = _opts(ncells=ncells, perturb=1, guess=guess, strategy=strategy, msbset=msbset,
-              fnormtol=1e-10, scsteptol=1e-12)
+              fnormtol=1e-10, scsteptol=1e-12, fused=fused)
     rc_d, st_d, u_d = device(o)
-    rc_o, st_o, u_o = oracle(o)
+    rc_o, st_o, u_o = oracle(_opts(ncells=ncells, perturb=1, guess=guess, strategy=strategy,
+                                   msbset=msbset, fnormtol=1e-10, scsteptol=1e-12))
     assert rc_d == rc_o == 0
     assert np.array_equal(st_d[:7], st_o[:7])
     assert np.array_equal(u_d, u_o)
@@ -92,6 +97,10 @@ def test_tree_reductions_large(guess, root):
     o = _opts(ncells=1 << 18, perturb=1, guess=guess, strategy=1, msbset=0,
               fnormtol=1e-10, scsteptol=1e-12, sequential=0)
     rc_d, st_d, u_d = device(o)
+    # the fused scaled norms change nothing, also with tree reductions
+    o.fused = 1
+    rc_f, st_f, u_f = device(o)
+    assert rc_f == rc_d and np.array_equal(st_f[:7], st_d[:7]) and np.array_equal(u_f, u_d)
     rc_o, st_o, _ = oracle(o)
     assert rc_d == rc_o == 0
     assert abs(int(st_d[1]) - int(st_o[1])) <= 1

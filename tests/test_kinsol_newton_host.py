@@ -64,6 +64,7 @@ def _newton_opts(**kw):
     o = _lib.KinNewtonOptions()
     o.ncells, o.variant, o.guess, o.perturb, o.strategy, o.msbset = 1, 1, 2, 0, 0, 1
     o.fnormtol, o.scsteptol, o.maxiter, o.jac, o.sequential = 1e-5, 1e-5, 0, 1, 0
+    o.fused = 0
     for k, v in kw.items():
         setattr(o, k, v)
     return o
