@@ -21,7 +21,10 @@
 - the test problem's deterministic sin / exp (variant 1) agree with the math
   library to a few ulp and its analytic Jacobian with central differences;
   the device twin (variant 1, analytic block Jacobian, per-block LU) retraces
-  the example's counts.
+  the example's counts;
+- input errors (infeasible initial guess, Picard, constraints with KIN_FP,
+  constraint values beyond +-2, an iterative linear solver, fused norms on
+  serial vectors) return the reference's codes.
 """
 import ctypes as C
 import json
@@ -289,3 +292,41 @@ def test_constraint_cut_uses_constrained_components_only():
     assert flag == -6 and nfe == 2
     assert u[0] == pytest.approx(1.0 - 0.45 * 2.0, abs=1e-15)
     assert u[1] == pytest.approx(1.0 - 0.45 * 8.0, abs=1e-15)
+
+
+def test_error_paths():
+    """Input errors are reported with the reference's return codes."""
+    core = _lib.core()
+    core.KINSetUseFusedKernels.restype = C.c_int
+    core.KINSetUseFusedKernels.argtypes = [C.c_void_p, C.c_int]
+    core.KINSetConstraints.restype = C.c_int
+    core.KINSetConstraints.argtypes = [C.c_void_p, C.c_void_p]
+    # a solve whose initial guess violates the constraints: KIN_ILL_INPUT
+    K = _Kin(1, lambda u: np.arctan(u), lambda u: [[1.0 / (1.0 + u[0] ** 2)]],
+             constraints=[1.0])
+    try:
+        flag, _, _, _ = K.solve([-1.0])
+        assert flag == -2
+        # Picard is not part of this library; KIN_FP refuses constraints
+        assert core.KINSol(K.k, K.u, 2, K.s, K.s) == -2
+        assert core.KINSol(K.k, K.u, 3, K.s, K.s) == -2
+        # fused norms need MI355X vectors
+        assert core.KINSetUseFusedKernels(K.k, 1) == -2
+        assert core.KINSetUseFusedKernels(K.k, 0) == 0
+        # a constraints vector with an entry beyond +-2 is rejected
+        orc = _lib.load_oracle()
+        bad = orc.N_VNew_Serial(1, K.ctx)
+        K.arr(bad)[:] = 3.0
+        assert core.KINSetConstraints(K.k, bad) == -2
+        # after removing the constraints the same start converges
+        assert core.KINSetConstraints(K.k, None) == 0
+        core.KINSetNumMaxIters(K.k, 50)
+        core.KINSetFuncNormTol(K.k, 1e-12)
+        flag, u, _, _ = K.solve([-1.0])
+        assert flag in (0, 2) and abs(u[0]) < 1e-10
+        # an iterative linear solver is refused by KINSetLinearSolver
+        LS2 = core.SUNLinSol_SPGMR(K.u, 0, 5, K.ctx)
+        assert core.KINSetLinearSolver(K.k, LS2, None) == -3
+        core.SUNLinSolFree(LS2)
+    finally:
+        K.close()
