@@ -358,6 +358,18 @@ def autotune(core, hip, prec, args_systems, weights=None):
 
 IMEX_SEED = 1003
 FAM_GETRS, FAM_GETRF, FAM_LINSUM, FAM_WRMS, FAM_ADR_FE, FAM_SCALE = 0, 1, 2, 3, 7, 8
+# HBM bytes per launch of the batched integrator's two-wavefront FP32-LU kernels, measured
+# with rocprofv3 --pmc (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors a plain 256 MiB
+# copy gives); a PMC run cannot happen inside the timed process, so the figures are recorded
+MEASURED_TRAFFIC = {
+    "k_batched_bdf<4, float, 2, false>": 8.66e6,
+    "k_batched_bdf<8, float, 2, false>": 12.28e6,
+    "k_batched_bdf<16, float, 2, false>": 13.03e6,
+    "k_batched_bdf<24, float, 2, false>": 12.21e6,
+    "k_batched_bdf<32, float, 2, false>": 16.19e6,
+    "k_batched_bdf<48, float, 2, false>": 18.65e6,
+    "k_batched_bdf<64, float, 2, false>": 87.34e6,
+}
 
 
 def bench_imex(args):
@@ -812,7 +824,13 @@ def bench_batched(args):
     roofline = {"bound": "mfma", "kernel": "k_batched_bdf<%d, float, %d, %s>" % (
                     bm[dom], w32[dom], "true" if learn_dom else "false"),
                 "achieved": round(achieved, 4), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFS, 5), "traffic": None,
+                "frac": round(achieved / FP64_PEAK_TFS, 5),
+                "traffic": MEASURED_TRAFFIC.get(
+                    "k_batched_bdf<%d, float, %d, %s>" % (bm[dom], w32[dom],
+                                                         "true" if learn_dom else "false")),
+                "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this benchmark, "
+                                  "scaled by a 256 MiB copy (x2.000 / x1.000); null for a "
+                                  "kernel not measured",
                 "flops_per_launch": dflop, "launch_ms": round(float(kms[dom]), 3),
                 "note": "compute/latency-bound persistent integrator; priced against the FP64 "
                         "vector peak (most of its arithmetic is FP64)"}
