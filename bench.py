@@ -361,6 +361,26 @@ FAM_GETRS, FAM_GETRF, FAM_LINSUM, FAM_WRMS, FAM_ADR_FE, FAM_SCALE = 0, 1, 2, 3, 
 # HBM bytes per launch of the batched integrator's two-wavefront FP32-LU kernels, measured
 # with rocprofv3 --pmc (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors a plain 256 MiB
 # copy gives); a PMC run cannot happen inside the timed process, so the figures are recorded
+# HBM bytes per launch of the streaming legs' dominant kernels at their benchmark sizes
+# (6,291,456 unknowns; 2^20 blocks of 6 x 6), measured the same way; equal to the
+# algorithmic bytes within 2 percent
+MEASURED_STREAM_TRAFFIC = {
+    ("kinsol-newton", "N_VLinearSum"): 151.0e6,
+    ("kinsol-newton", "N_VScale"): 100.7e6,
+    ("kinsol-newton", "k_getrs (6x6 blocks)"): 436.3e6,
+    ("kinsol-newton", "k_getrf (6x6 blocks)"): 629.2e6,
+    ("kinsol", "N_VLinearSum"): 151.0e6,
+    ("kinsol", "N_VScale"): 100.7e6,
+    ("imex", "N_VLinearSum"): 151.0e6,
+    ("imex", "N_VScale"): 100.7e6,
+}
+
+
+def stream_traffic(workload, kernel, n):
+    """Recorded HBM bytes per launch, only at the size they were measured at."""
+    return MEASURED_STREAM_TRAFFIC.get((workload, kernel)) if n == 6291456 else None
+
+
 MEASURED_TRAFFIC = {
     "k_batched_bdf<4, float, 2, false>": 8.66e6,
     "k_batched_bdf<8, float, 2, false>": 12.28e6,
@@ -433,7 +453,8 @@ def bench_imex(args):
     achieved = dk["bytes_per_launch"] / (dk["avg_us"] * 1e-6) / 1e9
     roofline = {"bound": "hbm", "kernel": dk["name"], "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "bytes_per_launch": dk["bytes_per_launch"],
+                "traffic": stream_traffic("imex", dk["name"], N),
+                "bytes_per_launch": dk["bytes_per_launch"],
                 "avg_launch_us": round(dk["avg_us"], 2)}
 
     cpu = None
@@ -547,7 +568,8 @@ def bench_kinsol(args):
     achieved = dk["bytes_per_launch"] / (dk["avg_us"] * 1e-6) / 1e9
     roofline = {"bound": "hbm", "kernel": dk["name"], "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "bytes_per_launch": dk["bytes_per_launch"],
+                "traffic": stream_traffic("kinsol", dk["name"], N),
+                "bytes_per_launch": dk["bytes_per_launch"],
                 "avg_launch_us": round(dk["avg_us"], 2)}
 
     cpu = None
@@ -662,7 +684,8 @@ def bench_kinsol_newton(args):
     achieved = dk["bytes_per_launch"] / (dk["avg_us"] * 1e-6) / 1e9
     roofline = {"bound": "hbm", "kernel": dk["name"], "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "bytes_per_launch": dk["bytes_per_launch"],
+                "traffic": stream_traffic("kinsol-newton", dk["name"], N),
+                "bytes_per_launch": dk["bytes_per_launch"],
                 "avg_launch_us": round(dk["avg_us"], 2)}
 
     cpu = None
@@ -827,7 +850,8 @@ def bench_batched(args):
                 "frac": round(achieved / FP64_PEAK_TFS, 5),
                 "traffic": MEASURED_TRAFFIC.get(
                     "k_batched_bdf<%d, float, %d, %s>" % (bm[dom], w32[dom],
-                                                         "true" if learn_dom else "false")),
+                                                         "true" if learn_dom else "false"))
+                if args.systems == 65536 else None,
                 "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this benchmark, "
                                   "scaled by a 256 MiB copy (x2.000 / x1.000); null for a "
                                   "kernel not measured",
