@@ -523,6 +523,7 @@ def bench_kinsol(args):
         o.ncells, o.variant, o.perturb, o.tol, o.maxiter = nc, 1, 1, 1e-11, 100
         o.m_aa, o.delay_aa, o.orth_aa = 5, 0, 2
         o.damping_fp, o.damping_aa, o.damping_fn, o.depth_fn, o.sequential = 1.0, 1.0, 0, 0, seq
+        o.fused = 1
         return o
 
     PD = C.POINTER(C.c_double)
@@ -600,7 +601,7 @@ def bench_kinsol(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "KINSol KIN_FP, Anderson depth 5 (CGS2), the analytic 3-equation "
+        "config": {"workload": "KINSol KIN_FP, Anderson depth 5 (CGS2), fused scaled norm, the analytic 3-equation "
                                "map (IEEE-only variant) on %d independent cells, tol 1e-11"
                                % ncells, "unknowns": N},
         "solver": {"nni": st["nni"], "nfe": st["nfe"], "fnorm": st["fnorm"]},

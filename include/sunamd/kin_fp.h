@@ -33,6 +33,7 @@ typedef struct
   int damping_fn;     /* attach the damping function */
   int depth_fn;       /* attach the depth function */
   int sequential;     /* MI355X vector only: reductions in serial order */
+  int fused;          /* MI355X vector only: KINSetUseFusedKernels (scaled norm) */
 } SUNAMD_KinOptions;
 
 /* stats (doubles): return flag, nni, nfe, fnorm, wall seconds */

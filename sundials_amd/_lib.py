@@ -119,6 +119,7 @@ class KinOptions(C.Structure):
         ("damping_fn", C.c_int),
         ("depth_fn", C.c_int),
         ("sequential", C.c_int),
+        ("fused", C.c_int),
     ]
 
 

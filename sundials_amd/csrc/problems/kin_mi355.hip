@@ -134,6 +134,7 @@ extern "C" int SUNAMD_KinSolve_Mi355(const SUNAMD_KinOptions* o, double* stats, 
     }
     if (o->damping_fn) { r |= KINSetDampingFn(kmem, cb_damping); }
     if (o->depth_fn) { r |= KINSetDepthFn(kmem, cb_depth); }
+    if (o->fused) { r |= KINSetUseFusedKernels(kmem, SUNTRUE); }
     ok = r == 0;
   }
   if (ok)

@@ -8,7 +8,8 @@ against the CPU oracle (the same restatement over the serial vector).
 - the IEEE-only map (variant 1) on 4096 perturbed cells with reductions in
   serial order: counts and every final value identical to the oracle bit for
   bit, for the plain, damped, accelerated (each QR update), delayed and
-  depth-function configurations;
+  depth-function configurations, with and without the fused scaled norm
+  (KINSetUseFusedKernels);
 - device-wide tree reductions on 2^18 cells: the iteration count within one
   of the oracle's, the solution to the tolerance.
 """
@@ -53,10 +54,11 @@ CONFIGS = [dict(), dict(damping_fp=0.5), dict(m_aa=3), dict(m_aa=3, orth_aa=1),
 
 @pytest.mark.parametrize("cfg", CONFIGS, ids=lambda c: ",".join("%s=%s" % kv for kv in c.items())
                          or "plain")
-def test_kin_cells_sequential_bitwise_vs_oracle(cfg):
+@pytest.mark.parametrize("fused", [0, 1])
+def test_kin_cells_sequential_bitwise_vs_oracle(cfg, fused):
     o = options(ncells=4096, perturb=1, variant=1, tol=1e-11, maxiter=60, **cfg)
     rc_o, st_o, u_o = run_oracle(o)
-    o.sequential = 1
+    o.sequential, o.fused = 1, fused
     rc_d, st_d, u_d = run_device(o)
     assert rc_o == rc_d
     assert (st_d["nni"], st_d["nfe"]) == (st_o["nni"], st_o["nfe"])
