@@ -6,12 +6,14 @@
  * Public names, argument meaning and return codes follow the reference
  * ARKODE API (include/arkode/arkode.h:188-295, include/arkode/arkode_arkstep.h,
  * include/arkode/arkode_ls.h) so a driver written against ARKStep runs
- * unchanged.  Provided: the ARK3(2)4L[2]SA IMEX pair (order 3, embedding 2),
- * adaptive or fixed steps with the I, PI or PID controller, the trivial
- * stage predictor, linearly implicit and autonomous problem options, Newton iterations through any SUNNonlinearSolver's
- * callbacks and a matrix-free iterative linear solver (SPGMR) with a user
- * preconditioner and Jacobian-vector product, stop time, Hermite or no
- * interpolation bookkeeping (no dense output).  Mass matrices, relaxation,
+ * unchanged.  Provided: the IMEX pairs ARK4(3)7L[2]SA (order 4, the
+ * reference's default) and ARK3(2)4L[2]SA (order 3), adaptive or fixed steps
+ * with the I, PI or PID controller, the trivial stage predictor, linearly
+ * implicit and autonomous problem options, Newton iterations through any
+ * SUNNonlinearSolver's callbacks and a matrix-free iterative linear solver
+ * (SPGMR) with a user preconditioner and a user or difference-quotient
+ * Jacobian-vector product, ARK_NORMAL / ARK_ONE_STEP, stop time, Hermite or
+ * no interpolation bookkeeping (no dense output).  Mass matrices, relaxation,
  * rootfinding, constraints, MRI forcing, adjoints and other tables are not
  * part of this library.
  * ---------------------------------------------------------------------------*/
@@ -125,12 +127,22 @@ int ARKodeGetNumLinConvFails(void* arkode_mem, long int* nlcfails);
 int ARKodeGetNumPrecEvals(void* arkode_mem, long int* npevals);
 int ARKodeGetNumPrecSolves(void* arkode_mem, long int* npsolves);
 int ARKodeGetNumJtimesEvals(void* arkode_mem, long int* njvevals);
+/* fi evaluations of the difference-quotient Jv (arkode_ls.c:1299) */
+int ARKodeGetNumLinRhsEvals(void* arkode_mem, long int* nfevalsLS);
 int ARKodeGetLastStep(void* arkode_mem, sunrealtype* hlast);
+int ARKodeGetActualInitStep(void* arkode_mem, sunrealtype* hinused);
+int ARKodeGetCurrentStep(void* arkode_mem, sunrealtype* hcur);
 int ARKodeGetCurrentTime(void* arkode_mem, sunrealtype* tcur);
 
-/* the ARK3(2)4L[2]SA coefficients (ARKodeButcherTable_LoadERK/LoadDIRK,
-   arkode_butcher_erk.def:218 / arkode_butcher_dirk.def:248): implicit_part 0
-   the explicit table, 1 the ESDIRK; A is 4x4 row-major */
+/* the pair of a method order (ARKodeButcherTable_LoadERK / LoadDIRK of the
+   ARKSTEP_DEFAULT_ARK_*TABLE_q pair): order 3 ARK3(2)4L[2]SA
+   (arkode_butcher_erk.def:218 / arkode_butcher_dirk.def:248), order 4 or
+   <= 0 ARK4(3)7L[2]SA (arkode_butcher_erk.def:371 / arkode_butcher_dirk.def:474).
+   implicit_part 0 the explicit table, 1 the ESDIRK; s stages, A (s x s
+   row-major, s <= 7), b, c, d, order q, embedding order p */
+int SUNAMD_ARKTable(int order, int implicit_part, int* s, double* A, double* b, double* c,
+                    double* d, int* q, int* p);
+/* the order-3 pair with A 4x4 row-major */
 int SUNAMD_ARK324Table(int implicit_part, double* A, double* b, double* c, double* d, int* q,
                        int* p);
 

@@ -231,10 +231,10 @@ static int d_jv(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y, N_Vector fy,
 }
 
 /* flags: 1 linearly implicit (not time dependent), 2 autonomous, 4 no
-   interpolant (else Hermite) */
+   interpolant (else Hermite); order selects the pair (3 or 4) */
 int oracle_ark_dahlquist(double lam_e, double lam_i, double h, double tf, double rtol,
-                         double atol, const char* controller, int flags, double* stats,
-                         double* yout)
+                         double atol, const char* controller, int flags, int order,
+                         double* stats, double* yout)
 {
   Dahl D         = {lam_e, lam_i};
   SUNContext ctx = NULL;
@@ -247,7 +247,7 @@ int oracle_ark_dahlquist(double lam_e, double lam_i, double h, double tf, double
   int ok = ark && L && ARKodeSStolerances(ark, rtol, atol) == 0 &&
            ARKodeSetUserData(ark, &D) == 0 && ARKodeSetLinearSolver(ark, L, NULL) == 0 &&
            ARKodeSetJacTimes(ark, NULL, d_jv) == 0 && ARKodeSetStopTime(ark, tf) == 0 &&
-           ARKodeSetMaxNumSteps(ark, 1000000) == 0;
+           ARKodeSetMaxNumSteps(ark, 1000000) == 0 && ARKodeSetOrder(ark, order) == 0;
   if (ok && h > 0.0) { ok = ARKodeSetFixedStep(ark, h) == 0; }
   if (ok && controller) { ok = ARKodeSetAdaptControllerByName(ark, controller) == 0; }
   if (ok && (flags & 1)) { ok = ARKodeSetLinear(ark, 0) == 0; }
@@ -274,6 +274,110 @@ int oracle_ark_dahlquist(double lam_e, double lam_i, double h, double tf, double
     stats[15] = hl;
     if (rc == ARK_TSTOP_RETURN) { rc = 0; }
     yout[0] = N_VGetArrayPointer(y)[0];
+  }
+  if (ark) { ARKodeFree(&ark); }
+  if (L) { SUNLinSolFree(L); }
+  N_VDestroy(y);
+  SUNContext_Free(&ctx);
+  return rc;
+}
+
+/* ---------------------------------------------------------------------------
+ * Kvaerno-Prothero-Robinson problem (test/unit_tests/problems/kpr.hpp) as the
+ * reference's ARKStep logging test drives it (test_logging_arkode_arkstep.cpp,
+ * arguments "2 1 0": IMEX split, Newton, SPGMR without preconditioning,
+ * maxl default, difference-quotient Jv, rtol 1e-6, atol 1e-10, three calls in
+ * ARK_ONE_STEP mode).  u' = a g1 + b g2 + rho'/(2u), v' = c g1 + d g2 +
+ * sigma'/(2v) with g1 = (-1 + u^2 - rho)/(2u), g2 = (-2 + v^2 - sigma)/(2v),
+ * rho = cos(t)/2, sigma = cos(20 t); (a, b, c, d) = (-2, 1/2, 1/2, -1); the
+ * rho'/sigma' terms are explicit, the rest implicit.  Exact solution
+ * u = sqrt(1 + rho), v = sqrt(2 + sigma).
+ * out[3k..3k+2] = (t, u, v) after k calls (k = 0 the initial state);
+ * stats: steps, attempts, error-test fails, NLS step fails, h0, last h,
+ * current h, fe, fi, NLS iters, NLS fails, LS setups, LS RHS evals, LS iters,
+ * LS fails, Jv evals, preconditioner setups, preconditioner solves.
+ * ---------------------------------------------------------------------------*/
+static const double KPR_ABCD[4] = {-2.0, 0.5, 0.5, -1.0};
+
+static double kpr_rho(double t) { return 0.5 * cos(t); }
+static double kpr_rho_p(double t) { return -0.5 * sin(t); }
+static double kpr_sig(double t) { return cos(20.0 * t); }
+static double kpr_sig_p(double t) { return -20.0 * sin(20.0 * t); }
+
+static int kpr_fe(sunrealtype t, N_Vector y, N_Vector f, void* ud)
+{
+  (void)ud;
+  const double* Y = N_VGetArrayPointer(y);
+  double* F       = N_VGetArrayPointer(f);
+  F[0]            = kpr_rho_p(t) / (2.0 * Y[0]);
+  F[1]            = kpr_sig_p(t) / (2.0 * Y[1]);
+  return 0;
+}
+
+static int kpr_fi(sunrealtype t, N_Vector y, N_Vector f, void* ud)
+{
+  const double* k = (const double*)ud;
+  const double* Y = N_VGetArrayPointer(y);
+  double* F       = N_VGetArrayPointer(f);
+  const double u = Y[0], v = Y[1];
+  const double g1 = (-1.0 + u * u - kpr_rho(t)) / (2.0 * u);
+  const double g2 = (-2.0 + v * v - kpr_sig(t)) / (2.0 * v);
+  F[0]            = k[0] * g1 + k[1] * g2;
+  F[1]            = k[2] * g1 + k[3] * g2;
+  return 0;
+}
+
+int oracle_ark_kpr(int ncalls, double* out, double* stats)
+{
+  SUNContext ctx = NULL;
+  if (SUNContext_Create(SUN_COMM_NULL, &ctx)) { return -1; }
+  N_Vector y = N_VNew_Serial(2, ctx);
+  double* Y  = N_VGetArrayPointer(y);
+  Y[0]       = sqrt(1.0 + kpr_rho(0.0));
+  Y[1]       = sqrt(2.0 + kpr_sig(0.0));
+  out[0]     = 0.0;
+  out[1]     = Y[0];
+  out[2]     = Y[1];
+  void* ark         = ARKStepCreate(kpr_fe, kpr_fi, 0.0, y, ctx);
+  SUNLinearSolver L = SUNLinSol_SPGMR(y, SUN_PREC_NONE, 0, ctx);
+  int rc            = -1;
+  int ok = ark && L && ARKodeSetUserData(ark, (void*)KPR_ABCD) == 0 &&
+           ARKodeSStolerances(ark, 1.0e-6, 1.0e-10) == 0 &&
+           ARKodeSetLinearSolver(ark, L, NULL) == 0;
+  if (ok)
+  {
+    double tret = 0.0, tout = 1.0;
+    for (int k = 1; k <= ncalls; k++)
+    {
+      rc = ARKodeEvolve(ark, tout, y, &tret, ARK_ONE_STEP);
+      if (rc < 0) { break; }
+      out[3 * k]     = tret;
+      out[3 * k + 1] = Y[0];
+      out[3 * k + 2] = Y[1];
+      tout += 1.0;
+    }
+    long v[15] = {0};
+    double h[3] = {0.0, 0.0, 0.0};
+    ARKodeGetNumSteps(ark, &v[0]);
+    ARKodeGetNumStepAttempts(ark, &v[1]);
+    ARKodeGetNumErrTestFails(ark, &v[2]);
+    ARKodeGetNumConvFails(ark, &v[3]);
+    ARKodeGetActualInitStep(ark, &h[0]);
+    ARKodeGetLastStep(ark, &h[1]);
+    ARKodeGetCurrentStep(ark, &h[2]);
+    ARKStepGetNumRhsEvals(ark, &v[4], &v[5]);
+    ARKodeGetNumNonlinSolvIters(ark, &v[6]);
+    ARKodeGetNumNonlinSolvConvFails(ark, &v[7]);
+    ARKodeGetNumLinSolvSetups(ark, &v[8]);
+    ARKodeGetNumLinRhsEvals(ark, &v[9]);
+    ARKodeGetNumLinIters(ark, &v[10]);
+    ARKodeGetNumLinConvFails(ark, &v[11]);
+    ARKodeGetNumJtimesEvals(ark, &v[12]);
+    ARKodeGetNumPrecEvals(ark, &v[13]);
+    ARKodeGetNumPrecSolves(ark, &v[14]);
+    const double sv[18] = {v[0], v[1], v[2], v[3], h[0], h[1], h[2], v[4], v[5],
+                           v[6], v[7], v[8], v[9], v[10], v[11], v[12], v[13], v[14]};
+    memcpy(stats, sv, sizeof(sv));
   }
   if (ark) { ARKodeFree(&ark); }
   if (L) { SUNLinSolFree(L); }

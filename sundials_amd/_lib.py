@@ -265,6 +265,8 @@ def core():
     _sig(lib, "SUNAMD_Pow", c_real, [c_real, c_real])
     _sig(lib, "SUNAMD_AdrCreate", P(Adr), [C.c_int, C.c_uint64])
     _sig(lib, "SUNAMD_AdrFree", None, [P(Adr)])
+    _sig(lib, "SUNAMD_ARKTable", C.c_int, [C.c_int, C.c_int, P(C.c_int)] + [P(c_real)] * 4 +
+         [P(C.c_int), P(C.c_int)])
     _sig(lib, "SUNAMD_ARK324Table", C.c_int, [C.c_int, P(c_real), P(c_real), P(c_real),
                                                  P(c_real), P(C.c_int), P(C.c_int)])
     _sig(lib, "SUNAMD_PowExact", c_real, [c_real, c_real])
@@ -392,7 +394,9 @@ def load_oracle():
     _sig(lib, "oracle_kin_newton_solve", C.c_int, [P(KinNewtonOptions), P(c_real), P(c_real)])
     _sig(lib, "oracle_kin_fertron_example", C.c_int, [P(c_real), P(c_real)])
     _sig(lib, "oracle_ark_dahlquist", C.c_int, [c_real, c_real, c_real, c_real, c_real, c_real,
-                                                C.c_char_p, C.c_int, P(c_real), P(c_real)])
+                                                C.c_char_p, C.c_int, C.c_int, P(c_real),
+                                                P(c_real)])
+    _sig(lib, "oracle_ark_kpr", C.c_int, [C.c_int, P(c_real), P(c_real)])
     _sig(lib, "oracle_batch_solve", C.c_int,
          [P(Batch), c_real, c_real, c_real, C.c_int, P(c_real), P(C.c_long), P(c_real)])
     return lib

@@ -101,7 +101,8 @@
 #define ARK_FULLRHS_END   1
 #define ARK_FULLRHS_OTHER 2
 
-#define NSTAGES 4
+/* largest stage count of the tables provided here (ARK4(3)7L[2]SA) */
+#define NSTAGES 7
 
 typedef struct
 {
@@ -165,7 +166,11 @@ typedef struct ARKodeMemRec
   sunrealtype ls_tcur, eplifac, nrmfac;
   N_Vector ls_ycur, ls_fcur, ytemp, x;
   sunbooleantype jbad;
-  long int msbj, nstlj, nje, npe, nli, nps, ncfl, njtsetup, njtimes;
+  /* no matrix and no preconditioner setup: the Newton iteration never calls
+     a linear-solver setup (arkLsInitialize, arkode_ls.c:3133-3139) */
+  sunbooleantype lsetup_disabled;
+  long int msbj, nstlj, nje, npe, nli, nps, ncfl, njtsetup, njtimes, nfeDQ;
+  sunbooleantype jtimesDQ;
   int ls_last_flag;
 } ARKodeMemRec;
 
@@ -213,8 +218,115 @@ static void ark324_tables(ButcherTable* Be, ButcherTable* Bi)
   Bi->A[3][3] = g;
 }
 
+/* ---------------------------------------------------------------------------
+ * ARK4(3)7L[2]SA pair, the reference's default IMEX method of order 4
+ * (ARKODE_ARK437L2SA_ERK_7_3_4 / _DIRK_7_3_4, arkode_butcher_erk.def:371 and
+ * arkode_butcher_dirk.def:474; Kennedy & Carpenter 2019).  The coefficients
+ * are ratios of integers, each formed by one correctly rounded division.
+ * -------------------------------------------------------------------------*/
+#define R(p, q) ((sunrealtype)(p) / (sunrealtype)(q))
+static void ark437_tables(ButcherTable* Be, ButcherTable* Bi)
+{
+  memset(Be, 0, sizeof(*Be));
+  memset(Bi, 0, sizeof(*Bi));
+  Be->s = Bi->s = 7;
+  Be->q = Bi->q = 4;
+  Be->p = Bi->p = 3;
+  /* shared weights, embedding and abscissae (b_0 = b_1 = d_0 = d_1 = c_0 = 0) */
+  const sunrealtype b[7] = {0.0, 0.0, R(9164257142617.0, 17756377923965.0),
+                            R(-10812980402763.0, 74029279521829.0),
+                            R(1335994250573.0, 5691609445217.0),
+                            R(2273837961795.0, 8368240463276.0), R(247.0, 2000.0)};
+  const sunrealtype d[7] = {0.0, 0.0, R(4469248916618.0, 8635866897933.0),
+                            R(-621260224600.0, 4094290005349.0),
+                            R(696572312987.0, 2942599194819.0),
+                            R(1532940081127.0, 5565293938103.0), R(2441.0, 20000.0)};
+  const sunrealtype c[7] = {0.0, R(247.0, 1000.0), R(4276536705230.0, 10142255878289.0),
+                            R(67.0, 200.0), R(3.0, 40.0), R(7.0, 10.0), 1.0};
+  for (int j = 0; j < 7; j++)
+  {
+    Be->b[j] = b[j];
+    Be->d[j] = d[j];
+    Be->c[j] = Bi->c[j] = c[j];
+  }
+  /* explicit table */
+  Be->A[1][0] = R(247.0, 1000.0);
+  Be->A[2][0] = R(247.0, 4000.0);
+  Be->A[2][1] = R(2694949928731.0, 7487940209513.0);
+  Be->A[3][0] = R(464650059369.0, 8764239774964.0);
+  Be->A[3][1] = R(878889893998.0, 2444806327765.0);
+  Be->A[3][2] = R(-952945855348.0, 12294611323341.0);
+  Be->A[4][0] = R(476636172619.0, 8159180917465.0);
+  Be->A[4][1] = R(-1271469283451.0, 7793814740893.0);
+  Be->A[4][2] = R(-859560642026.0, 4356155882851.0);
+  Be->A[4][3] = R(1723805262919.0, 4571918432560.0);
+  Be->A[5][0] = R(6338158500785.0, 11769362343261.0);
+  Be->A[5][1] = R(-4970555480458.0, 10924838743837.0);
+  Be->A[5][2] = R(3326578051521.0, 2647936831840.0);
+  Be->A[5][3] = R(-880713585975.0, 1841400956686.0);
+  Be->A[5][4] = R(-1428733748635.0, 8843423958496.0);
+  Be->A[6][0] = R(760814592956.0, 3276306540349.0);
+  Be->A[6][1] = R(760814592956.0, 3276306540349.0);
+  Be->A[6][2] = R(-47223648122716.0, 6934462133451.0);
+  Be->A[6][3] = R(71187472546993.0, 9669769126921.0);
+  Be->A[6][4] = R(-13330509492149.0, 9695768672337.0);
+  Be->A[6][5] = R(11565764226357.0, 8513123442827.0);
+  /* ESDIRK table: diagonal gamma = 0.1235, last row = b (stiffly accurate);
+     its b_6 is gamma, not the explicit table's 247/2000 (the same number) */
+  const sunrealtype g = R(1235.0, 10000.0);
+  for (int i = 1; i < 7; i++) { Bi->A[i][i] = g; }
+  Bi->A[1][0] = g;
+  Bi->A[2][0] = Bi->A[2][1] = R(624185399699.0, 4186980696204.0);
+  Bi->A[3][0] = Bi->A[3][1] = R(1258591069120.0, 10082082980243.0);
+  Bi->A[3][2] = R(-322722984531.0, 8455138723562.0);
+  Bi->A[4][0] = Bi->A[4][1] = R(-436103496990.0, 5971407786587.0);
+  Bi->A[4][2] = R(-2689175662187.0, 11046760208243.0);
+  Bi->A[4][3] = R(4431412449334.0, 12995360898505.0);
+  Bi->A[5][0] = Bi->A[5][1] = R(-2207373168298.0, 14430576638973.0);
+  Bi->A[5][2] = R(242511121179.0, 3358618340039.0);
+  Bi->A[5][3] = R(3145666661981.0, 7780404714551.0);
+  Bi->A[5][4] = R(5882073923981.0, 14490790706663.0);
+  for (int j = 0; j < 7; j++)
+  {
+    Bi->b[j] = (j == 6) ? g : b[j];
+    Bi->d[j] = d[j];
+    if (j < 6) { Bi->A[6][j] = b[j]; }
+  }
+}
+#undef R
+
 /* the pair's coefficients for inspection (ARKodeButcherTable_LoadERK /
    LoadDIRK of ARKODE_ARK324L2SA_*_4_2_3): A (4x4 row-major), b, c, d */
+/* tables for a method order: 3 gives ARK3(2)4L[2]SA, 4 (and the default,
+   order <= 0) ARK4(3)7L[2]SA, as arkStep_SetButcherTables selects the
+   ARKSTEP_DEFAULT_ARK_*TABLE_q pair (arkode_arkstep.c:2216-2262) */
+static int load_tables(int order, ButcherTable* Be, ButcherTable* Bi)
+{
+  if (order <= 0 || order == 4) { ark437_tables(Be, Bi); }
+  else if (order == 3) { ark324_tables(Be, Bi); }
+  else { return -1; }
+  return 0;
+}
+
+int SUNAMD_ARKTable(int order, int implicit_part, int* s, double* A, double* b, double* c,
+                    double* d, int* q, int* p)
+{
+  ButcherTable Be, Bi;
+  if (load_tables(order, &Be, &Bi) != 0) { return -1; }
+  const ButcherTable* B = implicit_part ? &Bi : &Be;
+  *s = B->s;
+  for (int i = 0; i < B->s; i++)
+  {
+    for (int j = 0; j < B->s; j++) { A[B->s * i + j] = B->A[i][j]; }
+    b[i] = B->b[i];
+    c[i] = B->c[i];
+    d[i] = B->d[i];
+  }
+  *q = B->q;
+  *p = B->p;
+  return 0;
+}
+
 int SUNAMD_ARK324Table(int implicit_part, double* A, double* b, double* c, double* d, int* q,
                        int* p)
 {
@@ -454,8 +566,8 @@ int ARKodeSetUserData(void* arkode_mem, void* user_data)
 int ARKodeSetOrder(void* arkode_mem, int ord)
 {
   ARKMEM_OR_RETURN();
-  /* only the order-3 ARK pair is provided */
-  if (ord > 0 && ord != 3) { return ARK_ILL_INPUT; }
+  /* the order-3 and order-4 ARK pairs are provided (order <= 0: default 4) */
+  if (ord > 0 && ord != 3 && ord != 4) { return ARK_ILL_INPUT; }
   ark->user_order = ord;
   return ARK_SUCCESS;
 }
@@ -593,9 +705,10 @@ int ARKodeSetLinearSolver(void* arkode_mem, SUNLinearSolver LS, SUNMatrix A)
   ark->LS      = LS;
   ark->pset    = NULL;
   ark->psolve  = NULL;
-  ark->jtsetup = NULL;
-  ark->jtimes  = NULL;
-  ark->msbj    = ARKLS_MSBJ;
+  ark->jtsetup  = NULL;
+  ark->jtimes   = NULL;
+  ark->jtimesDQ = SUNTRUE;
+  ark->msbj     = ARKLS_MSBJ;
   ark->eplifac = ARKLS_EPLIN;
   if (SUNLinSolSetATimes(LS, ark, arkLsATimes) != SUN_SUCCESS) { return ARKLS_ILL_INPUT; }
   if (SUNLinSolSetPreconditioner(LS, ark, NULL, NULL) != SUN_SUCCESS) { return ARKLS_ILL_INPUT; }
@@ -626,10 +739,10 @@ int ARKodeSetJacTimes(void* arkode_mem, ARKLsJacTimesSetupFn jtsetup,
 {
   ARKMEM_OR_RETURN();
   if (ark->LS == NULL) { return ARKLS_LMEM_NULL; }
-  /* the difference-quotient product is not part of this library */
-  if (jtimes == NULL) { return ARKLS_ILL_INPUT; }
-  ark->jtsetup = jtsetup;
-  ark->jtimes  = jtimes;
+  /* a NULL product selects the difference quotient (arkode_ls.c:929-944) */
+  ark->jtimesDQ = (jtimes == NULL);
+  ark->jtsetup  = (jtimes == NULL) ? NULL : jtsetup;
+  ark->jtimes   = jtimes;
   return ARKLS_SUCCESS;
 }
 
@@ -813,7 +926,8 @@ static int nls_init(ARKodeMem ark)
   ark->nls_fails = 0;
   if (SUNNonlinSolSetConvTestFn(ark->NLS, nls_convtest, ark) != SUN_SUCCESS) { return ARK_NLS_INIT_FAIL; }
   if (SUNNonlinSolSetMaxIters(ark->NLS, ark->maxcor) != SUN_SUCCESS) { return ARK_NLS_INIT_FAIL; }
-  if (SUNNonlinSolSetLSetupFn(ark->NLS, ark->LS ? nls_lsetup : NULL) != SUN_SUCCESS) { return ARK_NLS_INIT_FAIL; }
+  const sunbooleantype lsetup = ark->LS && !ark->lsetup_disabled;
+  if (SUNNonlinSolSetLSetupFn(ark->NLS, lsetup ? nls_lsetup : NULL) != SUN_SUCCESS) { return ARK_NLS_INIT_FAIL; }
   if (SUNNonlinSolSetLSolveFn(ark->NLS, ark->LS ? nls_lsolve : NULL) != SUN_SUCCESS) { return ARK_NLS_INIT_FAIL; }
   if (SUNNonlinSolSetSysFn(ark->NLS, nls_residual) != SUN_SUCCESS) { return ARK_NLS_INIT_FAIL; }
   if (SUNNonlinSolInitialize(ark->NLS) != SUN_SUCCESS) { return ARK_NLS_INIT_FAIL; }
@@ -825,7 +939,7 @@ static int stage_nls(ARKodeMem ark, int nflag)
 {
   sunbooleantype callLSetup;
   long int nls_iters_inc = 0, nls_fails_inc = 0;
-  if (ark->LS)
+  if (ark->LS && !ark->lsetup_disabled)
   {
     if (ark->linear)
     {
@@ -943,11 +1057,36 @@ static int ls_solve(ARKodeMem ark, N_Vector b, sunrealtype tnow, N_Vector ynow,
   }
 }
 
+/* arkLsDQJtimes (arkode_ls.c:2842-2880): Jv ~ [fi(t, y + sig v) - fi(t, y)] / sig
+   with sig = 1 / ||v||_WRMS, shrunk by 1/4 while fi fails recoverably */
+#define MAX_DQITERS 3
+static int dq_jtimes(ARKodeMem ark, N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y,
+                     N_Vector fy, N_Vector work)
+{
+  sunrealtype sig = ONE / N_VWrmsNorm(v, ark->ewt);
+  int retval      = 0;
+  for (int iter = 0; iter < MAX_DQITERS; iter++)
+  {
+    N_VLinearSum(sig, v, ONE, y, work);
+    retval = ark->fi(t, work, Jv, ark->user_data);
+    ark->nfeDQ++;
+    if (retval == 0) { break; }
+    if (retval < 0) { return -1; }
+    sig *= 0.25;
+  }
+  if (retval > 0) { return +1; }
+  const sunrealtype siginv = ONE / sig;
+  N_VLinearSum(siginv, Jv, -siginv, fy, Jv);
+  return 0;
+}
+
 static int arkLsATimes(void* arkode_mem, N_Vector v, N_Vector z)
 {
   ARKodeMem ark = (ARKodeMem)arkode_mem;
-  int retval    = ark->jtimes(v, z, ark->ls_tcur, ark->ls_ycur, ark->ls_fcur, ark->user_data,
-                              ark->ytemp);
+  int retval    = ark->jtimesDQ
+                    ? dq_jtimes(ark, v, z, ark->ls_tcur, ark->ls_ycur, ark->ls_fcur, ark->ytemp)
+                    : ark->jtimes(v, z, ark->ls_tcur, ark->ls_ycur, ark->ls_fcur,
+                                  ark->user_data, ark->ytemp);
   ark->njtimes++;
   if (retval != 0) { return retval; }
   N_VLinearSum(ONE, v, -ark->gamma, z, z);
@@ -1235,7 +1374,7 @@ static int initial_setup(ARKodeMem ark, sunrealtype tout)
   }
   if (ewt_set(ark, ark->yn, ark->ewt) != 0) { return ARK_ILL_INPUT; }
   /* arkStep_Init (FIRST_INIT) */
-  ark324_tables(&ark->Be, &ark->Bi);
+  if (load_tables(ark->user_order, &ark->Be, &ark->Bi) != 0) { return ARK_ILL_INPUT; }
   ark->stages = ark->Bi.s;
   ark->q = ark->hq_order = ark->Bi.q;
   ark->p = ark->hp_order = ark->Bi.p;
@@ -1244,8 +1383,9 @@ static int initial_setup(ARKodeMem ark, sunrealtype tout)
   {
     /* arkLsInitialize: counters, then the solver */
     ark->nje = ark->npe = ark->nli = ark->nps = ark->ncfl = ark->njtsetup = ark->njtimes = 0;
+    ark->nfeDQ = 0;
     ark->nstlj = 0;
-    if (ark->jtimes == NULL) { return ARK_LINIT_FAIL; }
+    ark->lsetup_disabled = (ark->pset == NULL);
     if (SUNLinSolInitialize(ark->LS) != SUN_SUCCESS) { return ARK_LINIT_FAIL; }
   }
   if (ark->NLS)
@@ -1605,7 +1745,10 @@ GETTER(ARKodeGetNumLinConvFails, long int, ark->ncfl)
 GETTER(ARKodeGetNumPrecEvals, long int, ark->npe)
 GETTER(ARKodeGetNumPrecSolves, long int, ark->nps)
 GETTER(ARKodeGetNumJtimesEvals, long int, ark->njtimes)
+GETTER(ARKodeGetNumLinRhsEvals, long int, ark->nfeDQ)
 GETTER(ARKodeGetLastStep, sunrealtype, ark->hold)
+GETTER(ARKodeGetActualInitStep, sunrealtype, ark->h0u)
+GETTER(ARKodeGetCurrentStep, sunrealtype, ark->next_h)
 GETTER(ARKodeGetCurrentTime, sunrealtype, ark->tcur)
 
 int ARKStepGetNumRhsEvals(void* arkode_mem, long int* nfe_evals, long int* nfi_evals)

@@ -39,23 +39,39 @@ def run_device(p, o):
     return _run(_lib.hip().SUNAMD_AdrSolve_Mi355, p, o)
 
 
-def dahlquist(lam_e, lam_i, h, tf, rtol=1e-10, atol=1e-14, controller=None, flags=0):
+def dahlquist(lam_e, lam_i, h, tf, rtol=1e-10, atol=1e-14, controller=None, flags=0, order=3):
     st = np.zeros(_lib.ADR_STATS)
     y = np.zeros(1)
     rc = _lib.load_oracle().oracle_ark_dahlquist(
         lam_e, lam_i, h, tf, rtol, atol, controller.encode() if controller else None, flags,
-        st.ctypes.data_as(PD), y.ctypes.data_as(PD))
+        order, st.ctypes.data_as(PD), y.ctypes.data_as(PD))
     return rc, dict(zip(_lib.ADR_STAT_NAMES, st)), y[0]
 
 
-def tables():
+def tables(order=3):
     core = _lib.core()
     out = {}
     for part, name in ((0, "E"), (1, "I")):
-        A, b, c, d = np.zeros(16), np.zeros(4), np.zeros(4), np.zeros(4)
-        q, pp = C.c_int(), C.c_int()
-        core.SUNAMD_ARK324Table(part, A.ctypes.data_as(PD), b.ctypes.data_as(PD),
-                                c.ctypes.data_as(PD), d.ctypes.data_as(PD), C.byref(q),
-                                C.byref(pp))
-        out[name] = (A.reshape(4, 4), b, c, d, q.value, pp.value)
+        A, b, c, d = np.zeros(49), np.zeros(7), np.zeros(7), np.zeros(7)
+        ns, q, pp = C.c_int(), C.c_int(), C.c_int()
+        assert core.SUNAMD_ARKTable(order, part, C.byref(ns), A.ctypes.data_as(PD),
+                                    b.ctypes.data_as(PD), c.ctypes.data_as(PD),
+                                    d.ctypes.data_as(PD), C.byref(q), C.byref(pp)) == 0
+        s = ns.value
+        out[name] = (A[:s * s].reshape(s, s), b[:s], c[:s], d[:s], q.value, pp.value)
     return out
+
+
+def kpr(ncalls=3):
+    """The reference's ARKStep logging test on the KPR problem (IMEX,
+    Newton, SPGMR, difference-quotient Jv, ARK_ONE_STEP) run on the oracle:
+    rows (t, u, v) and the statistics named as ARKodePrintAllStats prints
+    them."""
+    out, st = np.zeros(3 * (ncalls + 1)), np.zeros(18)
+    rc = _lib.load_oracle().oracle_ark_kpr(ncalls, out.ctypes.data_as(PD), st.ctypes.data_as(PD))
+    names = ["Steps", "Step attempts", "Error test fails", "NLS step fails",
+             "Initial step size", "Last step size", "Current step size",
+             "Explicit RHS fn evals", "Implicit RHS fn evals", "NLS iters", "NLS fails",
+             "LS setups", "LS RHS fn evals", "LS iters", "LS fails", "Jac-times evals",
+             "Prec setup evals", "Prec solves"]
+    return rc, out.reshape(ncalls + 1, 3), dict(zip(names, st))

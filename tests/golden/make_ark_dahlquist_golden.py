@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
 """Writes ark_dahlquist_ark324.json: the per-step counts the reference's
-ARKStep Dahlquist unit test prints for the IMEX pair ARK3(2)4L[2]SA
+ARKStep Dahlquist unit test prints for the IMEX pairs ARK3(2)4L[2]SA (keys
+"hermite", "none") and ARK4(3)7L[2]SA (keys "ark437_hermite", "ark437_none")
 (test/unit_tests/arkode/CXX_serial/ark_test_dahlquist_ark_0_{0,-1}_0.out:
 identity mass matrix, Hermite or no interpolant, trivial predictor; the test
 sets a fixed step 0.01, lambda_e = lambda_i = -1, a linearly implicit
@@ -14,11 +15,10 @@ import re
 REF = "/root/reference/test/unit_tests/arkode/CXX_serial"
 
 
-def section(path):
+def section(path, dirk="ARKODE_ARK324L2SA_DIRK_4_2_3", erk="ARKODE_ARK324L2SA_ERK_4_2_3"):
     lines = open(path).read().split("\n")
     start = next(i for i, l in enumerate(lines)
-                 if l.strip() == "DIRK: ARKODE_ARK324L2SA_DIRK_4_2_3" and
-                 lines[i + 1].strip() == "ERK: ARKODE_ARK324L2SA_ERK_4_2_3")
+                 if l.strip() == "DIRK: " + dirk and lines[i + 1].strip() == "ERK: " + erk)
     steps, cur, which = [], None, None
     for l in lines[start + 2:]:
         if l.startswith("====") and cur is not None and steps:
@@ -52,6 +52,8 @@ def main():
     for tag, fname in (("hermite", "ark_test_dahlquist_ark_0_0_0.out"),
                        ("none", "ark_test_dahlquist_ark_0_-1_0.out")):
         out[tag] = section(os.path.join(REF, fname))
+        out["ark437_" + tag] = section(os.path.join(REF, fname), "ARKODE_ARK437L2SA_DIRK_7_3_4",
+                                       "ARKODE_ARK437L2SA_ERK_7_3_4")
     here = os.path.dirname(os.path.abspath(__file__))
     with open(os.path.join(here, "ark_dahlquist_ark324.json"), "w") as f:
         json.dump(out, f, indent=1)

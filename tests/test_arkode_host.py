@@ -15,16 +15,25 @@
   fi once more per Newton iteration), the stop time is hit exactly, and the
   adaptive solution's error against a 10^4x tighter run is of the order of
   the tolerance.
-Parity unpinned: the adaptive controller's step sequence on a problem with
-an output in the reference's own examples (those use a band or dense direct
-solver, which this library does not provide)."""
+- the ARK4(3)7L[2]SA pair (the reference's default order 4) satisfies the
+  order-4 conditions of both tables, every order-4 coupling condition and
+  the embedding's order-3 conditions, and reproduces the same Dahlquist
+  unit test's per-step counts;
+- the adaptive path is pinned by the reference's ARKStep logging test
+  (test/unit_tests/logging/test_logging_arkode_arkstep_2_1_0.out: KPR
+  problem, IMEX, Newton, SPGMR with the difference-quotient Jv, three
+  ARK_ONE_STEP calls): the initial step from arkHin, the three step times
+  and states (u, v) to all 16 printed digits, and every count of
+  ARKodePrintAllStats (fe, fi, NLS and linear iterations, DQ RHS
+  evaluations, setups, failures), so the I controller's exponent and safety
+  factor, arkHin's iteration and the stage loop are each fixed by it."""
 import json
 import os
 
 import numpy as np
 import pytest
 
-from arkode_util import dahlquist, tables, make_problem, options, run_oracle
+from arkode_util import dahlquist, kpr, tables, make_problem, options, run_oracle
 from sundials_amd import _lib
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -56,6 +65,33 @@ def test_ark324_order_conditions():
     assert AI[2, 2] == g and AI[3, 3] == g and np.array_equal(AI[3], b)
 
 
+def test_ark437_order_conditions():
+    T = tables(4)
+    AE, AI = T["E"][0], T["I"][0]
+    b, c, d = T["E"][1], T["E"][2], T["E"][3]
+    for name in ("E", "I"):
+        assert T[name][4:] == (4, 3) and T[name][0].shape == (7, 7)
+        assert np.array_equal(T[name][1], b) and np.array_equal(T[name][2], c)
+        assert np.array_equal(T[name][3], d)
+    tol = 1e-14
+    assert abs(b.sum() - 1) < tol and abs(b @ c - 0.5) < tol
+    assert abs(b @ c ** 2 - 1 / 3) < tol and abs(b @ c ** 3 - 0.25) < tol
+    assert abs(b @ c ** 4 - 0.2) > 1e-4  # exactly order 4
+    for A in (AE, AI):
+        assert np.allclose(A.sum(axis=1), c, atol=tol, rtol=0)
+        assert abs(b @ (A @ c) - 1 / 6) < tol
+        assert abs(b @ (c * (A @ c)) - 1 / 8) < tol
+        assert abs(b @ (A @ c ** 2) - 1 / 12) < tol
+        for A2 in (AE, AI):  # coupling conditions
+            assert abs(b @ (A @ (A2 @ c)) - 1 / 24) < tol
+        assert abs(d @ (A @ c) - 1 / 6) < tol
+    assert abs(d.sum() - 1) < tol and abs(d @ c - 0.5) < tol and abs(d @ c ** 2 - 1 / 3) < tol
+    assert np.all(np.triu(AE) == 0)
+    g = AI[1, 1]
+    assert AI[0, 0] == 0 and np.all(np.diag(AI)[1:] == g) and np.all(np.triu(AI, 1) == 0)
+    assert np.array_equal(AI[6], b)  # stiffly accurate
+
+
 def _stability(ze, zi):
     T = tables()
     AE, AI, b = T["E"][0], T["I"][0], T["E"][1]
@@ -83,17 +119,38 @@ def test_fixed_step_order_three():
     assert all(6.5 < r < 9.5 for r in ratios), ratios
 
 
+@pytest.mark.parametrize("order", [3, 4])
 @pytest.mark.parametrize("interp,flags", [("hermite", 3), ("none", 7)])
-def test_dahlquist_counts_match_reference(interp, flags):
+def test_dahlquist_counts_match_reference(interp, flags, order):
     g = json.load(open(os.path.join(HERE, "golden", "ark_dahlquist_ark324.json")))
-    assert len(g[interp]) == 4
-    for row in g[interp]:
+    key = interp if order == 3 else "ark437_" + interp
+    assert len(g[key]) == 4
+    for row in g[key]:
         k = row["step"]
         rc, st, _ = dahlquist(g["lambda_e"], g["lambda_i"], g["h"], g["h"] * k, rtol=g["rtol"],
-                              atol=g["atol"], flags=flags)
+                              atol=g["atol"], flags=flags, order=order)
         assert rc == 0 and st["nst"] == k
         assert (st["nni"], st["nfe"], st["nfi"]) == (row["nni"], row["nfe"], row["nfi"]), \
             (k, st["nni"], st["nfe"], st["nfi"], row)
+
+
+def test_kpr_logging_run_matches_reference():
+    """Every printed number of test_logging_arkode_arkstep_2_1_0.out: the
+    rows with the format the driver uses (16 significant digits) and the
+    statistics with ARKodePrintAllStats' (15 significant digits for step
+    sizes)."""
+    g = json.load(open(os.path.join(HERE, "golden", "ark_kpr_logging.json")))
+    rc, rows, st = kpr(3)
+    assert rc == 0
+    for got, ref in zip(rows, g["rows_t_u_v"]):
+        assert ["%.15e" % x for x in got] == ["%.15e" % x for x in ref], (got, ref)
+    ref = g["stats"]
+    for key in ("Initial step size", "Last step size", "Current step size"):
+        assert float("%.15g" % st[key]) == ref[key], (key, st[key], ref[key])
+    for key, val in st.items():
+        if "step size" not in key:
+            assert int(val) == ref[key], (key, val, ref[key])
+    assert "%.15g" % rows[-1][0] == "%.15g" % ref["Current time"]
 
 
 def test_stop_time_reached_exactly():
