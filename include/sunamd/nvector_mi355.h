@@ -68,6 +68,14 @@ long int N_VGetNumLaunches_Mi355(N_Vector v);
 #define SUNAMD_REDUCE_TREE       0
 #define SUNAMD_REDUCE_SEQUENTIAL 1
 SUNErrCode N_VSetReductionOrder_Mi355(N_Vector v, int order);
+/* HIP failures: ops that return SUNErrCode return SUN_ERR_EXT_FAIL, and the
+   first failure on a vector's execution context (its stream) is also kept
+   there, because void ops (N_VLinearSum, ...) cannot return it: after one,
+   reductions return NaN until it is read.  Returns that hipError_t code (0:
+   none) and clears it; -1 for a vector that is not an MI355X vector.  (The
+   reference reports through SUNContext's last error; this library never
+   writes into the SUNContext, whose layout belongs to the linked core.) */
+int N_VGetHipError_Mi355(N_Vector v);
 
 /* Classical Gram-Schmidt step on MI355X vectors with SUNClassicalGS's
    arguments, coefficients and reorthogonalisation rule

@@ -16,8 +16,9 @@
  * (sundials_errors.h:33-100, sundials_linearsolver.h:222-238,
  * sundials_nonlinearsolver.h:200-204).
  *
- * When this header is compiled next to the real SUNDIALS headers the layout
- * check in oracle/abi_layout_check.c verifies the two agree.
+ * tests/test_abi.py checks every ops slot's name, return and parameter types
+ * and every object member against the reference headers, and the compiled
+ * offsets and sizes against the layout those declarations give on LP64.
  * ---------------------------------------------------------------------------*/
 #ifndef SUNAMD_SUNDIALS_ABI_H
 #define SUNAMD_SUNDIALS_ABI_H

@@ -845,10 +845,10 @@ static int cvNlsResidual(N_Vector ycor, N_Vector res, void* mem)
   cv->nfe++;
   if (retval < 0) { return CV_RHSFUNC_FAIL; }
   if (retval > 0) { return RHSFUNC_RECVR; }
-  if (cv->usefused)
+  /* cvode_nls.c:385-391; the fused kernel declines (non-zero) for a vector
+     it does not handle or on a launch error: the op sequence then runs */
+  if (cv->usefused && cv->fused_resid(cv->rl1, -cv->gamma, cv->zn[1], ycor, cv->ftemp, res) == 0)
   {
-    /* cvode_nls.c:385-391 */
-    cv->fused_resid(cv->rl1, -cv->gamma, cv->zn[1], ycor, cv->ftemp, res);
     return CV_SUCCESS;
   }
   N_VLinearSum(cv->rl1, cv->zn[1], ONE, ycor, res);
@@ -865,14 +865,18 @@ static int cvEwtSet(N_Vector ycur, N_Vector weight, CVodeMem cv)
   if (cv->usefused && (cv->itol == CV_SS || cv->itol == CV_SV))
   {
     /* cvode.c:4794-4806, 4834-4846: weights are formed regardless of the
-       component test, which then looks at tempv */
-    if (cv->itol == CV_SS)
+       component test, which then looks at tempv; a declined fused call
+       (non-zero) falls back to the op sequence below */
+    const int rc = (cv->itol == CV_SS)
+                     ? cv->fused_ewt_ss(cv->atolmin0, cv->reltol, cv->Sabstol, ycur, cv->tempv,
+                                        weight)
+                     : cv->fused_ewt_sv(cv->atolmin0, cv->reltol, cv->Vabstol, ycur, cv->tempv,
+                                        weight);
+    if (rc == 0)
     {
-      cv->fused_ewt_ss(cv->atolmin0, cv->reltol, cv->Sabstol, ycur, cv->tempv, weight);
+      if (cv->atolmin0 && N_VMin(cv->tempv) <= ZERO) { return -1; }
+      return 0;
     }
-    else { cv->fused_ewt_sv(cv->atolmin0, cv->reltol, cv->Vabstol, ycur, cv->tempv, weight); }
-    if (cv->atolmin0 && N_VMin(cv->tempv) <= ZERO) { return -1; }
-    return 0;
   }
   switch (cv->itol)
   {

@@ -14,10 +14,11 @@
 // them afterwards.
 //
 // Reduction: each workgroup folds a contiguous range into one partial; one
-// workgroup folds the partials and writes the result into a pinned host slot
-// on the vector's stream, which the caller synchronises.  Scratch (partials
-// and the host slot) is allocated once per device and kept for the process;
-// calls that overlap on one device (several streams or threads) race on it.
+// workgroup folds the partials and writes the result into the pinned result
+// slot of the vector's execution context, on the vector's stream, which the
+// caller synchronises.  The partials are the context's reduction scratch too,
+// so solves on different streams never share scratch (the same rule as every
+// other reduction of the vector).
 // -----------------------------------------------------------------------------
 #include <hip/hip_runtime.h>
 
@@ -32,38 +33,15 @@ namespace
 
 constexpr int TPB      = 256;
 constexpr int MAX_GRID = 2048;
-constexpr int MAX_DEV  = 16;
 
-struct KinScratch
-{
-  double* partials = nullptr; // MAX_GRID doubles on the device
-  double* host     = nullptr; // one pinned, device-mapped slot
-  double* host_dev = nullptr; // its device address
-};
-KinScratch g_scratch[MAX_DEV];
+} // namespace
 
-KinScratch* scratch()
+// reduction scratch of a vector's execution context (nvector_mi355.hip)
+extern "C" int sunamd_nv_reduction_scratch(N_Vector v, double** partials, long* capacity,
+                                           double** result);
+
+namespace
 {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEV) { return nullptr; }
-  KinScratch* s = &g_scratch[dev];
-  if (!s->partials)
-  {
-    if (hipMalloc((void**)&s->partials, MAX_GRID * sizeof(double)) != hipSuccess)
-    {
-      s->partials = nullptr;
-      return nullptr;
-    }
-    if (hipHostMalloc((void**)&s->host, sizeof(double), hipHostMallocMapped) != hipSuccess ||
-        hipHostGetDevicePointer((void**)&s->host_dev, s->host, 0) != hipSuccess)
-    {
-      (void)hipFree(s->partials);
-      s->partials = nullptr;
-      return nullptr;
-    }
-  }
-  return s;
-}
 
 __device__ __forceinline__ double wave_max(double m)
 {
@@ -134,23 +112,25 @@ double scnorm(N_Vector v, N_Vector u, N_Vector s)
   }
   const long n = (long)N_VGetLength(v);
   if (n == 0) { return 0.0; }
-  KinScratch* sc = scratch();
-  if (!sc) { return -1.0; }
+  double *partials = nullptr, *result = nullptr;
+  long cap         = 0;
+  if (sunamd_nv_reduction_scratch(v, &partials, &cap, &result) != 0) { return -1.0; }
   long g = (n + 4L * TPB - 1) / (4L * TPB); // at least 4 elements per thread
   if (g > MAX_GRID) { g = MAX_GRID; }
+  if (g > cap) { g = cap; }
   const long chunk = (n + g - 1) / g;
   g                = (n + chunk - 1) / chunk;
   hipStream_t st   = (hipStream_t)N_VGetStream_Mi355(v);
   k_kin_scnorm<KIND><<<(unsigned)g, TPB, 0, st>>>(
     N_VGetDeviceArrayPointer(v), KIND == 0 ? N_VGetDeviceArrayPointer(u) : nullptr,
-    N_VGetDeviceArrayPointer(s), n, chunk, sc->partials);
-  k_kin_scnorm_finish<<<1, TPB, 0, st>>>(sc->partials, (int)g, sc->host_dev);
+    N_VGetDeviceArrayPointer(s), n, chunk, partials);
+  k_kin_scnorm_finish<<<1, TPB, 0, st>>>(partials, (int)g, result);
   if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess)
   {
     std::fprintf(stderr, "[sunamd] fused KINSOL norm: HIP error\n");
     return -1.0;
   }
-  return *(volatile double*)sc->host;
+  return *(volatile double*)result;
 }
 
 } // namespace

@@ -55,6 +55,7 @@ struct Mi355Exec
   double* d_gs;           // Gram-Schmidt coefficients on the device (lazy, GS_MAX + 2)
   double* h_gs;           // pinned copy of them (lazy)
   long int launches;
+  hipError_t err;         // first HIP failure on this context since it was last read
   bool own_stream;
   bool is_default;
 };
@@ -92,12 +93,39 @@ static void hip_fail(SUNContext ctx, hipError_t e, const char* what, int line)
     if (e_ != hipSuccess) { hip_fail((ctx), e_, #call, __LINE__); }   \
   } while (0)
 
+// the same, also kept as the context's sticky error: void ops cannot return
+// it, so the next reduction returns NaN, the next op returning SUNErrCode
+// returns SUN_ERR_EXT_FAIL, and N_VGetHipError_Mi355 reports it
+static inline void exec_record(Mi355Exec* ex, hipError_t e)
+{
+  if (ex && e != hipSuccess && ex->err == hipSuccess) { ex->err = e; }
+}
+
+#define HIPCHK_EX(v, call)                                            \
+  do {                                                                \
+    hipError_t e_ = (call);                                           \
+    if (e_ != hipSuccess)                                             \
+    {                                                                 \
+      hip_fail((v)->sunctx, e_, #call, __LINE__);                     \
+      exec_record(EXEC(v), e_);                                       \
+    }                                                                 \
+  } while (0)
+
 #define LAUNCHCHK(v)                                                  \
   do {                                                                \
     EXEC(v)->launches++;                                              \
     hipError_t e_ = hipGetLastError();                                \
-    if (e_ != hipSuccess) { hip_fail((v)->sunctx, e_, "kernel launch", __LINE__); } \
+    if (e_ != hipSuccess)                                             \
+    {                                                                 \
+      hip_fail((v)->sunctx, e_, "kernel launch", __LINE__);           \
+      exec_record(EXEC(v), e_);                                       \
+    }                                                                 \
   } while (0)
+
+static inline SUNErrCode exec_status(const Mi355Exec* ex)
+{
+  return (ex->err == hipSuccess) ? SUN_SUCCESS : SUN_ERR_EXT_FAIL;
+}
 
 // One execution context per device is shared by every vector created without
 // an explicit stream, so that all vector, matrix and solver work issued by one
@@ -853,6 +881,21 @@ static void mi_linearsum(sunrealtype a, N_Vector x, sunrealtype b, N_Vector y, N
 // launch accounting for kernels issued on a vector's stream from other files
 extern "C" void sunamd_nv_count_launch(N_Vector v) { EXEC(v)->launches++; }
 
+// the reduction scratch of a vector's execution context, for reductions
+// issued on its stream from other files: *partials holds *capacity doubles on
+// the device, *result one pinned, device-mapped slot (the same storage the
+// vector's own reductions use, so calls on one context stay ordered by its
+// stream and contexts on other streams never share it)
+extern "C" int sunamd_nv_reduction_scratch(N_Vector v, double** partials, long* capacity,
+                                           double** result)
+{
+  if (!SUNAMD_NVectorIsMi355(v) || !EXEC(v)->d_partials || !EXEC(v)->h_result) { return -1; }
+  *partials = EXEC(v)->d_partials;
+  *capacity = (long)NV_MAX_RED_BLOCKS * (NV_MAX_MULTI + 1);
+  *result   = EXEC(v)->h_result;
+  return 0;
+}
+
 extern "C" int SUNAMD_NVectorIsMi355(N_Vector v)
 {
   return (v && v->ops && v->ops->nvlinearsum == mi_linearsum) ? 1 : 0;
@@ -927,8 +970,8 @@ static double mi_reduce(N_Vector x, N_Vector y, N_Vector id, double identity)
   }
   if (fam >= 0) { sunamd_prof_end(fam, ex->stream); }
   LAUNCHCHK(x);
-  HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
-  return ex->h_result[0];
+  HIPCHK_EX(x, hipStreamSynchronize(ex->stream));
+  return (ex->err == hipSuccess) ? ex->h_result[0] : NAN;
 }
 
 static sunrealtype mi_dotprod(N_Vector x, N_Vector y) { return mi_reduce<RK_DOT>(x, y, NULL, 0.0); }
@@ -1013,10 +1056,10 @@ static SUNErrCode mi_dotprodmulti(int nvec, N_Vector x, N_Vector* Y, sunrealtype
       for (int k = 0; k < nv; k++) { P.p[k] = DDATA(Y[base + k]); }
       k_dotmulti_seq<<<1, 64, 0, ex->stream>>>(DDATA(x), P, nv, n, ex->h_result);
       LAUNCHCHK(x);
-      HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
+      HIPCHK_EX(x, hipStreamSynchronize(ex->stream));
       for (int k = 0; k < nv; k++) { d[base + k] = ex->h_result[k]; }
     }
-    return SUN_SUCCESS;
+    return exec_status(ex);
   }
   // chunks of up to 8 vectors keep the per-lane accumulators in registers;
   // results land in the pinned slots, read back every NV_MAX_MULTI products
@@ -1045,12 +1088,12 @@ static SUNErrCode mi_dotprodmulti(int nvec, N_Vector x, N_Vector* Y, sunrealtype
     int filled = base + nv - group;
     if (filled == NV_MAX_MULTI || base + nv == nvec)
     {
-      HIPCHK(x->sunctx, hipStreamSynchronize(ex->stream));
+      HIPCHK_EX(x, hipStreamSynchronize(ex->stream));
       for (int k = 0; k < filled; k++) { d[group + k] = ex->h_result[k]; }
       group += filled;
     }
   }
-  return SUN_SUCCESS;
+  return exec_status(ex);
 }
 
 static SUNErrCode mi_lincomb(int nvec, sunrealtype* c, N_Vector* X, N_Vector z)
@@ -1100,7 +1143,7 @@ static SUNErrCode mi_lincomb(int nvec, sunrealtype* c, N_Vector* X, N_Vector z)
     LAUNCHCHK(z);
     done += (done == 0) ? nv : nv - 1;
   }
-  return SUN_SUCCESS;
+  return exec_status(ex);
 }
 
 static SUNErrCode mi_scaleaddmulti(int nvec, sunrealtype* a, N_Vector x, N_Vector* Y,
@@ -1134,7 +1177,7 @@ static SUNErrCode mi_scaleaddmulti(int nvec, sunrealtype* a, N_Vector x, N_Vecto
     }
     LAUNCHCHK(x);
   }
-  return SUN_SUCCESS;
+  return exec_status(ex);
 }
 
 static SUNErrCode mi_scalevectorarray(int nvec, sunrealtype* c, N_Vector* X, N_Vector* Z)
@@ -1167,7 +1210,7 @@ static SUNErrCode mi_scalevectorarray(int nvec, sunrealtype* c, N_Vector* X, N_V
     }
     LAUNCHCHK(Z[0]);
   }
-  return SUN_SUCCESS;
+  return exec_status(ex);
 }
 
 static SUNErrCode mi_linearsumvectorarray(int nvec, sunrealtype a, N_Vector* X, sunrealtype b,
@@ -1175,21 +1218,21 @@ static SUNErrCode mi_linearsumvectorarray(int nvec, sunrealtype a, N_Vector* X, 
 {
   if (nvec < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
   for (int i = 0; i < nvec; i++) { mi_linearsum(a, X[i], b, Y[i], Z[i]); }
-  return SUN_SUCCESS;
+  return exec_status(EXEC(Z[0]));
 }
 
 static SUNErrCode mi_constvectorarray(int nvec, sunrealtype c, N_Vector* Z)
 {
   if (nvec < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
   for (int i = 0; i < nvec; i++) { mi_const(c, Z[i]); }
-  return SUN_SUCCESS;
+  return exec_status(EXEC(Z[0]));
 }
 
 static SUNErrCode mi_wrmsnormvectorarray(int nvec, N_Vector* X, N_Vector* W, sunrealtype* nrm)
 {
   if (nvec < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
   for (int i = 0; i < nvec; i++) { nrm[i] = mi_wrmsnorm(X[i], W[i]); }
-  return SUN_SUCCESS;
+  return exec_status(EXEC(X[0]));
 }
 
 static SUNErrCode mi_wrmsnormmaskvectorarray(int nvec, N_Vector* X, N_Vector* W, N_Vector id,
@@ -1197,7 +1240,7 @@ static SUNErrCode mi_wrmsnormmaskvectorarray(int nvec, N_Vector* X, N_Vector* W,
 {
   if (nvec < 1) { return SUN_ERR_ARG_OUTOFRANGE; }
   for (int i = 0; i < nvec; i++) { nrm[i] = mi_wrmsnormmask(X[i], W[i], id); }
-  return SUN_SUCCESS;
+  return exec_status(EXEC(X[0]));
 }
 
 static SUNErrCode mi_scaleaddmultivectorarray(int nvec, int nsum, sunrealtype* a, N_Vector* X,
@@ -1683,9 +1726,10 @@ extern "C" SUNErrCode SUNAMD_ClassicalGS_Mi355(N_Vector* v, sunrealtype** h, int
     }
     LAUNCHCHK(vk);
   }
-  HIPCHK(vk->sunctx, hipMemcpyAsync(ex->h_gs, ex->d_gs, sizeof(double) * (nv + 2),
-                                    hipMemcpyDeviceToHost, ex->stream));
-  HIPCHK(vk->sunctx, hipStreamSynchronize(ex->stream));
+  HIPCHK_EX(vk, hipMemcpyAsync(ex->h_gs, ex->d_gs, sizeof(double) * (nv + 2),
+                               hipMemcpyDeviceToHost, ex->stream));
+  HIPCHK_EX(vk, hipStreamSynchronize(ex->stream));
+  if (ex->err != hipSuccess) { return SUN_ERR_EXT_FAIL; }
   const double* d      = ex->h_gs;
   const double vk_norm = (d[nv] <= 0.0) ? 0.0 : sqrt(d[nv]);
   for (int i = 0; i < nv; i++) { h[i][k_minus_1] = d[i]; }
@@ -1711,6 +1755,15 @@ extern "C" SUNErrCode SUNAMD_ClassicalGS_Mi355(N_Vector* v, sunrealtype** h, int
     *new_vk_norm = (nn <= 0.0) ? 0.0 : sqrt(nn);
   }
   return SUN_SUCCESS;
+}
+
+extern "C" int N_VGetHipError_Mi355(N_Vector v)
+{
+  if (!SUNAMD_NVectorIsMi355(v)) { return -1; }
+  Mi355Exec* ex = EXEC(v);
+  const int e   = (int)ex->err;
+  ex->err       = hipSuccess;
+  return e;
 }
 
 extern "C" SUNErrCode N_VSetReductionOrder_Mi355(N_Vector v, int order)

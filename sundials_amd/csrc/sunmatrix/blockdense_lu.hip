@@ -687,6 +687,15 @@ static int lu_solve(SUNLinearSolver S, SUNMatrix A, N_Vector x, N_Vector b, sunr
 {
   (void)tol;
   BlockLU* c = LU(S);
+  // the solve runs on the matrix's stream: vectors on another stream would
+  // race with it
+  if (!SUNAMD_NVectorIsMi355(x) || !SUNAMD_NVectorIsMi355(b) ||
+      (hipStream_t)N_VGetStream_Mi355(x) != c->stream ||
+      (hipStream_t)N_VGetStream_Mi355(b) != c->stream)
+  {
+    c->last_flag = SUN_ERR_ARG_INCOMPATIBLE;
+    return SUN_ERR_ARG_INCOMPATIBLE;
+  }
   double* xd = N_VGetDeviceArrayPointer(x);
   double* bd = N_VGetDeviceArrayPointer(b);
   if (!xd || !bd) { return SUN_ERR_ARG_CORRUPT; }
@@ -747,6 +756,11 @@ extern "C" SUNLinearSolver SUNLinSol_Mi355BlockLU(N_Vector y, SUNMatrix A, int p
   if (precision != SUNAMD_LU_FP64 && precision != SUNAMD_LU_FP32) { return NULL; }
   if (N_VGetLength(y) != BM(A)->nblocks * BM(A)->m) { return NULL; }
   if (BM(A)->m > 64) { return NULL; }
+  // factor and solves are ordered with the vector ops only on one stream
+  if (!SUNAMD_NVectorIsMi355(y) || (hipStream_t)N_VGetStream_Mi355(y) != BM(A)->stream)
+  {
+    return NULL;
+  }
   SUNLinearSolver S = SUNLinSolNewEmpty(sunctx);
   if (!S) { return NULL; }
   S->ops->gettype    = lu_gettype;

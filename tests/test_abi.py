@@ -1,8 +1,15 @@
 """The C-ABI libraries load on a CPU-only host and export every function the
-public headers declare (no compute calls are made here)."""
+public headers declare (no compute calls are made here).  The restated
+generic-object structs match the reference's declarations: every ops slot's
+name, return type and parameter types in order, every field of the object
+structs, and -- compiled with the host C compiler -- every offset and size,
+against the layout the reference's declared field types give on LP64 with
+its default scalar types (sunrealtype double, sunindextype 64-bit)."""
 import ctypes as C
 import os
 import re
+import shutil
+import subprocess
 
 import pytest
 
@@ -90,3 +97,115 @@ def test_abi_ops_tables(struct):
     if not os.path.exists(ref):
         pytest.skip("reference headers not present on this host")
     assert ours == _fields(open(ref).read(), struct)
+
+
+REF_INC = "/root/reference/include/sundials"
+OBJECTS = {"_generic_N_Vector": "sundials_nvector.h", "_generic_SUNMatrix": "sundials_matrix.h",
+           "_generic_SUNLinearSolver": "sundials_linearsolver.h",
+           "_generic_SUNNonlinearSolver": "sundials_nonlinearsolver.h"}
+
+
+def _strip(text):
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.sub(r"//[^\n]*", "", text)
+
+
+def _slots(text, struct):
+    """(return type, name, parameter types) of every function-pointer slot."""
+    body = re.search(r"struct %s\s*\{(.*?)\};" % struct, _strip(text), re.S).group(1)
+    out = []
+    for decl in body.split(";"):
+        d = " ".join(decl.split())
+        if not d:
+            continue
+        m = re.match(r"^(.*?)\(\s*\*\s*(\w+)\s*\)\s*\((.*)\)$", d)
+        assert m, d
+        args = tuple(" ".join(a.replace("*", " * ").split()) for a in m.group(3).split(","))
+        out.append((" ".join(m.group(1).replace("*", " * ").split()), m.group(2), args))
+    return out
+
+
+def _members(text, struct):
+    """(type, name) of the members of a plain struct."""
+    body = re.search(r"struct %s\s*\{(.*?)\};" % struct, _strip(text), re.S).group(1)
+    out = []
+    for decl in body.split(";"):
+        d = " ".join(decl.replace("*", " * ").split())
+        if d:
+            t, n = d.rsplit(" ", 1)
+            out.append((t, n))
+    return out
+
+
+def _ref(header):
+    path = os.path.join(REF_INC, header)
+    if not os.path.exists(path):
+        pytest.skip("reference headers not present on this host")
+    return open(path).read()
+
+
+@pytest.mark.parametrize("struct", sorted(STRUCTS))
+def test_abi_ops_signatures_match_reference(struct):
+    ours = _slots(open(os.path.join(INC, "sundials_abi.h")).read(), struct)
+    ref = _slots(_ref(STRUCTS[struct][0]), struct)
+    assert ours == ref
+
+
+@pytest.mark.parametrize("struct", sorted(OBJECTS))
+def test_abi_object_members_match_reference(struct):
+    ours = _members(open(os.path.join(INC, "sundials_abi.h")).read(), struct)
+    assert ours == _members(_ref(OBJECTS[struct]), struct)
+
+
+def _ref_default_index_bits():
+    path = "/root/reference/cmake/SundialsBuildOptionsPre.cmake"
+    if not os.path.exists(path):
+        pytest.skip("reference build files not present on this host")
+    # the non-xSDK branch: if(USE_XSDK_DEFAULTS) ... else() <default> endif()
+    m = re.search(r'else\(\)\s*sundials_option\(\s*SUNDIALS_INDEX_SIZE\s+STRING\s+"[^"]*"\s+'
+                  r'"(\d+)"', open(path).read())
+    assert m
+    return int(m.group(1))
+
+
+def test_abi_compiled_layout():
+    """Offsets and sizes of this library's structs as the C compiler lays them
+    out equal those the reference's member types give: every ops slot and
+    object member a pointer (8 bytes on LP64, in declaration order), and the
+    scalar types of the reference's default build."""
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    assert _ref_default_index_bits() == 64
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "sunamd/sundials_abi.h"',
+             "int main(void) {",
+             'printf("scalar %zu %zu %zu\\n", sizeof(sunrealtype), sizeof(sunindextype), '
+             "sizeof(sunbooleantype));"]
+    expect = {"scalar": [8, _ref_default_index_bits() // 8, 4]}
+    for struct, (header, n) in STRUCTS.items():
+        names = [x[1] for x in _slots(_ref(header), struct)]
+        offs = " ".join("%zu" for _ in names)
+        args = ", ".join("offsetof(struct %s, %s)" % (struct, f) for f in names)
+        lines.append('printf("%s %%zu %s\\n", sizeof(struct %s), %s);' % (struct, offs, struct,
+                                                                         args))
+        expect[struct] = [8 * n] + [8 * i for i in range(n)]
+    for struct, header in OBJECTS.items():
+        mem = _members(_ref(header), struct)
+        assert all("*" in t or t in ("N_Vector_Ops", "SUNMatrix_Ops", "SUNLinearSolver_Ops",
+                                     "SUNNonlinearSolver_Ops", "SUNContext") for t, _ in mem)
+        offs = " ".join("%zu" for _ in mem)
+        args = ", ".join("offsetof(struct %s, %s)" % (struct, f) for _, f in mem)
+        lines.append('printf("%s %%zu %s\\n", sizeof(struct %s), %s);' % (struct, offs, struct,
+                                                                         args))
+        expect[struct] = [8 * len(mem)] + [8 * i for i in range(len(mem))]
+    lines.append("return 0; }")
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "layout.c"), os.path.join(d, "layout")
+        open(src, "w").write("\n".join(lines) + "\n")
+        subprocess.run([cc, "-std=c11", "-I" + os.path.join(ROOT, "include"), src, "-o", exe],
+                       check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    got = {ln.split()[0]: [int(x) for x in ln.split()[1:]] for ln in out.splitlines()}
+    assert got == expect

@@ -195,12 +195,15 @@ def test_coupled_fused_kernels_change_nothing(libs, order, m, R, ncells):
     """CVODE's fused error-weight and residual kernels (the counterparts of
     cvode_fused_gpu.cpp) and the fused Nordsieck predictor / undo form every
     element with the roundings of the op sequences they replace: with them
-    on, the integration is bit-identical to the unfused run (and, in
-    sequential order, to the oracle).  3 x 33 = 99 unknowns exercises the odd
-    tail element of the paired kernels."""
+    on, the integration is bit-identical to the unfused run and, in
+    sequential order, to the CPU oracle (every counter, the step sizes and
+    the final state).  3 x 33 = 99 unknowns exercises the odd tail element of
+    the paired kernels."""
     core, hip, orc = libs
     p = _prob(core, m, R, ncells, 1001)
     n = ncells * m
+    y_cpu, ost = np.zeros(n), np.zeros(16)
+    assert orc.oracle_massaction_solve_coupled(p, 1e-6, 1e-10, 1.0, dp(y_cpu), dp(ost)) == 0
     runs = []
     for fused in (0, 1):
         r = hip.SUNAMD_CoupledCreate_Mi355(p, 1e-6, 1e-10, 0, order)
@@ -210,10 +213,14 @@ def test_coupled_fused_kernels_change_nothing(libs, order, m, R, ncells):
         assert hip.SUNAMD_CoupledRun_Mi355(r, 1.0, C.byref(st)) == 0
         y = np.zeros(n)
         assert hip.SUNAMD_CoupledGetState_Mi355(r, dp(y)) == 0
-        runs.append((_counts(st), st.launches, y))
+        runs.append((_counts(st), st.launches, y, (st.hinused, st.hlast, st.tcur)))
         hip.SUNAMD_CoupledFree_Mi355(r)
     core.SUNAMD_MassActionFree(p)
-    (c0, l0, y0), (c1, l1, y1) = runs
-    assert c0 == c1
+    (c0, l0, y0, h0), (c1, l1, y1, h1) = runs
+    assert c0 == c1 and h0 == h1
     assert np.array_equal(y0, y1)
     assert l1 < l0  # fewer launches
+    if order == SEQ:
+        assert c1 == tuple(int(v) for v in ost[:8])
+        assert h1 == (ost[10], ost[11], ost[13])
+        assert np.array_equal(y1, y_cpu)
