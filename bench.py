@@ -44,6 +44,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 M_SPECIES, R_REACTIONS, SEED = 32, 48, 1002
 RTOL, ATOL, TOUT = 1e-6, 1e-10, 1.0
+TIGHT_RTOL, TIGHT_ATOL = 1e-10, 1e-14   # reference solution for accuracy reports
 
 # uneven batch
 BATCH_M = (4, 8, 16, 24, 32, 48, 64)
@@ -232,29 +233,49 @@ def batch_flops(cnt, msys, rsys):
             + 30 * m * nst)
 
 
-def batch_cpu_baseline(nsys_sample, threads):
+def _oracle_batch(nsys, threads, rtol, atol):
+    """The CPU oracle on the first nsys systems of the measured batch (the
+    same systems, in the same order and layout, as its first nsys)."""
     from sundials_amd import _lib
 
     core, orc = _lib.core(), _lib.load_oracle()
     n = len(BATCH_M)
-    b = core.SUNAMD_BatchCreate(nsys_sample, BATCH_SEED, n, (C.c_int * n)(*BATCH_M),
+    b = core.SUNAMD_BatchCreate(nsys, BATCH_SEED, n, (C.c_int * n)(*BATCH_M),
                                 (C.c_double * n)(*BATCH_W), 1.5, 0.2)
     bc = b.contents
     y = np.zeros(bc.ytotal)
     cnt = np.zeros((bc.nsys, _lib.BS_COUNT), dtype=np.int64)
     wall = C.c_double()
-    rc = orc.oracle_batch_solve(b, RTOL, ATOL, TOUT, threads,
+    rc = orc.oracle_batch_solve(b, rtol, atol, TOUT, threads,
                                 y.ctypes.data_as(C.POINTER(C.c_double)),
                                 cnt.ctypes.data_as(C.POINTER(C.c_long)), C.byref(wall))
     core.SUNAMD_BatchFree(b)
+    return rc, y, cnt, wall.value
+
+
+def batch_cpu_baseline(nsys_sample, threads):
+    rc, y, cnt, wall = _oracle_batch(nsys_sample, threads, RTOL, ATOL)
     if rc != 0:
-        return None
+        return None, None, None
     nst = int(cnt[:, 0].sum())
-    return {"value": nst / wall.value, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-            "sample": "CPU oracle (serial vector + dense LU, one CVode run per system, OpenMP "
-                      "over systems) on the first %d of the 65,536 systems (%d cell-steps, "
-                      "%.1f s, %s, nproc=%d)" % (nsys_sample, nst, wall.value, cpu_model(),
-                                                 os.cpu_count() or 0)}
+    return ({"value": nst / wall, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+             "sample": "CPU oracle (serial vector + dense LU, one CVode run per system, OpenMP "
+                       "over systems) on the first %d of the 65,536 systems (%d cell-steps, "
+                       "%.1f s, %s, nproc=%d)" % (nsys_sample, nst, wall, cpu_model(),
+                                                  os.cpu_count() or 0)}, y, cnt)
+
+
+def _wrms_per_system(y, y_ref, yoff, msys):
+    """Per-system WRMS of y - y_ref with the run's weights 1/(rtol|y_ref| + atol)."""
+    w = 1.0 / (RTOL * np.abs(y_ref) + ATOL)
+    d2 = ((y - y_ref) * w) ** 2
+    csum = np.concatenate(([0.0], np.cumsum(d2)))
+    return np.sqrt((csum[yoff + msys] - csum[yoff]) / msys)
+
+
+def _err_summary(e):
+    return {"median": round(float(np.median(e)), 4), "p99": round(float(np.percentile(e, 99)), 4),
+            "max": round(float(e.max()), 4)}
 
 
 def _batch(core, nsys, seed):
@@ -379,6 +400,21 @@ MEASURED_STREAM_TRAFFIC = {
 def stream_traffic(workload, kernel, n):
     """Recorded HBM bytes per launch, only at the size they were measured at."""
     return MEASURED_STREAM_TRAFFIC.get((workload, kernel)) if n == 6291456 else None
+
+
+# the kernel source the MEASURED_TRAFFIC figures were taken at (first 16 hex digits of the
+# SHA-256 of bdf_device.h followed by bdf_args.h); a changed kernel reports traffic null
+MEASURED_TRAFFIC_SOURCE = "fa969dec90e61d87"
+
+
+def batched_kernel_source_hash():
+    import hashlib
+
+    h = hashlib.sha256()
+    for f in ("bdf_device.h", "bdf_args.h"):
+        with open(os.path.join(ROOT, "sundials_amd", "csrc", "batched", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 MEASURED_TRAFFIC = {
@@ -845,32 +881,71 @@ def bench_batched(args):
     dflop = float(fl[sb == dom].sum())
     achieved = dflop / (kms[dom] * 1e-3) / 1e12
     learn_dom = bool(mask & (1 << dom)) or w32[dom] >= 4
-    roofline = {"bound": "mfma", "kernel": "k_batched_bdf<%d, float, %d, %s>" % (
-                    bm[dom], w32[dom], "true" if learn_dom else "false"),
+    kname = "k_batched_bdf<%d, float, %d, %s>" % (bm[dom], w32[dom],
+                                                 "true" if learn_dom else "false")
+    src = batched_kernel_source_hash()
+    traffic_ok = args.systems == 65536 and src == MEASURED_TRAFFIC_SOURCE
+    roofline = {"bound": "valu", "kernel": kname,
                 "achieved": round(achieved, 4), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFS, 5),
-                "traffic": MEASURED_TRAFFIC.get(
-                    "k_batched_bdf<%d, float, %d, %s>" % (bm[dom], w32[dom],
-                                                         "true" if learn_dom else "false"))
-                if args.systems == 65536 else None,
-                "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this benchmark, "
-                                  "scaled by a 256 MiB copy (x2.000 / x1.000); null for a "
-                                  "kernel not measured",
+                "traffic": MEASURED_TRAFFIC.get(kname) if traffic_ok else None,
+                "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this benchmark "
+                                  "at kernel source %s, scaled by a 256 MiB copy (x2.000 / "
+                                  "x1.000); null when the kernel source (now %s) or the batch "
+                                  "size differs" % (MEASURED_TRAFFIC_SOURCE, src),
                 "flops_per_launch": dflop, "launch_ms": round(float(kms[dom]), 3),
-                "note": "compute/latency-bound persistent integrator; priced against the FP64 "
-                        "vector peak (most of its arithmetic is FP64)"}
+                "note": "VALU issue- and latency-bound persistent integrator (no MFMA: per-lane "
+                        "FP64 scalar control logic, register-row LU); algorithmic flops priced "
+                        "against the FP64 vector peak"}
 
     # accuracy of the FP32-LU form against the FP64 form (= CPU oracle bit for bit)
-    yoff = np.ctypeslib.as_array(bc.sys_yoff, shape=(bc.nsys,))
-    w = 1.0 / (RTOL * np.abs(y64) + ATOL)
-    d2 = ((y32 - y64) * w) ** 2
-    csum = np.concatenate(([0.0], np.cumsum(d2)))
-    werr = np.sqrt((csum[yoff + msys] - csum[yoff]) / msys)
+    yoff = np.ctypeslib.as_array(bc.sys_yoff, shape=(bc.nsys,)).copy()
+    werr = _wrms_per_system(y32, y64, yoff, msys)
     # relative error of components above the absolute tolerance
     relmax = float(np.max(np.abs(y32 - y64) / np.maximum(np.abs(y64), ATOL)))
 
-    cpu = None if args.no_cpu else batch_cpu_baseline(args.cpu_systems,
-                                                      min(16, os.cpu_count() or 1))
+    # the CPU oracle on the first cpu_systems systems: the timed CPU baseline,
+    # the FP64 leg's parity (every counter and every final value bit for bit),
+    # and a reference solution at 10^4x tighter tolerances for the accuracy of
+    # both forms
+    cpu, parity, accuracy = None, None, None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        ns = min(args.cpu_systems, args.systems)
+        cpu, ycpu, ccpu = batch_cpu_baseline(ns, threads)
+        if cpu is None:
+            print("[bench] CPU oracle failed", file=sys.stderr)
+            sys.exit(1)
+        nyc = ycpu.size
+        same_c = bool(np.array_equal(c64[:ns], ccpu))
+        same_y = bool(np.array_equal(y64[:nyc], ycpu))
+        parity = {"systems": ns, "counters_equal": same_c, "final_states_bitwise_equal": same_y,
+                  "oracle": "CPU oracle (serial vector + dense LU), rtol %g atol %g" % (RTOL, ATOL)}
+        print("[bench] fp64 leg vs CPU oracle on %d systems: counters %s, states %s"
+              % (ns, "equal" if same_c else "DIFFER", "bitwise equal" if same_y else "DIFFER"),
+              file=sys.stderr, flush=True)
+        if not (same_c and same_y):
+            print("[bench] the plain FP64 leg does not reproduce the oracle", file=sys.stderr)
+            sys.exit(1)
+        rc_t, yt, ct, wall_t = _oracle_batch(ns, threads, TIGHT_RTOL, TIGHT_ATOL)
+        if rc_t == 0:
+            yo, ms_ = yoff[:ns], msys[:ns]
+            e64 = _wrms_per_system(y64[:nyc], yt, yo, ms_)
+            e32 = _wrms_per_system(y32[:nyc], yt, yo, ms_)
+            worst = np.argsort(-np.maximum(e64, e32))[:5]
+            accuracy = {
+                "reference": "CPU oracle at rtol %g atol %g on the same %d systems (%.1f s)"
+                             % (TIGHT_RTOL, TIGHT_ATOL, ns, wall_t),
+                "weights": "1/(%g |y_ref| + %g), per-system WRMS" % (RTOL, ATOL),
+                "plain_fp64": _err_summary(e64), "fp32_lu": _err_summary(e32),
+                "fp32_lu_vs_fp64_on_sample": _err_summary(werr[:ns]),
+                "worst_systems": [{"system": int(s_), "m": int(msys[s_]),
+                                   "fp64": round(float(e64[s_]), 3),
+                                   "fp32_lu": round(float(e32[s_]), 3),
+                                   "fp32_lu_vs_fp64": round(float(werr[s_]), 3)}
+                                  for s_ in worst]}
+            print("[bench] error vs tight reference: fp64 %s, fp32-lu %s"
+                  % (accuracy["plain_fp64"], accuracy["fp32_lu"]), file=sys.stderr, flush=True)
     core.SUNAMD_BatchFree(b)
 
     value = nst32 * args.steps / elapsed
@@ -904,6 +979,8 @@ def bench_batched(args):
         "error_vs_fp64": {"max_weighted_rms": float(werr.max()),
                           "median_weighted_rms": float(np.median(werr)),
                           "max_rel": relmax},
+        "fp64_parity_vs_cpu_oracle": parity,
+        "error_vs_tight_reference": accuracy,
         "learned": learned,
         "autotune": {"fp64": tune64, "fp32": tune32},
         "roofline": roofline,
