@@ -134,6 +134,13 @@ int CVodeGetNumInitialCorrectionsUsed(void* cvode_mem, long int* nused);
 int CVodeGetAllStats(void* cvode_mem, long int* counts /* [8] */,
                      int* orders /* [2] */, sunrealtype* steps /* [4] */);
 
+/* every statistic, as CVodePrintAllStats (cvode_io.c:1626-1714) writes it:
+   SUN_OUTPUTFORMAT_TABLE one "name = value" line each (names padded to 29
+   columns, reals "%.15g"), SUN_OUTPUTFORMAT_CSV "name,value" pairs joined by
+   commas (reals "% .15e"), with no line end, as the reference writes them;
+   rootfinding is not part of this library, so its "Root fn evals" entry is 0 */
+int CVodePrintAllStats(void* cvode_mem, FILE* outfile, SUNOutputFormat fmt);
+
 void CVodeFree(void** cvode_mem);
 
 #ifdef __cplusplus

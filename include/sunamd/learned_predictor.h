@@ -78,6 +78,34 @@ int SUNAMD_MLPLossGrad(SUNAMD_MLP* M, const float* x, const float* t, const floa
 int SUNAMD_MLPTrain(SUNAMD_MLP* M, const float* x_dev, const float* t_dev,
                     const float* mask_dev, long n, int batch, int steps, float lr, float* loss);
 
+/* ---- training-pair log file (versioned binary, no reference counterpart) --
+   Header: magic "SUNAMDTP", version 1, header size 64, rows, 144 inputs and
+   64 targets per row, and the logging runs' seed, rtol, atol and stride; then
+   per row the system's species count m (uint32), the 144 float32 inputs and
+   the 64 float32 targets (little-endian).  The species mask of a row is its
+   first m entries.  Return codes: 0 success, -1 bad argument, -2 cannot open,
+   -3 I/O error or short file, -4 a mask that is not a prefix of ones (write),
+   -5 not a log file, -6 other version or row shape, -7 rows out of range,
+   -8 corrupt row. */
+typedef struct
+{
+  int version;
+  long rows;
+  uint64_t seed;
+  double rtol, atol;
+  int stride;
+} SUNAMD_TrainingLogInfo;
+
+/* host arrays x[rows x 144], t[rows x 64], mask[rows x 64] (the layout the
+   batched integrator logs into); info supplies seed / rtol / atol / stride
+   (NULL: zeros) */
+int SUNAMD_TrainingLogWrite(const char* path, const float* x, const float* t,
+                            const float* mask, long rows, const SUNAMD_TrainingLogInfo* info);
+int SUNAMD_TrainingLogInfoRead(const char* path, SUNAMD_TrainingLogInfo* info);
+/* rows [row0, row0 + nrows) into host arrays of the write layout */
+int SUNAMD_TrainingLogRead(const char* path, long row0, long nrows, float* x, float* t,
+                           float* mask);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,13 @@ typedef long int suncountertype;
 typedef int SUNErrCode;
 typedef int SUNComm;
 #define SUN_COMM_NULL 0
+
+/* statistics output formats (sundials_types.h:190-194) */
+typedef enum
+{
+  SUN_OUTPUTFORMAT_TABLE,
+  SUN_OUTPUTFORMAT_CSV
+} SUNOutputFormat;
 #define SUN_RCONST(x) x
 #define SUN_BIG_REAL DBL_MAX
 #define SUN_SMALL_REAL DBL_MIN

@@ -360,3 +360,52 @@ int oracle_tridiag_psolve(void* data, N_Vector r_vec, N_Vector z_vec, sunrealtyp
   for (sunindextype i = 0; i < P->N; i++) { z[i] = r[i] / d[i]; }
   return 0;
 }
+
+/* cvRoberts_dns's statistics output (examples/cvode/serial/cvRoberts_dns.c:
+   198-229): after every CVode return CVodePrintAllStats in CSV format to
+   csv_path (one line per return: the example's file, whose lines the
+   driver separates), and at the end the TABLE format to table_path */
+int oracle_roberts_stats(int nout, const char* csv_path, const char* table_path)
+{
+  SUNContext ctx;
+  if (SUNContext_Create(SUN_COMM_NULL, &ctx)) { return -1; }
+  N_Vector y      = N_VNew_Serial(3, ctx);
+  N_Vector abstol = N_VNew_Serial(3, ctx);
+  sunrealtype* yv = N_VGetArrayPointer(y);
+  sunrealtype* av = N_VGetArrayPointer(abstol);
+  yv[0] = 1.0;
+  yv[1] = 0.0;
+  yv[2] = 0.0;
+  av[0] = 1.0e-8;
+  av[1] = 1.0e-14;
+  av[2] = 1.0e-6;
+  void* cv = CVodeCreate(CV_BDF, ctx);
+  int rc   = CVodeInit(cv, rob_f, 0.0, y);
+  rc |= CVodeSVtolerances(cv, 1.0e-4, abstol);
+  SUNMatrix A       = SUNDenseMatrix(3, 3, ctx);
+  SUNLinearSolver L = SUNLinSol_Dense(y, A, ctx);
+  rc |= CVodeSetLinearSolver(cv, L, A);
+  rc |= CVodeSetJacFn(cv, rob_jac);
+  FILE* csv = fopen(csv_path, "w");
+  if (rc || !csv) { rc = -2; }
+  sunrealtype tout = 0.4, t = 0.0;
+  for (int k = 0; k < nout && rc == 0; k++)
+  {
+    int flag = CVode(cv, tout, y, &t, CV_NORMAL);
+    if (flag < 0) { rc = flag; }
+    if (CVodePrintAllStats(cv, csv, SUN_OUTPUTFORMAT_CSV) != CV_SUCCESS) { rc = -3; }
+    fprintf(csv, "\n");
+    tout *= 10.0;
+  }
+  if (csv) { fclose(csv); }
+  FILE* tab = fopen(table_path, "w");
+  if (!tab || CVodePrintAllStats(cv, tab, SUN_OUTPUTFORMAT_TABLE) != CV_SUCCESS) { rc = -4; }
+  if (tab) { fclose(tab); }
+  CVodeFree(&cv);
+  SUNLinSolFree(L);
+  SUNMatDestroy(A);
+  N_VDestroy(y);
+  N_VDestroy(abstol);
+  SUNContext_Free(&ctx);
+  return rc;
+}

@@ -1748,6 +1748,74 @@ int CVodeGetAllStats(void* cvode_mem, long int* counts, int* orders, sunrealtype
   return CV_SUCCESS;
 }
 
+/* CVodePrintAllStats (cvode_io.c:1626-1714) with the reference's number
+   formats (SUN_FORMAT_G "%.15g", SUN_FORMAT_E "% .15e", sundials_types.h:113-114)
+   and table width (SUN_TABLE_WIDTH 29, sundials_utils.h:31) */
+#define STATS_TABLE_WIDTH 29
+static void stats_real(FILE* fp, SUNOutputFormat fmt, int start, const char* name,
+                       sunrealtype value)
+{
+  if (fmt == SUN_OUTPUTFORMAT_TABLE) { fprintf(fp, "%-*s = %.15g\n", STATS_TABLE_WIDTH, name, value); }
+  else { fprintf(fp, "%s%s,% .15e", start ? "" : ",", name, value); }
+}
+
+static void stats_long(FILE* fp, SUNOutputFormat fmt, int start, const char* name, long value)
+{
+  if (fmt == SUN_OUTPUTFORMAT_TABLE) { fprintf(fp, "%-*s = %ld\n", STATS_TABLE_WIDTH, name, value); }
+  else { fprintf(fp, "%s%s,%ld", start ? "" : ",", name, value); }
+}
+
+int CVodePrintAllStats(void* cvode_mem, FILE* outfile, SUNOutputFormat fmt)
+{
+  CVodeMem cv = (CVodeMem)cvode_mem;
+  if (!cv) { return CV_MEM_NULL; }
+  if ((fmt != SUN_OUTPUTFORMAT_TABLE && fmt != SUN_OUTPUTFORMAT_CSV) || !outfile)
+  {
+    return CV_ILL_INPUT;
+  }
+  stats_real(outfile, fmt, 1, "Current time", cv->tn);
+  stats_long(outfile, fmt, 0, "Steps", cv->nst);
+  stats_long(outfile, fmt, 0, "Error test fails", cv->netf);
+  stats_long(outfile, fmt, 0, "NLS step fails", cv->ncfn);
+  stats_real(outfile, fmt, 0, "Initial step size", cv->h0u);
+  stats_real(outfile, fmt, 0, "Last step size", cv->hu);
+  stats_real(outfile, fmt, 0, "Current step size", cv->next_h);
+  stats_long(outfile, fmt, 0, "Last method order", cv->qu);
+  stats_long(outfile, fmt, 0, "Current method order", cv->next_q);
+  stats_long(outfile, fmt, 0, "Stab. lim. order reductions", 0L);
+  stats_long(outfile, fmt, 0, "RHS fn evals", cv->nfe);
+  stats_long(outfile, fmt, 0, "NLS iters", cv->nni);
+  stats_long(outfile, fmt, 0, "NLS fails", cv->nnf);
+  if (cv->nst > 0)
+  {
+    stats_real(outfile, fmt, 0, "NLS iters per step", (sunrealtype)cv->nni / (sunrealtype)cv->nst);
+  }
+  stats_long(outfile, fmt, 0, "LS setups", cv->nsetups);
+  if (cv->ls.LS)
+  {
+    /* direct solvers: no difference-quotient RHS, preconditioner or
+       Jacobian-times evaluations */
+    stats_long(outfile, fmt, 0, "Jac fn evals", cv->ls.nje);
+    stats_long(outfile, fmt, 0, "LS RHS fn evals", 0L);
+    stats_long(outfile, fmt, 0, "Prec setup evals", 0L);
+    stats_long(outfile, fmt, 0, "Prec solves", 0L);
+    stats_long(outfile, fmt, 0, "LS iters", cv->ls.nli);
+    stats_long(outfile, fmt, 0, "LS fails", cv->ls.ncfl);
+    stats_long(outfile, fmt, 0, "Jac-times setups", 0L);
+    stats_long(outfile, fmt, 0, "Jac-times evals", 0L);
+    if (cv->nni > 0)
+    {
+      stats_real(outfile, fmt, 0, "LS iters per NLS iter",
+                 (sunrealtype)cv->ls.nli / (sunrealtype)cv->nni);
+      stats_real(outfile, fmt, 0, "Jac evals per NLS iter",
+                 (sunrealtype)cv->ls.nje / (sunrealtype)cv->nni);
+      stats_real(outfile, fmt, 0, "Prec evals per NLS iter", ZERO);
+    }
+  }
+  stats_long(outfile, fmt, 0, "Root fn evals", 0L);
+  return CV_SUCCESS;
+}
+
 void CVodeFree(void** cvode_mem)
 {
   if (!cvode_mem || !*cvode_mem) { return; }

@@ -36,6 +36,17 @@ def parse_out(path):
     return returns, final
 
 
+def final_table_lines(path):
+    lines = open(path).read().splitlines()
+    i = lines.index("Final Statistics:")
+    out = []
+    for ln in lines[i + 1:]:
+        if " = " not in ln:
+            break
+        out.append(ln)
+    return out
+
+
 def parse_csv(path):
     rows = []
     for ln in open(path).read().splitlines():
@@ -59,6 +70,8 @@ def main():
                   "t1": 0.4, "tmult": 10.0, "nout": 12},
         "returns": returns,
         "final": final,
+        # the final statistics table as printed (CVodePrintAllStats, TABLE)
+        "final_table_lines": final_table_lines(os.path.join(EX, "cvRoberts_dns.out")),
     }
     json.dump(doc, open(OUT, "w"), indent=1)
     print("wrote", OUT, len(returns), "returns")

@@ -41,6 +41,8 @@ HEADERS = {
 
 # declared in a header whose functions live in the other library
 SPLIT = {"batched_bdf.h": {"SUNAMD_BatchCreate", "SUNAMD_BatchFree"},
+         "learned_predictor.h": {"SUNAMD_TrainingLogWrite", "SUNAMD_TrainingLogInfoRead",
+                                 "SUNAMD_TrainingLogRead"},
          "imex_adr.h": {"SUNAMD_AdrSolve_Mi355"}}
 
 _DECL = re.compile(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", re.M)

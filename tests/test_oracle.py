@@ -2,7 +2,9 @@
 Newton) against the reference's own golden outputs and known answers.
 
 - cvRoberts_dns: every counter and step size of the published run
-  (examples/cvode/serial/cvRoberts_dns.out, cvRoberts_dns_stats.csv).
+  (examples/cvode/serial/cvRoberts_dns.out, cvRoberts_dns_stats.csv), and
+  CVodePrintAllStats' two formats: the TABLE text line for line as the
+  example prints it, the CSV field for field against the stats file.
 - N_Vector known answers from test/unit_tests/nvector/test_nvector.c
   (constant fills with closed-form results).
 - Dense LU: block-diagonal full-dense GETRF equals per-block GETRF bit for
@@ -48,6 +50,46 @@ def ctx():
 # ----------------------------------------------------------------------------
 # BDF integrator pinned by the published cvRoberts_dns run
 # ----------------------------------------------------------------------------
+def test_roberts_print_all_stats(orc, tmp_path):
+    """CVodePrintAllStats after cvRoberts_dns's run: the TABLE equals the
+    example's "Final Statistics" text line for line (names and %.15g value
+    text), except "Root fn evals" (rootfinding is not restated: 0 here);
+    each normal return's CSV line has the stats file's fields in its order
+    with equal values (the published file is from a release that named the
+    first field "Time" and printed reals with 16 digits; the current writer
+    prints "Current time" and "% .15e")."""
+    g = json.load(open(os.path.join(HERE, "golden", "cvRoberts_dns.json")))
+    csv_path, tab_path = tmp_path / "stats.csv", tmp_path / "stats.txt"
+    assert orc.oracle_roberts_stats(12, str(csv_path).encode(), str(tab_path).encode()) == 0
+    ours = open(tab_path).read().splitlines()
+    ref = g["final_table_lines"]
+    assert len(ours) == len(ref)
+    for a, b in zip(ours, ref):
+        # the writer pads names to 29 columns (sundials_utils.h:31); the
+        # published text was printed with 28: compare names and value text
+        assert a.index(" = ") == 29
+        na, va = (x.strip() for x in a.split(" = "))
+        nb, vb = (x.strip() for x in b.split(" = "))
+        assert na == nb
+        assert va == ("0" if nb == "Root fn evals" else vb), (na, va, vb)
+    rows = open(csv_path).read().splitlines()
+    normal = [r["stats"] for r in g["returns"] if not r["root"]]
+    assert len(rows) == len(normal) == 12
+    for row, ref_row in zip(rows, normal):
+        f = row.split(",")
+        pairs = [(f[k], f[k + 1]) for k in range(0, len(f), 2)]
+        names = [("Time" if n == "Current time" else n) for n, _ in pairs]
+        assert names == list(ref_row.keys())
+        for (n, v), rn in zip(pairs, ref_row):
+            if rn == "Root fn evals":
+                continue
+            if "." in v or "e" in v:
+                assert v.startswith(" ") or v.startswith("-")  # "% .15e"
+                assert float(v) == pytest.approx(ref_row[rn], rel=1e-15, abs=0), (n, v)
+            else:
+                assert int(v) == ref_row[rn], (n, v)
+
+
 def test_roberts_matches_published_run(orc):
     g = json.load(open(os.path.join(HERE, "golden", "cvRoberts_dns.json")))
     normal = [r for r in g["returns"] if not r["root"]]
