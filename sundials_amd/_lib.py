@@ -151,6 +151,12 @@ ADR_STAT_NAMES = ("nst", "attempts", "netf", "ncfn", "nfe", "nfi", "nni", "nnf",
                   "nli", "npe", "nps", "njtimes", "ncfl", "wall", "hlast")
 
 
+class TrainingLogInfo(C.Structure):
+    """Mirror of SUNAMD_TrainingLogInfo (include/sunamd/learned_predictor.h)."""
+    _fields_ = [("version", C.c_int), ("rows", C.c_long), ("seed", C.c_uint64),
+                ("rtol", C.c_double), ("atol", C.c_double), ("stride", C.c_int)]
+
+
 class Batch(C.Structure):
     """Mirror of SUNAMD_Batch (include/sunamd/batched_bdf.h)."""
     _fields_ = [
@@ -265,6 +271,11 @@ def core():
     _sig(lib, "SUNAMD_Pow", c_real, [c_real, c_real])
     _sig(lib, "SUNAMD_AdrCreate", P(Adr), [C.c_int, C.c_uint64])
     _sig(lib, "SUNAMD_AdrFree", None, [P(Adr)])
+    _sig(lib, "SUNAMD_TrainingLogWrite", C.c_int,
+         [C.c_char_p, c_void_p, c_void_p, c_void_p, C.c_long, P(TrainingLogInfo)])
+    _sig(lib, "SUNAMD_TrainingLogInfoRead", C.c_int, [C.c_char_p, P(TrainingLogInfo)])
+    _sig(lib, "SUNAMD_TrainingLogRead", C.c_int,
+         [C.c_char_p, C.c_long, C.c_long, c_void_p, c_void_p, c_void_p])
     _sig(lib, "SUNAMD_ARKTable", C.c_int, [C.c_int, C.c_int, P(C.c_int)] + [P(c_real)] * 4 +
          [P(C.c_int), P(C.c_int)])
     _sig(lib, "SUNAMD_ARK324Table", C.c_int, [C.c_int, P(c_real), P(c_real), P(c_real),
@@ -382,6 +393,7 @@ def load_oracle():
     _sig(lib, "oracle_block_getrf", c_index, [P(c_real), c_index, P(c_index)])
     _sig(lib, "oracle_block_getrs", None, [P(c_real), c_index, P(c_index), P(c_real)])
     _sig(lib, "oracle_roberts_run", C.c_int, [C.c_int, P(c_real)])
+    _sig(lib, "oracle_roberts_stats", C.c_int, [C.c_int, C.c_char_p, C.c_char_p])
     _sig(lib, "oracle_massaction_solve_coupled", C.c_int,
          [P(MassAction), c_real, c_real, c_real, P(c_real), P(c_real)])
     _sig(lib, "oracle_massaction_solve_per_cell", C.c_int,
