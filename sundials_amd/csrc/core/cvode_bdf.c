@@ -118,8 +118,14 @@
 #define SUNRsqrt(x)  ((x) <= 0.0 ? 0.0 : sqrt((x)))
 /* SUNRpowerR is pow() in the reference (sundials_math.h:235); the correctly
    rounded sunamd_pow gives the same result wherever libm's pow is correctly
-   rounded, and the same result on host and device everywhere */
+   rounded, and the same result on host and device everywhere.  A build with
+   SUNAMD_SUNRPOWER_LIBM defined uses the C library's pow instead, as the
+   reference does (used only to measure what the substitution changes). */
+#ifdef SUNAMD_SUNRPOWER_LIBM
+#define SUNRpowerR(b, e) pow((b), (e))
+#else
 #define SUNRpowerR(b, e) sunamd_pow((b), (e))
+#endif
 
 /* ---- integrator state ---------------------------------------------------- */
 typedef struct
