@@ -89,6 +89,7 @@ int CVodeSetMaxOrd(void* cvode_mem, int maxord);
 int CVodeSetInitStep(void* cvode_mem, sunrealtype hin);
 int CVodeSetMinStep(void* cvode_mem, sunrealtype hmin);
 int CVodeSetMaxStep(void* cvode_mem, sunrealtype hmax);
+int CVodeSetLSetupFrequency(void* cvode_mem, long int msbp);
 int CVodeSetStopTime(void* cvode_mem, sunrealtype tstop);
 int CVodeSetNonlinearSolver(void* cvode_mem, SUNNonlinearSolver NLS);
 int CVodeSetLinearSolver(void* cvode_mem, SUNLinearSolver LS, SUNMatrix A);

@@ -199,6 +199,7 @@ typedef struct CVodeMemRec
   sunrealtype h0u, hu;
   int qu;
   long int ninit_used;
+  sunbooleantype nonzero_start; /* this step's Newton iteration started from a proposal */
 
   /* nonlinear solver */
   SUNNonlinearSolver NLS;
@@ -501,6 +502,16 @@ int CVodeSetMinStep(void* cvode_mem, sunrealtype hmin)
   }
   if (hmin * cv->hmax_inv > ONE) { return CV_ILL_INPUT; }
   cv->hmin = hmin;
+  return CV_SUCCESS;
+}
+
+/* cvode_io.c:847: 0 restores the default (20 steps), negative is an error */
+int CVodeSetLSetupFrequency(void* cvode_mem, long int msbp)
+{
+  CVodeMem cv = (CVodeMem)cvode_mem;
+  if (!cv) { return CV_MEM_NULL; }
+  if (msbp < 0) { return CV_ILL_INPUT; }
+  cv->msbp = (msbp == 0) ? MSBP_DEFAULT : msbp;
   return CV_SUCCESS;
 }
 
@@ -834,7 +845,10 @@ static int cvNlsConvTest(SUNNonlinearSolver NLS, N_Vector ycor, N_Vector delta,
   sunrealtype dcon = del * SUNMIN(ONE, cv->crate) / tol;
   if (dcon <= ONE)
   {
-    cv->acnrm    = (m == 0) ? del : N_VWrmsNorm(ycor, ewt);
+    /* after one iteration from a zero start the correction is the update
+       itself (the reference's shortcut); from a proposed start its norm has
+       to be taken (after a restart from zero it equals del bit for bit) */
+    cv->acnrm    = (m == 0 && !cv->nonzero_start) ? del : N_VWrmsNorm(ycor, ewt);
     cv->acnrmcur = SUNTRUE;
     return CV_SUCCESS;
   }
@@ -1194,11 +1208,13 @@ static int cvNls(CVodeMem cv, int nflag)
 
   /* initial guess for the correction: zero, or the caller's proposal */
   N_VConst(ZERO, cv->acor);
+  cv->nonzero_start = SUNFALSE;
   if (cv->initcor)
   {
     if (cv->initcor(cv->tn, cv->h, cv->gamma, cv->zn[0], cv->acor, cv->user_data) == 0)
     {
       cv->ninit_used++;
+      cv->nonzero_start = SUNTRUE;
     }
     else { N_VConst(ZERO, cv->acor); }
   }
