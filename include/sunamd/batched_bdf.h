@@ -111,6 +111,32 @@ int SUNAMD_BatchedBDFGetLearnedStats(SUNAMD_BatchedBDF* B, long int* tried, long
 int SUNAMD_BatchedBDFGetKernelTimes(SUNAMD_BatchedBDF* B, double* ms);
 void SUNAMD_BatchedBDFFree(SUNAMD_BatchedBDF* B);
 
+/* Allocation-free integration of one size bucket.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  It runs the same kernel as SUNAMD_BatchedBDFRun (two
+   wavefronts per SIMD) on ncells systems of m species with the R reactions
+   of idx, all caller-owned device buffers laid out as in SUNAMD_MassAction:
+   idx[4R], kf[ncells*R], kb[ncells*R] (cell-major), y0[ncells*m] ->
+   yout[ncells*m], counters[ncells*SUNAMD_BS_COUNT].  The work queue, the
+   validated reaction table and the per-species term lists live in a fixed
+   device array of the library (about 33 KB per device), so R is limited to
+   SUNAMD_NOALLOC_MAX_REACTIONS; m must be one of 1-8, 10, 12, 16, 24, 32,
+   48, 64.  mxsteps as in SUNAMD_BatchedBDFSetMaxNumSteps.  The call is
+   asynchronous on `stream` (a hipStream_t of the current device; NULL is
+   the default stream) and calls no allocator; the results are complete
+   when the stream is.  An index of idx outside [0, m) is caught on the
+   device: every system then returns y0 with flag -22 (CV_ILL_INPUT).
+   Calls that overlap on one device share the array and race: serialise
+   them (one stream, or events).  Returns 0 when queued, -1 for unsupported
+   sizes or NULL buffers (nothing queued), -2 on a HIP error. */
+#define SUNAMD_NOALLOC_MAX_REACTIONS 1024
+int SUNAMD_BatchedBDFSolveBucketNoAlloc(int m, int R, int64_t ncells, const int* idx,
+                                        const double* kf, const double* kb, const double* y0,
+                                        double rtol, double atol, double tout, long int mxsteps,
+                                        int lu_precision, double* yout, long int* counters,
+                                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -357,6 +357,9 @@ def hip():
          [c_void_p, c_void_p, c_void_p, c_void_p, C.c_long, C.c_int])
     _sig(lib, "SUNAMD_BatchedBDFGetTrainingLogCount", C.c_long, [c_void_p])
     _sig(lib, "SUNAMD_BatchedBDFGetLearnedStats", C.c_int, [c_void_p, P(C.c_long), P(C.c_long)])
+    _sig(lib, "SUNAMD_BatchedBDFSolveBucketNoAlloc", C.c_int,
+         [C.c_int, C.c_int, C.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_real, c_real,
+          c_real, C.c_long, C.c_int, c_void_p, c_void_p, c_void_p])
     PFl = P(C.c_float)
     _sig(lib, "SUNAMD_MLPCreate", c_void_p, [C.c_uint64, C.c_int])
     _sig(lib, "SUNAMD_MLPFree", None, [c_void_p])

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "sunamd/batched_bdf.h"
 #include "sunamd/sunmatrix_mi355block.h"
@@ -468,4 +469,196 @@ extern "C" void SUNAMD_BatchedBDFFree(SUNAMD_BatchedBDF* B)
   free(B->sys_cell);
   free(B->sys_yoff);
   free(B);
+}
+
+// -----------------------------------------------------------------------------
+// Allocation-free integration of one size bucket
+// -----------------------------------------------------------------------------
+// The caller owns every input and output buffer; the only device memory beyond
+// them is this file-scope array: the work-queue ticket, the validation flag,
+// the reaction table as validated, and the per-species term lists the kernel
+// reads.  A one-workgroup kernel fills it on the caller's stream, so a call
+// touches no host memory after its argument checks.  Each device holds its own
+// copy; calls that overlap on one device share it and race.
+namespace
+{
+constexpr int NA_NEXT  = 0;                            // 2 ints: u64 work-queue ticket
+constexpr int NA_BAD   = 2;                            // 1 int: table failed validation
+constexpr int NA_SOFF  = 64;                           // up to 72 ints: m+1 offsets
+constexpr int NA_IDX   = 136;                          // 4R ints: reaction table (validated)
+constexpr int NA_SENT  = NA_IDX + 4 * SUNAMD_NOALLOC_MAX_REACTIONS; // 4R ints: term lists
+constexpr int NA_INTS  = NA_SENT + 4 * SUNAMD_NOALLOC_MAX_REACTIONS;
+constexpr int MAX_DEVS = 64;
+} // namespace
+
+static __device__ __align__(256) int scratch[NA_INTS];
+
+static std::mutex g_scratch_mu;
+static int* g_scratch_ptr[MAX_DEVS];
+
+// device address of `scratch` on the current device (looked up once per device)
+static int* scratch_address()
+{
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+  {
+    hfail(e, "hipGetDevice");
+    return nullptr;
+  }
+  if (dev < 0 || dev >= MAX_DEVS) { return nullptr; }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr)
+  {
+    void* p = nullptr;
+    e       = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+    {
+      hfail(e, "hipGetSymbolAddress(scratch)");
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = (int*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+// Validates the reaction table (every index in [0, m)), copies it into the
+// scratch, and builds species s's term list (4*reaction + role, in reaction
+// and role order: the order the host integrator accumulates f_s and row s of
+// the Jacobian in).  An invalid table leaves zeros in the copy and sets the
+// flag, which makes the integrator return every system unintegrated.
+__global__ void __launch_bounds__(64) k_noalloc_prepare(const int* __restrict__ idx, int m, int R,
+                                                        int* __restrict__ sc)
+{
+  __shared__ int s_idx[4 * SUNAMD_NOALLOC_MAX_REACTIONS];
+  __shared__ int s_sent[4 * SUNAMD_NOALLOC_MAX_REACTIONS];
+  __shared__ int s_off[72];
+  __shared__ int s_bad;
+  const int t = threadIdx.x;
+  if (t == 0) { s_bad = 0; }
+  __syncthreads();
+  for (int e = t; e < 4 * R; e += 64)
+  {
+    const int v = idx[e];
+    const bool ok = (v >= 0) && (v < m);
+    s_idx[e]      = ok ? v : 0;
+    if (!ok) { atomicOr(&s_bad, 1); }
+  }
+  __syncthreads();
+  if (t == 0)
+  {
+    for (int s = 0; s <= m; s++) { s_off[s] = 0; }
+    if (s_bad == 0)
+    {
+      for (int e = 0; e < 4 * R; e++) { s_off[s_idx[e] + 1]++; }
+      for (int s = 0; s < m; s++) { s_off[s + 1] += s_off[s]; }
+      // s_sent doubles as the fill cursor's home: cursors in s_off shifted back
+      for (int e = 0; e < 4 * R; e++)
+      {
+        const int s        = s_idx[e];
+        s_sent[s_off[s]++] = e;
+      }
+      for (int s = m; s > 0; s--) { s_off[s] = s_off[s - 1]; }
+      s_off[0] = 0;
+    }
+    else
+    {
+      for (int e = 0; e < 4 * R; e++) { s_sent[e] = 0; }
+    }
+    sc[NA_NEXT]     = 0;
+    sc[NA_NEXT + 1] = 0;
+    sc[NA_BAD]      = s_bad;
+  }
+  __syncthreads();
+  for (int e = t; e < 4 * R; e += 64)
+  {
+    sc[NA_IDX + e]  = s_idx[e];
+    sc[NA_SENT + e] = s_sent[e];
+  }
+  for (int s = t; s <= m; s += 64) { sc[NA_SOFF + s] = s_off[s]; }
+}
+
+extern "C" int SUNAMD_BatchedBDFSolveBucketNoAlloc(int m, int R, int64_t ncells, const int* idx,
+                                                    const double* kf, const double* kb,
+                                                    const double* y0, double rtol, double atol,
+                                                    double tout, long int mxsteps,
+                                                    int lu_precision, double* yout,
+                                                    long int* counters, void* stream)
+{
+  static const int sizes[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16, 24, 32, 48, 64};
+  bool known = false;
+  for (int s : sizes) { known = known || (s == m); }
+  if (!known || R < 1 || R > SUNAMD_NOALLOC_MAX_REACTIONS || ncells < 0 ||
+      (lu_precision != SUNAMD_LU_FP64 && lu_precision != SUNAMD_LU_FP32))
+  {
+    fprintf(stderr,
+            "[sundials_amd batched_bdf] no-alloc solve: unsupported m = %d / R = %d (1..%d) / "
+            "ncells = %lld / precision %d\n",
+            m, R, SUNAMD_NOALLOC_MAX_REACTIONS, (long long)ncells, lu_precision);
+    return -1;
+  }
+  const int gpw = 64 / pow2ceil(m);
+  if (sunamd_bdf::bucket_lds_bytes(m, R, gpw, false) > 60000)
+  {
+    fprintf(stderr, "[sundials_amd batched_bdf] no-alloc solve: m=%d R=%d needs too much LDS\n",
+            m, R);
+    return -1;
+  }
+  if (ncells == 0) { return 0; }
+  if (!idx || !kf || !kb || !y0 || !yout || !counters) { return -1; }
+  int* sc = scratch_address();
+  if (sc == nullptr) { return -2; }
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        v > 0)
+    {
+      cus = v;
+    }
+  }
+  hipStream_t st = (hipStream_t)stream;
+  k_noalloc_prepare<<<1, 64, 0, st>>>(idx, m, R, sc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { return hfail(e, "no-alloc prepare launch") - 1; }
+
+  BucketArgs A;
+  memset(&A, 0, sizeof(A));
+  A.m      = m;
+  A.R      = R;
+  A.ncells = (long)ncells;
+  A.idx    = sc + NA_IDX;
+  A.soff   = sc + NA_SOFF;
+  A.sent   = sc + NA_SENT;
+  A.kf     = kf;
+  A.kb     = kb;
+  A.y0     = y0;
+  A.yout   = yout;
+  A.stats  = counters;
+  A.next   = (unsigned long long*)(sc + NA_NEXT);
+  A.bad    = sc + NA_BAD;
+  A.rtol   = rtol;
+  A.atol   = atol;
+  A.tout   = tout;
+  A.mxstep = (mxsteps == 0) ? MXSTEP_DEFAULT : mxsteps;
+  const long want = (A.ncells + gpw - 1) / gpw;
+  const long cap  = (long)cus * 16;
+  const int grid  = (int)(want < cap ? want : cap);
+  // two wavefronts per SIMD: the register budget the autotuner picks for
+  // every bucket of the benchmark batch
+  switch (m)
+  {
+#define BDF_CASE(MM)                                                                     \
+  case MM:                                                                               \
+    if (lu_precision == SUNAMD_LU_FP32) { sunamd_bdf::launch<MM, float>(A, grid, st, 2); } \
+    else { sunamd_bdf::launch<MM, double>(A, grid, st, 2); }                             \
+    break;
+    BDF_SIZES(BDF_CASE)
+#undef BDF_CASE
+  default: return -1;
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess) { return hfail(e, "no-alloc batched BDF launch") - 1; }
+  return 0;
 }

@@ -43,6 +43,9 @@ struct BucketArgs
   int log_stride;
   // learned-guess counters of the bucket: [steps tried, guesses kept]
   unsigned long long* lstats;
+  // non-null: a flag the reaction table's validation set on the device; when
+  // it is nonzero every system is returned unintegrated with flag -22
+  const int* bad;
 };
 
 // LDS floats per lane group for the MLP inputs and hidden layers

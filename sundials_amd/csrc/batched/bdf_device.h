@@ -1479,6 +1479,12 @@ __device__ __forceinline__ bool g_acquire(Grp<M, T>& g, Sys<M, T>& S, double tou
       S.ratef = g.valid ? (float)(log10(1.0 + ks) * 0.1) : 0.0f;
     }
     for (int k = 0; k < QMAX + 2; k++) { g.sc->tau[k] = 0.0; }
+    if (A.bad != nullptr && *A.bad != 0)
+    {
+      // the reaction table failed validation (CV_ILL_INPUT): nothing is integrated
+      g_store(g, S, S.zn[0], -22);
+      continue;
+    }
 
     S.ewt   = g_ewt(g, S.zn[0]);
     S.zn[1] = g_rhs(g, S.zn[0]);
