@@ -103,6 +103,80 @@ def cpu_baseline(core, cells):
                                                            os.cpu_count() or 0)}
 
 
+def vector_cpu_baseline(n):
+    """The oracle's serial vector ops and their OpenMP form on the host cores,
+    at the microbench's length, one timed call each after a warm-up."""
+    from sundials_amd import _lib
+
+    orc = _lib.load_oracle()
+    out = {}
+    for threads in (1, min(16, os.cpu_count() or 1)):
+        ms = np.zeros(5)
+        if orc.oracle_bench_vector_ops(n, 8, 1, threads,
+                                       ms.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+            return None
+        gbs = [24 * n, 16 * n, 72 * n, 24 * n, 16 * n] / (ms * 1e-3) / 1e9
+        out["threads_%d" % threads] = {
+            "linearsum_gbs": round(gbs[0], 2), "wrmsnorm_gbs": round(gbs[1], 2),
+            "dotprodmulti8_gbs": round(gbs[2], 2), "axpy_inplace_gbs": round(gbs[3], 2),
+            "dotprod_gbs": round(gbs[4], 2), "ms": [round(v, 2) for v in ms]}
+    out["sample"] = "oracle vector ops on the host, N = %d, %s, nproc=%d" % (
+        n, cpu_model(), os.cpu_count() or 0)
+    return out
+
+
+def vector_microbench(args, hip):
+    """The N_Vector microbench at N = --micro-n on the device (HIP events), with
+    copy ceilings measured in the same process and the host baseline."""
+    ms = np.zeros(8)
+    if hip.SUNAMD_BenchVectorOps(args.micro_n, 8, 20,
+                                 ms.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+        return None
+    n = args.micro_n
+    names = ("linearsum", "wrmsnorm", "dotprodmulti8", "axpy_inplace", "dotprod")
+    gbs = np.array([24, 16, 72, 24, 16, 16, 16, 16], dtype=np.float64) * n / (ms * 1e-3) / 1e9
+    ceiling = float(max(gbs[5], gbs[6], gbs[7]))
+    micro = {"n": n}
+    for i, k in enumerate(names):
+        micro[k + "_gbs"] = round(float(gbs[i]), 1)
+    micro["copy_gbs"] = {"chunked_256": round(float(gbs[5]), 1),
+                         "chunked_1024": round(float(gbs[6]), 1),
+                         "hipMemcpy": round(float(gbs[7]), 1)}
+    micro["frac_of_hbm_peak"] = {k: round(float(gbs[i]) / HBM_PEAK_GBS, 3)
+                                 for i, k in enumerate(names)}
+    micro["frac_of_copy_ceiling"] = {k: round(float(gbs[i]) / ceiling, 3)
+                                     for i, k in enumerate(names)}
+    micro["ms"] = [round(float(v), 4) for v in ms]
+    if not args.no_cpu:
+        micro["cpu_baseline"] = vector_cpu_baseline(n)
+    return micro
+
+
+def bench_vector(args):
+    """--workload vector: the N_Vector microbench alone (LinearSum, WrmsNorm,
+    DotProdMulti(8), in-place LinearSum, DotProd at N = 2^28 FP64) against the
+    copy ceiling and HBM peak; value = LinearSum GB/s."""
+    from sundials_amd import _lib
+
+    micro = vector_microbench(args, _lib.hip())
+    if micro is None:
+        print("[bench] microbench failed", file=sys.stderr)
+        sys.exit(1)
+    ms = micro["ms"][0]
+    return {"metric": "N_Vector op HBM GB/s (LinearSum z = a x + b y, algorithmic bytes)",
+            "value": micro["linearsum_gbs"], "unit": "GB/s", "n_gpus": 1, "steps": 20,
+            "warmup": 1, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "N_Vector microbench, N = %d FP64, x, y ~ U(-1,1), "
+                                   "w ~ U(1e-3,1), DotProdMulti over 8 vectors" % args.micro_n},
+            "roofline": {"bound": "hbm", "achieved": micro["linearsum_gbs"],
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(micro["linearsum_gbs"] / HBM_PEAK_GBS, 4),
+                         "traffic": 24.0 * args.micro_n},
+            "cpu_baseline": micro.get("cpu_baseline"),
+            "microbench": micro}
+
+
 def bench_coupled(args):
     import torch
     from sundials_amd import _lib
@@ -172,19 +246,7 @@ def bench_coupled(args):
                      "value": ncells * st.nst / st.wall_s}
         hip.SUNAMD_CoupledSetFusedKernels_Mi355(run, 0)
 
-    micro = None
-    if not args.no_micro:
-        ms = np.zeros(5)
-        if hip.SUNAMD_BenchVectorOps(args.micro_n, 8, 20, ms.ctypes.data_as(
-                C.POINTER(C.c_double))) == 0:
-            n = args.micro_n
-            micro = {"n": n,
-                     "linearsum_gbs": round(24 * n / (ms[0] * 1e-3) / 1e9, 1),
-                     "wrmsnorm_gbs": round(16 * n / (ms[1] * 1e-3) / 1e9, 1),
-                     "dotprodmulti8_gbs": round(72 * n / (ms[2] * 1e-3) / 1e9, 1),
-                     "axpy_inplace_gbs": round(24 * n / (ms[3] * 1e-3) / 1e9, 1),
-                     "dotprod_gbs": round(16 * n / (ms[4] * 1e-3) / 1e9, 1),
-                     "ms": [round(v, 4) for v in ms]}
+    micro = None if args.no_micro else vector_microbench(args, hip)
 
     hip.SUNAMD_CoupledFree_Mi355(run)
     core.SUNAMD_MassActionFree(prob)
@@ -285,9 +347,14 @@ def _batch(core, nsys, seed):
 
 
 def train_predictor(args, core, hip):
-    """Training pairs from plain FP64 runs of an independent batch, then Adam
-    on the GPU with the library's MFMA kernels.  Returns (network, info)."""
+    """Training pairs from plain FP64 runs of an independent batch (one row per
+    step, up to --train-pairs rows), a sample of them round-tripped through the
+    versioned training-pair log file, then Adam on the GPU with the library's
+    MFMA kernels over one pass of the pairs.  Returns (network, info)."""
+    import tempfile
+
     import torch
+    from sundials_amd import _lib
 
     t0 = time.perf_counter()
     bt = _batch(core, args.train_systems, TRAIN_SEED)
@@ -306,23 +373,44 @@ def train_predictor(args, core, hip):
     n = int(hip.SUNAMD_BatchedBDFGetTrainingLogCount(B))
     hip.SUNAMD_BatchedBDFFree(B)
     core.SUNAMD_BatchFree(bt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    # the first rows through the on-disk format and back, compared exactly
+    ns = min(n, 100000)
+    hx, ht, hm = (a[:ns].cpu().numpy() for a in (x, t, m))
+    info = _lib.TrainingLogInfo(1, ns, TRAIN_SEED, RTOL, ATOL, 1)
+    rx, rt, rm = np.zeros_like(hx), np.zeros_like(ht), np.zeros_like(hm)
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "pairs.sunamdtp").encode()
+        ok = (core.SUNAMD_TrainingLogWrite(path, hx.ctypes.data, ht.ctypes.data, hm.ctypes.data,
+                                           ns, C.byref(info)) == 0 and
+              core.SUNAMD_TrainingLogRead(path, 0, ns, rx.ctypes.data, rt.ctypes.data,
+                                          rm.ctypes.data) == 0)
+        fsize = os.path.getsize(path) if ok else 0
+    roundtrip = bool(ok and np.array_equal(hx, rx) and np.array_equal(ht, rt)
+                     and np.array_equal(hm, rm))
+    if not roundtrip:
+        raise RuntimeError("training-pair log round trip failed")
     perm = torch.randperm(n, device="cuda", generator=torch.Generator("cuda").manual_seed(0))
     x, t, m = x[:n][perm].contiguous(), t[:n][perm].contiguous(), m[:n][perm].contiguous()
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    M = hip.SUNAMD_MLPCreate(17, 1024)
+    t2 = time.perf_counter()
+    batch = 1024
+    steps = args.train_steps if args.train_steps > 0 else max(1, n // batch)
+    M = hip.SUNAMD_MLPCreate(17, batch)
     loss = C.c_float()
     if hip.SUNAMD_MLPTrain(M, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
-                           C.c_void_p(m.data_ptr()), n, 1024, args.train_steps, 1e-3,
-                           C.byref(loss)) != 0:
+                           C.c_void_p(m.data_ptr()), n, batch, steps, 1e-3, C.byref(loss)) != 0:
         raise RuntimeError("training failed")
-    t2 = time.perf_counter()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
     del x, t, m
-    info = {"pairs": n, "train_systems": args.train_systems, "steps": args.train_steps,
-            "batch": 1024, "loss": float(loss.value), "log_s": round(t1 - t0, 2),
-            "train_s": round(t2 - t1, 2)}
-    print("[bench] predictor: %d pairs, loss %.4g, train %.1f s" % (n, loss.value, t2 - t1),
-          file=sys.stderr, flush=True)
+    info = {"pairs": n, "train_systems": args.train_systems, "steps": steps, "batch": batch,
+            "samples_seen": steps * batch, "loss": float(loss.value),
+            "log_s": round(t1 - t0, 2), "train_s": round(t3 - t2, 2),
+            "log_file_roundtrip": {"rows": ns, "bytes": fsize, "identical": roundtrip}}
+    print("[bench] predictor: %d pairs, %d steps x %d, loss %.4g, train %.1f s"
+          % (n, steps, batch, loss.value, t3 - t2), file=sys.stderr, flush=True)
     return M, info
 
 
@@ -866,9 +954,44 @@ def bench_batched(args):
         hip.SUNAMD_BatchedBDFGetLearnedStats(B, C.byref(tried), C.byref(kept))
         learned["steps_tried"], learned["guesses_kept"] = tried.value, kept.value
     hip.SUNAMD_BatchedBDFFree(B)
-    if mlp is not None:
-        hip.SUNAMD_MLPFree(mlp)
     nst32 = int(c32[:, 0].sum())
+    # the learned start in every bucket of the measured batch (whatever the
+    # autotuner kept), beside the timed product form: per bucket kernel time,
+    # Newton iterations per step and guesses kept
+    learned_all, y_la = None, None
+    if mlp is not None:
+        BL = make(LU_FP32, w32)
+        hip.SUNAMD_BatchedBDFSetLearnedPredictor(BL, hip.SUNAMD_MLPDeviceWeights(mlp),
+                                                 (1 << nb) - 1)
+        run(BL, "learned warmup")
+        t_la = run(BL, "learned, every bucket")
+        y_la, c_la = results(BL)
+        kms_la = np.zeros(nb)
+        hip.SUNAMD_BatchedBDFGetKernelTimes(BL, kms_la.ctypes.data_as(C.POINTER(C.c_double)))
+        tr = np.zeros(nb, dtype=np.int64)
+        kp = np.zeros(nb, dtype=np.int64)
+        hip.SUNAMD_BatchedBDFGetLearnedStatsByBucket(BL, tr.ctypes.data_as(C.POINTER(C.c_long)),
+                                                     kp.ctypes.data_as(C.POINTER(C.c_long)))
+        hip.SUNAMD_BatchedBDFFree(BL)
+        per = {}
+        for k in range(nb):
+            sel = sb == k
+            per["m%d" % bm[k]] = {
+                "kernel_ms": round(float(kms_la[k]), 3),
+                "kernel_ms_without": round(float(kms[k]), 3),
+                "newton_iters_per_step": round(float(c_la[sel, 4].sum() / c_la[sel, 0].sum()), 4),
+                "newton_iters_per_step_without": round(float(c32[sel, 4].sum() / c32[sel, 0].sum()),
+                                                       4),
+                "steps": int(c_la[sel, 0].sum()), "steps_without": int(c32[sel, 0].sum()),
+                "guesses_tried": int(tr[k]), "guesses_kept": int(kp[k])}
+        nst_la = int(c_la[:, 0].sum())
+        learned_all = {"ms": round(1e3 * t_la, 3), "value": nst_la / t_la,
+                       "newton_iters_per_step": float(c_la[:, 4].sum() / max(nst_la, 1)),
+                       "speedup_vs_plain_fp64": (nst_la / t_la) / (nst64 / t64),
+                       "buckets": per}
+        print("[bench] learned start in every bucket: %.3f s, %.4f Newton iters/step"
+              % (t_la, learned_all["newton_iters_per_step"]), file=sys.stderr, flush=True)
+        hip.SUNAMD_MLPFree(mlp)
 
     # roofline: the longest bucket kernel of the last timed run
     fl = batch_flops(c32, msys, rsys)
@@ -932,6 +1055,9 @@ def bench_batched(args):
             yo, ms_ = yoff[:ns], msys[:ns]
             e64 = _wrms_per_system(y64[:nyc], yt, yo, ms_)
             e32 = _wrms_per_system(y32[:nyc], yt, yo, ms_)
+            if learned_all is not None:
+                learned_all["error_vs_tight_reference"] = _err_summary(
+                    _wrms_per_system(y_la[:nyc], yt, yo, ms_))
             worst = np.argsort(-np.maximum(e64, e32))[:5]
             accuracy = {
                 "reference": "CPU oracle at rtol %g atol %g on the same %d systems (%.1f s)"
@@ -982,6 +1108,7 @@ def bench_batched(args):
         "fp64_parity_vs_cpu_oracle": parity,
         "error_vs_tight_reference": accuracy,
         "learned": learned,
+        "learned_every_bucket": learned_all,
         "autotune": {"fp64": tune64, "fp32": tune32},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -990,12 +1117,91 @@ def bench_batched(args):
     return line
 
 
+PHASE_NAMES = ("acquire", "guard", "predict", "newton", "rhs", "jac", "getrf", "getrs", "norm",
+               "errtest", "complete", "prepare", "pow", "learn", "output", "idle")
+
+
+def bench_phases(args):
+    """Where a step's time goes: the uneven batch integrated by the phase-clock
+    build of the batched integrator (two wavefronts per SIMD, learned start
+    off), FP64 and FP32-LU forms.  Per bucket: shader-clock ticks per step
+    spent in each phase by the group that owns a system (for m > 32 the group
+    is the whole wavefront), summed over the bucket's groups and divided by
+    its steps."""
+    from sundials_amd import _lib
+
+    core, hp = _lib.core(), _lib.hip_phase()
+    b = _batch(core, args.systems, BATCH_SEED)
+    bc = b.contents
+    nb = bc.nbuckets
+    bm = [bc.bucket[k].contents.m for k in range(nb)]
+    sb = np.ctypeslib.as_array(bc.sys_bucket, shape=(bc.nsys,)).copy()
+    out = {}
+    for prec, tag in ((LU_FP64, "fp64"), (LU_FP32, "fp32_lu")):
+        B = hp.SUNAMD_BatchedBDFCreate(b, RTOL, ATOL, prec)
+        hp.SUNAMD_BatchedBDFSetMaxNumSteps(B, BATCH_MXSTEP)
+        hp.SUNAMD_BatchedBDFSetOccupancy(B, (C.c_int * nb)(*([2] * nb)))
+        wall = C.c_double()
+        for _ in range(2):
+            if hp.SUNAMD_BatchedBDFRun(B, TOUT, C.byref(wall)) != 0:
+                raise RuntimeError("phase run failed")
+        NV = 2 * 16 + 2
+        raw = np.zeros(nb * NV, dtype=np.uint64)
+        if hp.SUNAMD_BatchedBDFGetPhaseTicks(
+                B, raw.ctypes.data_as(C.POINTER(C.c_ulonglong))) != nb:
+            raise RuntimeError("phase ticks unavailable")
+        raw = raw.reshape(nb, NV).astype(np.float64)
+        kms = np.zeros(nb)
+        hp.SUNAMD_BatchedBDFGetKernelTimes(B, kms.ctypes.data_as(C.POINTER(C.c_double)))
+        y = np.zeros(bc.ytotal)
+        cnt = np.zeros((bc.nsys, _lib.BS_COUNT), dtype=np.int64)
+        hp.SUNAMD_BatchedBDFGetResults(B, y.ctypes.data_as(C.POINTER(C.c_double)),
+                                       cnt.ctypes.data_as(C.POINTER(C.c_long)))
+        hp.SUNAMD_BatchedBDFFree(B)
+        buckets = {}
+        for k in range(nb):
+            sel = sb == k
+            nst = float(cnt[sel, 0].sum())
+            ticks, entries = raw[k, :16], raw[k, 16:32]
+            mark = raw[k, 32] / max(raw[k, 33], 1.0)   # ticks of one phase change
+            # each entry's own clock read lands in the phase entered: take it off
+            corrected = np.maximum(ticks - entries * mark, 0.0)
+            per_step = corrected / max(nst, 1.0)
+            buckets["m%d" % bm[k]] = {
+                "systems": int(sel.sum()), "steps": int(nst), "kernel_ms": round(float(kms[k]), 2),
+                "newton_iters_per_step": round(float(cnt[sel, 4].sum()) / max(nst, 1.0), 4),
+                "setups_per_step": round(float(cnt[sel, 2].sum()) / max(nst, 1.0), 4),
+                "ticks_per_phase_change": round(float(mark), 1),
+                "phase_changes_per_step": round(float(entries.sum()) / max(nst, 1.0), 2),
+                "ticks_per_step": {PHASE_NAMES[p]: round(float(per_step[p]), 1)
+                                   for p in range(16)},
+                "raw_ticks_per_step": round(float(ticks.sum()) / max(nst, 1.0), 1),
+                "total_ticks_per_step": round(float(per_step.sum()), 1)}
+            print("[phases] %s m=%2d: %7.0f ticks/step (raw %.0f, %.1f changes x %.0f)  %s" % (
+                tag, bm[k], per_step.sum(), ticks.sum() / max(nst, 1.0),
+                entries.sum() / max(nst, 1.0), mark,
+                " ".join("%s %.0f" % (PHASE_NAMES[p], per_step[p]) for p in range(16)
+                         if per_step[p] >= 0.02 * per_step.sum())), file=sys.stderr, flush=True)
+        out[tag] = {"wall_ms": round(wall.value * 1e3, 2), "buckets": buckets}
+    core.SUNAMD_BatchFree(b)
+    return {"metric": "phase clock per step, batched BDF (profiling aid)", "value": None,
+            "unit": "shader-clock ticks per step", "n_gpus": 1, "data": "synthetic",
+            "config": {"workload": "uneven batch: %d independent mass-action systems, m in %s, "
+                                   "two wavefronts per SIMD, learned start off"
+                                   % (args.systems, list(BATCH_M))},
+            "clock": "s_memtime (shader clock) read at each phase change; ticks of one lane "
+                     "group per step of its system, summed over a bucket's groups / its steps, "
+                     "less the measured cost of one phase change per entry",
+            "results": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["batched", "coupled", "imex", "kinsol", "kinsol-newton"],
+    ap.add_argument("--workload", choices=["batched", "coupled", "imex", "kinsol", "kinsol-newton",
+                             "phases", "vector"],
                     default="batched")
     ap.add_argument("--kin-cells", type=int, default=1 << 21, help="kinsol: cells (3 unknowns)")
     ap.add_argument("--kin-cpu-cells", type=int, default=1 << 21, help="kinsol: CPU-oracle cells")
@@ -1018,14 +1224,17 @@ def main():
     ap.add_argument("--waves", choices=["auto", "1", "2", "3", "4"], default="auto",
                     help="wavefronts per SIMD for every bucket (auto: calibrated per bucket; "
                          "fixed values also keep the learned start off)")
-    ap.add_argument("--train-systems", type=int, default=4096)
-    ap.add_argument("--train-pairs", type=int, default=1500000)
-    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-systems", type=int, default=32768,
+                    help="systems of the independent batch whose plain runs are logged")
+    ap.add_argument("--train-pairs", type=int, default=10000000,
+                    help="training pairs kept (one per logged step)")
+    ap.add_argument("--train-steps", type=int, default=0,
+                    help="Adam steps of 1024 pairs (0: one pass over the pairs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(systems are independent; DESIGN.md describes how the batch would shard)",
+        print("bench.py measures one GPU: this build has no multi-GPU path (the systems are "
+              "independent, so a batch would shard by system ranges with no exchange)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -1043,6 +1252,10 @@ def main():
         line = bench_kinsol(args)
     elif args.workload == "kinsol-newton":
         line = bench_kinsol_newton(args)
+    elif args.workload == "phases":
+        line = bench_phases(args)
+    elif args.workload == "vector":
+        line = bench_vector(args)
     else:
         line = bench_coupled(args)
     print(json.dumps(line))

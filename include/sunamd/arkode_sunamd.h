@@ -56,6 +56,7 @@ extern "C" {
 #define ARK_TOO_CLOSE        -27
 #define ARK_VECTOROP_ERR     -28
 #define ARK_NLS_OP_ERR       -32
+#define ARK_USER_PREDICT_FAIL -39
 #define ARK_CONTROLLER_ERR   -47
 
 #define ARKLS_SUCCESS   0
@@ -77,6 +78,13 @@ typedef int (*ARKLsJacTimesSetupFn)(sunrealtype t, N_Vector y, N_Vector fy,
 typedef int (*ARKLsJacTimesVecFn)(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y,
                                   N_Vector fy, void* user_data, N_Vector tmp);
 
+/* arkode.h:188: a user stage predictor, called for every implicit stage
+   after the built-in (trivial) prediction with the stage time and the
+   prediction zpred, which it may overwrite; < 0: unrecoverable failure
+   (ARK_USER_PREDICT_FAIL), > 0: the step attempt is retried as after a
+   convergence failure */
+typedef int (*ARKStagePredictFn)(sunrealtype t, N_Vector zpred, void* user_data);
+
 /* arkode_arkstep.h: ARKStepCreate(fe, fi, t0, y0, sunctx) */
 void* ARKStepCreate(ARKRhsFn fe, ARKRhsFn fi, sunrealtype t0, N_Vector y0,
                     SUNContext sunctx);
@@ -86,6 +94,8 @@ int ARKodeSetUserData(void* arkode_mem, void* user_data);
 int ARKodeSetOrder(void* arkode_mem, int maxord);
 int ARKodeSetStopTime(void* arkode_mem, sunrealtype tstop);
 int ARKodeSetMaxNumSteps(void* arkode_mem, long int mxsteps);
+/* arkode.h:295 (ARKodeSetStagePredictFn); NULL removes it */
+int ARKodeSetStagePredictFn(void* arkode_mem, ARKStagePredictFn PredictStage);
 int ARKodeSetFixedStep(void* arkode_mem, sunrealtype hfixed);
 int ARKodeSetInitStep(void* arkode_mem, sunrealtype hin);
 int ARKodeSetInterpolantType(void* arkode_mem, int itype);

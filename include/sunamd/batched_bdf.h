@@ -17,9 +17,11 @@
  *   - x^y in the step-size ratios is the correctly rounded sunamd_pow used by
  *     the host integrator.
  * In FP64 mode every system's step count, iteration counts and final state
- * equal the host integrator's on the same system, bit for bit.  In FP32 mode
- * the Newton matrix is factored in single precision (residuals, updates and
- * norms stay FP64).
+ * equal the host integrator's on the same system, bit for bit.  FP32 mode is
+ * the fast product form: the Newton matrix is factored and solved in single
+ * precision, WRMS norms are summed as a butterfly over the group's lanes and
+ * x^y in the step ratios uses the hardware log2 / exp2 (relative error
+ * ~1e-7); residuals, Newton updates and the norms themselves stay FP64.
  * ---------------------------------------------------------------------------*/
 #ifndef SUNAMD_BATCHED_BDF_H
 #define SUNAMD_BATCHED_BDF_H
@@ -90,9 +92,12 @@ int SUNAMD_BatchedBDFSetOccupancy(SUNAMD_BatchedBDF* B, const int* waves);
    Newton solve always starts from a zero correction, cvode.c:3095-3109):
    weights = SUNAMD_MLPDeviceWeights of a trained network (NULL: off),
    bit k of bucket_mask enables it for bucket k.  At the first attempt of
-   every step after the first, the network's correction and the zero
-   correction are both evaluated and the one with the smaller weighted
-   residual norm starts the Newton iteration (two RHS evaluations). */
+   every step after the first, the network proposes a correction (evaluated
+   once per wavefront for all of its lane groups, bf16 weights as
+   16 x 16 x 32 MFMA operands, FP32 accumulation); the network's correction
+   and the zero correction are both evaluated and the one with the smaller
+   weighted residual norm starts the Newton iteration (two RHS
+   evaluations). */
 int SUNAMD_BatchedBDFSetLearnedPredictor(SUNAMD_BatchedBDF* B, const void* weights,
                                          int bucket_mask);
 /* Training-pair log: device arrays x[capacity x 144], t[capacity x 64],
@@ -105,11 +110,45 @@ int SUNAMD_BatchedBDFSetTrainingLog(SUNAMD_BatchedBDF* B, float* x, float* t, fl
 long int SUNAMD_BatchedBDFGetTrainingLogCount(SUNAMD_BatchedBDF* B);
 /* steps of the last run where the learned guess was tried / kept */
 int SUNAMD_BatchedBDFGetLearnedStats(SUNAMD_BatchedBDF* B, long int* tried, long int* kept);
+/* the same per bucket (arrays of the bucket count, bucket order); returns
+   the number of buckets */
+int SUNAMD_BatchedBDFGetLearnedStatsByBucket(SUNAMD_BatchedBDF* B, long int* tried,
+                                             long int* kept);
 /* kernel time of every bucket in the last run, in bucket order (HIP events
    on the bucket's stream around its launch; the buckets run concurrently);
    returns the number of buckets */
 int SUNAMD_BatchedBDFGetKernelTimes(SUNAMD_BatchedBDF* B, double* ms);
 void SUNAMD_BatchedBDFFree(SUNAMD_BatchedBDF* B);
+
+/* Phase clock (profiling aid, no counterpart in the reference).  A library
+   built with SUNAMD_BDF_PHASES (libsundials_amd_phase.so, the batched
+   integrator only, two wavefronts per SIMD) reads the shader clock at every
+   phase change of every lane group.  For the last run, summed over bucket
+   k's groups, out[k * SUNAMD_BDF_PHASE_VALUES + ...] receives: [p] ticks
+   spent in phase p, [SUNAMD_BDF_NPHASES + p] entries into phase p, then the
+   ticks and the number of a calibration run of back-to-back phase changes
+   (each group's, at its start) -- the cost of one change, which lands in the
+   phase entered.  Returns the number of buckets, or -1 from a library built
+   without the clock. */
+#define SUNAMD_BDF_NPHASES  16
+#define SUNAMD_BDF_PHASE_VALUES (2 * SUNAMD_BDF_NPHASES + 2)
+#define SUNAMD_PH_ACQUIRE   0  /* taking a system: state set-up, f(t0, y0), initial step */
+#define SUNAMD_PH_GUARD     1  /* error weights, step-limit and accuracy guards */
+#define SUNAMD_PH_PREDICT   2  /* order / step change, predictor, BDF coefficients */
+#define SUNAMD_PH_NEWTON    3  /* the Newton iteration's own arithmetic and control */
+#define SUNAMD_PH_RHS       4  /* right-hand side evaluations */
+#define SUNAMD_PH_JAC       5  /* Jacobian and Newton matrix I - gamma J */
+#define SUNAMD_PH_GETRF     6  /* LU factorisation */
+#define SUNAMD_PH_GETRS     7  /* triangular solves */
+#define SUNAMD_PH_NORM      8  /* WRMS norms */
+#define SUNAMD_PH_ERRTEST   9  /* failure handling and the local error test */
+#define SUNAMD_PH_COMPLETE  10 /* history update after an accepted step */
+#define SUNAMD_PH_PREPARE   11 /* next step size and order */
+#define SUNAMD_PH_POW       12 /* x^y in the step-ratio formulas */
+#define SUNAMD_PH_LEARN     13 /* learned start: features and network */
+#define SUNAMD_PH_OUTPUT    14 /* tout test, interpolation, results */
+#define SUNAMD_PH_IDLE      15 /* group without a system while its wavefront runs */
+int SUNAMD_BatchedBDFGetPhaseTicks(SUNAMD_BatchedBDF* B, unsigned long long* out);
 
 /* Allocation-free integration of one size bucket.  No counterpart in the
    reference: it is for callers that cannot allocate device memory (code

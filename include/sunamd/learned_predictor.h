@@ -71,6 +71,12 @@ int SUNAMD_MLPForward(SUNAMD_MLP* M, const float* x, int B, float* y);
 int SUNAMD_MLPLossGrad(SUNAMD_MLP* M, const float* x, const float* t, const float* mask, int B,
                        float* loss, float* grad);
 
+/* the batched integrator's own in-kernel evaluation of the network (one
+   wavefront per 16 rows, bf16 weights as 16 x 16 x 32 MFMA operands, FP32
+   accumulation) on host arrays x[rows x 144] -> y[rows x 64]; weights =
+   SUNAMD_MLPDeviceWeights.  For checking it against SUNAMD_MLPForward and a
+   CPU reference. */
+int SUNAMD_MLPForwardWave(const void* weights, const float* x, int rows, float* y);
 /* training on device-resident data: x[n x 144], t[n x 64], mask[n x 64]
    (device pointers); runs `steps` Adam steps on consecutive batches of
    `batch` rows (wrapping around), learning rate lr; *loss receives the mean

@@ -41,7 +41,11 @@ int SUNAMD_ProfilerGet(int family, double* total_ms, long int* count);
    timed calls (after one warm-up), measured with HIP events on the vector
    stream: out_ms[0] LinearSum z = a x + b y, [1] WrmsNorm(x, w),
    [2] DotProdMulti(nvec_multi, x, Y), [3] LinearSum y = x + y (in place),
-   [4] DotProd(x, y). */
+   [4] DotProd(x, y); then the copy z = x that bounds them: [5] in the vector
+   kernels' chunked form (256 threads, 8 pairs per lane, 32 workgroups per
+   CU), [6] the same with 1024 threads and 8 workgroups per CU (the fastest
+   copy of a sweep of shapes), [7] hipMemcpyAsync device to device.  out_ms
+   holds 8 values; n must be even. */
 int SUNAMD_BenchVectorOps(sunindextype n, int nvec_multi, int reps, double* out_ms);
 
 #ifdef __cplusplus

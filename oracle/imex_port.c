@@ -203,6 +203,10 @@ int oracle_adr_solve(const SUNAMD_Adr* p, const SUNAMD_AdrOptions* o, double* st
 typedef struct
 {
   double le, li;
+  int pmode;   /* user stage predictor: 0 none, 1 keeps zpred, 2 asks for a
+                  retry on its first call, 3 fails, 4 replaces zpred by the
+                  exact solution at the stage time */
+  long pcalls; /* stage-predictor calls */
 } Dahl;
 
 static int d_fe(sunrealtype t, N_Vector y, N_Vector f, void* ud)
@@ -230,13 +234,24 @@ static int d_jv(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y, N_Vector fy,
   return 0;
 }
 
-/* flags: 1 linearly implicit (not time dependent), 2 autonomous, 4 no
-   interpolant (else Hermite); order selects the pair (3 or 4) */
-int oracle_ark_dahlquist(double lam_e, double lam_i, double h, double tf, double rtol,
-                         double atol, const char* controller, int flags, int order,
-                         double* stats, double* yout)
+static int d_predict(sunrealtype t, N_Vector zpred, void* ud)
 {
-  Dahl D         = {lam_e, lam_i};
+  Dahl* D = (Dahl*)ud;
+  D->pcalls++;
+  switch (D->pmode)
+  {
+  case 2: return (D->pcalls == 1) ? 1 : 0;
+  case 3: return -1;
+  case 4: N_VGetArrayPointer(zpred)[0] = exp((D->le + D->li) * t); return 0;
+  default: return 0;
+  }
+}
+
+static int dahl_run(double lam_e, double lam_i, double h, double tf, double rtol, double atol,
+                    const char* controller, int flags, int order, int pmode, double* stats,
+                    double* yout, long* pcalls)
+{
+  Dahl D         = {lam_e, lam_i, pmode, 0};
   SUNContext ctx = NULL;
   if (SUNContext_Create(SUN_COMM_NULL, &ctx)) { return -1; }
   N_Vector y                = N_VNew_Serial(1, ctx);
@@ -253,6 +268,7 @@ int oracle_ark_dahlquist(double lam_e, double lam_i, double h, double tf, double
   if (ok && (flags & 1)) { ok = ARKodeSetLinear(ark, 0) == 0; }
   if (ok && (flags & 2)) { ok = ARKodeSetAutonomous(ark, SUNTRUE) == 0; }
   if (ok && (flags & 4)) { ok = ARKodeSetInterpolantType(ark, ARK_INTERP_NONE) == 0; }
+  if (ok && pmode > 0) { ok = ARKodeSetStagePredictFn(ark, d_predict) == 0; }
   if (ok)
   {
     sunrealtype tret = 0.0;
@@ -275,11 +291,33 @@ int oracle_ark_dahlquist(double lam_e, double lam_i, double h, double tf, double
     if (rc == ARK_TSTOP_RETURN) { rc = 0; }
     yout[0] = N_VGetArrayPointer(y)[0];
   }
+  if (pcalls) { *pcalls = D.pcalls; }
   if (ark) { ARKodeFree(&ark); }
   if (L) { SUNLinSolFree(L); }
   N_VDestroy(y);
   SUNContext_Free(&ctx);
   return rc;
+}
+
+/* flags: 1 linearly implicit (not time dependent), 2 autonomous, 4 no
+   interpolant (else Hermite); order selects the pair (3 or 4) */
+int oracle_ark_dahlquist(double lam_e, double lam_i, double h, double tf, double rtol,
+                         double atol, const char* controller, int flags, int order,
+                         double* stats, double* yout)
+{
+  return dahl_run(lam_e, lam_i, h, tf, rtol, atol, controller, flags, order, 0, stats, yout,
+                  NULL);
+}
+
+/* the adaptive Dahlquist run (default controller, Hermite interpolant) with
+   a user stage predictor of the given mode (Dahl.pmode); *pcalls receives
+   its number of calls */
+int oracle_ark_dahlquist_predict(double lam_e, double lam_i, double tf, double rtol,
+                                 double atol, int order, int pmode, double* stats, double* yout,
+                                 long* pcalls)
+{
+  return dahl_run(lam_e, lam_i, 0.0, tf, rtol, atol, NULL, 0, order, pmode, stats, yout,
+                  pcalls);
 }
 
 /* ---------------------------------------------------------------------------

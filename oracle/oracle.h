@@ -55,6 +55,11 @@ SUNMatrix SUNBlockDenseMatrix_Oracle(sunindextype nblocks, sunindextype m,
 sunrealtype* SUNBlockDenseMatrix_Oracle_Data(SUNMatrix A);
 SUNLinearSolver SUNLinSol_BlockDense_Oracle(N_Vector y, SUNMatrix A, SUNContext sunctx);
 
+/* host N_Vector microbenchmark (bench_port.c): ms per call of [0] LinearSum
+   z = 0.3x + 0.7y, [1] WrmsNorm, [2] DotProdMulti(nvec), [3] y = x + y,
+   [4] DotProd; nthreads 1 = serial vector, > 1 = OpenMP loops */
+int oracle_bench_vector_ops(long n, int nvec, int reps, int nthreads, double* out_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -289,6 +289,48 @@ def core():
     return lib
 
 
+def _batched_sigs(lib):
+    """ctypes signatures of the batched per-system BDF entry points."""
+    P = C.POINTER
+    _sig(lib, "SUNAMD_BatchedBDFCreate", c_void_p, [P(Batch), c_real, c_real, C.c_int])
+    _sig(lib, "SUNAMD_BatchedBDFSetMaxNumSteps", C.c_int, [c_void_p, C.c_long])
+    _sig(lib, "SUNAMD_BatchedBDFRun", C.c_int, [c_void_p, c_real, P(c_real)])
+    _sig(lib, "SUNAMD_BatchedBDFGetResults", C.c_int, [c_void_p, P(c_real), P(C.c_long)])
+    _sig(lib, "SUNAMD_BatchedBDFGetKernelTimes", C.c_int, [c_void_p, P(c_real)])
+    _sig(lib, "SUNAMD_BatchedBDFSetLearnedPredictor", C.c_int, [c_void_p, c_void_p, C.c_int])
+    _sig(lib, "SUNAMD_BatchedBDFSetOccupancy", C.c_int, [c_void_p, P(C.c_int)])
+    _sig(lib, "SUNAMD_BatchedBDFSetTrainingLog", C.c_int,
+         [c_void_p, c_void_p, c_void_p, c_void_p, C.c_long, C.c_int])
+    _sig(lib, "SUNAMD_BatchedBDFGetTrainingLogCount", C.c_long, [c_void_p])
+    _sig(lib, "SUNAMD_BatchedBDFGetLearnedStats", C.c_int, [c_void_p, P(C.c_long), P(C.c_long)])
+    _sig(lib, "SUNAMD_BatchedBDFGetLearnedStatsByBucket", C.c_int,
+         [c_void_p, P(C.c_long), P(C.c_long)])
+    _sig(lib, "SUNAMD_BatchedBDFSolveBucketNoAlloc", C.c_int,
+         [C.c_int, C.c_int, C.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_real, c_real,
+          c_real, C.c_long, C.c_int, c_void_p, c_void_p, c_void_p])
+    _sig(lib, "SUNAMD_BatchedBDFGetPhaseTicks", C.c_int, [c_void_p, P(C.c_ulonglong)])
+    _sig(lib, "SUNAMD_BatchedBDFFree", None, [c_void_p])
+
+
+_hip_phase = None
+
+
+def hip_phase():
+    """Load libsundials_amd_phase.so: the batched integrator alone, built with
+    the phase clock (a profiling aid; same entry points as hip())."""
+    global _hip_phase
+    if _hip_phase is not None:
+        return _hip_phase
+    core()
+    path = os.path.join(LIBDIR, "libsundials_amd_phase.so")
+    if not os.path.exists(path):
+        raise RuntimeError("libsundials_amd_phase.so not built: run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    _batched_sigs(lib)
+    _hip_phase = lib
+    return lib
+
+
 def hip():
     """Load libsundials_amd.so (the HIP product path); raises if missing."""
     global _hip
@@ -346,31 +388,18 @@ def hip():
     _sig(lib, "SUNAMD_KinSolve_Mi355", C.c_int, [P(KinOptions), P(c_real), P(c_real)])
     _sig(lib, "SUNAMD_KinNewtonSolve_Mi355", C.c_int,
          [P(KinNewtonOptions), P(c_real), P(c_real)])
-    _sig(lib, "SUNAMD_BatchedBDFCreate", c_void_p, [P(Batch), c_real, c_real, C.c_int])
-    _sig(lib, "SUNAMD_BatchedBDFSetMaxNumSteps", C.c_int, [c_void_p, C.c_long])
-    _sig(lib, "SUNAMD_BatchedBDFRun", C.c_int, [c_void_p, c_real, P(c_real)])
-    _sig(lib, "SUNAMD_BatchedBDFGetResults", C.c_int, [c_void_p, P(c_real), P(C.c_long)])
-    _sig(lib, "SUNAMD_BatchedBDFGetKernelTimes", C.c_int, [c_void_p, P(c_real)])
-    _sig(lib, "SUNAMD_BatchedBDFSetLearnedPredictor", C.c_int, [c_void_p, c_void_p, C.c_int])
-    _sig(lib, "SUNAMD_BatchedBDFSetOccupancy", C.c_int, [c_void_p, P(C.c_int)])
-    _sig(lib, "SUNAMD_BatchedBDFSetTrainingLog", C.c_int,
-         [c_void_p, c_void_p, c_void_p, c_void_p, C.c_long, C.c_int])
-    _sig(lib, "SUNAMD_BatchedBDFGetTrainingLogCount", C.c_long, [c_void_p])
-    _sig(lib, "SUNAMD_BatchedBDFGetLearnedStats", C.c_int, [c_void_p, P(C.c_long), P(C.c_long)])
-    _sig(lib, "SUNAMD_BatchedBDFSolveBucketNoAlloc", C.c_int,
-         [C.c_int, C.c_int, C.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_real, c_real,
-          c_real, C.c_long, C.c_int, c_void_p, c_void_p, c_void_p])
+    _batched_sigs(lib)
     PFl = P(C.c_float)
     _sig(lib, "SUNAMD_MLPCreate", c_void_p, [C.c_uint64, C.c_int])
     _sig(lib, "SUNAMD_MLPFree", None, [c_void_p])
     _sig(lib, "SUNAMD_MLPGetParams", C.c_int, [c_void_p, PFl])
     _sig(lib, "SUNAMD_MLPSetParams", C.c_int, [c_void_p, PFl])
     _sig(lib, "SUNAMD_MLPDeviceWeights", c_void_p, [c_void_p])
+    _sig(lib, "SUNAMD_MLPForwardWave", C.c_int, [c_void_p, PFl, C.c_int, PFl])
     _sig(lib, "SUNAMD_MLPForward", C.c_int, [c_void_p, PFl, C.c_int, PFl])
     _sig(lib, "SUNAMD_MLPLossGrad", C.c_int, [c_void_p, PFl, PFl, PFl, C.c_int, PFl, PFl])
     _sig(lib, "SUNAMD_MLPTrain", C.c_int,
          [c_void_p, c_void_p, c_void_p, c_void_p, C.c_long, C.c_int, C.c_int, C.c_float, PFl])
-    _sig(lib, "SUNAMD_BatchedBDFFree", None, [c_void_p])
     _hip = lib
     return lib
 
@@ -414,6 +443,11 @@ def load_oracle():
     _sig(lib, "oracle_ark_kpr", C.c_int, [C.c_int, P(c_real), P(c_real)])
     _sig(lib, "oracle_batch_solve", C.c_int,
          [P(Batch), c_real, c_real, c_real, C.c_int, P(c_real), P(C.c_long), P(c_real)])
+    _sig(lib, "oracle_ark_dahlquist_predict", C.c_int,
+         [c_real, c_real, c_real, c_real, c_real, C.c_int, C.c_int, P(c_real), P(c_real),
+          P(C.c_long)])
+    _sig(lib, "oracle_bench_vector_ops", C.c_int,
+         [C.c_long, C.c_int, C.c_int, C.c_int, P(c_real)])
     return lib
 
 

@@ -63,6 +63,7 @@ struct SUNAMD_BatchedBDF_
   long log_cap;
   int log_stride;
   unsigned long long* dcount; // [0] log rows, [1 + 2k], [2 + 2k]: bucket k tried / kept
+  unsigned long long* dphase; // phase clock build: SUNAMD_BDF_PHASE_VALUES per bucket
 };
 
 static int hfail(hipError_t e, const char* what)
@@ -98,10 +99,11 @@ static bool launch_bucket(const SUNAMD_BatchedBDF* B, int k, double tout, int gr
     A.log_stride = B->log_stride > 0 ? B->log_stride : 1;
   }
   A.lstats = B->dcount + 1 + 2 * k;
+  A.phase  = B->dphase ? B->dphase + (size_t)k * SUNAMD_BDF_PHASE_VALUES : nullptr;
   {
     const int gpw = 64 / pow2ceil(d.m);
-    if (sunamd_bdf::bucket_lds_bytes(d.m, d.R, gpw, A.mlp_w != nullptr || A.log_x != nullptr) >
-        60000)
+    if (sunamd_bdf::bucket_lds_bytes(d.m, d.R, gpw, A.mlp_w != nullptr, A.log_x != nullptr,
+                                     (int)sizeof(T)) > 60000)
     {
       fprintf(stderr, "[sundials_amd batched_bdf] bucket m=%d R=%d needs too much LDS\n", d.m,
               d.R);
@@ -153,6 +155,17 @@ extern "C" SUNAMD_BatchedBDF* SUNAMD_BatchedBDFCreate(const SUNAMD_Batch* batch,
     free(B);
     return NULL;
   }
+#ifdef SUNAMD_BDF_PHASES
+  {
+    const size_t pb = sizeof(unsigned long long) * SUNAMD_BATCH_MAXB * SUNAMD_BDF_PHASE_VALUES;
+    if (hipMalloc((void**)&B->dphase, pb) != hipSuccess)
+    {
+      (void)hipFree(B->dcount);
+      free(B);
+      return NULL;
+    }
+  }
+#endif
   int dev = 0;
   hipDeviceProp_t prop;
   B->num_cus = 256;
@@ -293,6 +306,12 @@ extern "C" int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* w
   {
     hipError_t e = hipMemset(B->dcount + 1, 0, sizeof(unsigned long long) * 2 * SUNAMD_BATCH_MAXB);
     if (e != hipSuccess) { return hfail(e, "counter reset"); }
+    if (B->dphase)
+    {
+      e = hipMemset(B->dphase, 0,
+                    sizeof(unsigned long long) * SUNAMD_BATCH_MAXB * SUNAMD_BDF_PHASE_VALUES);
+      if (e != hipSuccess) { return hfail(e, "phase reset"); }
+    }
   }
   for (int n = 0; n < B->nb; n++)
   {
@@ -437,11 +456,37 @@ extern "C" int SUNAMD_BatchedBDFGetLearnedStats(SUNAMD_BatchedBDF* B, long int* 
   return 0;
 }
 
+extern "C" int SUNAMD_BatchedBDFGetLearnedStatsByBucket(SUNAMD_BatchedBDF* B, long int* tried,
+                                                        long int* kept)
+{
+  unsigned long long c[1 + 2 * SUNAMD_BATCH_MAXB];
+  if (!B || !tried || !kept ||
+      hipMemcpy(c, B->dcount, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess)
+  {
+    return -1;
+  }
+  for (int k = 0; k < B->nb; k++)
+  {
+    tried[k] = (long)c[1 + 2 * k];
+    kept[k]  = (long)c[2 + 2 * k];
+  }
+  return B->nb;
+}
+
 extern "C" int SUNAMD_BatchedBDFGetKernelTimes(SUNAMD_BatchedBDF* B, double* ms)
 {
   if (!B || !ms) { return -1; }
   for (int k = 0; k < B->nb; k++) { ms[k] = B->b[k].last_ms; }
   return B->nb;
+}
+
+extern "C" int SUNAMD_BatchedBDFGetPhaseTicks(SUNAMD_BatchedBDF* B, unsigned long long* out)
+{
+  if (!B || !out || !B->dphase) { return -1; }
+  hipError_t e = hipMemcpy(out, B->dphase,
+                           sizeof(unsigned long long) * (size_t)B->nb * SUNAMD_BDF_PHASE_VALUES,
+                           hipMemcpyDeviceToHost);
+  return e == hipSuccess ? B->nb : hfail(e, "phase copy");
 }
 
 extern "C" void SUNAMD_BatchedBDFFree(SUNAMD_BatchedBDF* B)
@@ -465,6 +510,7 @@ extern "C" void SUNAMD_BatchedBDFFree(SUNAMD_BatchedBDF* B)
     if (d.stream) { (void)hipStreamDestroy(d.stream); }
   }
   if (B->dcount) { (void)hipFree(B->dcount); }
+  if (B->dphase) { (void)hipFree(B->dphase); }
   free(B->sys_bucket);
   free(B->sys_cell);
   free(B->sys_yoff);
@@ -598,7 +644,8 @@ extern "C" int SUNAMD_BatchedBDFSolveBucketNoAlloc(int m, int R, int64_t ncells,
     return -1;
   }
   const int gpw = 64 / pow2ceil(m);
-  if (sunamd_bdf::bucket_lds_bytes(m, R, gpw, false) > 60000)
+  if (sunamd_bdf::bucket_lds_bytes(m, R, gpw, false, false,
+                                   lu_precision == SUNAMD_LU_FP32 ? 4 : 8) > 60000)
   {
     fprintf(stderr, "[sundials_amd batched_bdf] no-alloc solve: m=%d R=%d needs too much LDS\n",
             m, R);

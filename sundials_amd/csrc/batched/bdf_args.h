@@ -46,27 +46,39 @@ struct BucketArgs
   // non-null: a flag the reaction table's validation set on the device; when
   // it is nonzero every system is returned unintegrated with flag -22
   const int* bad;
+  // instrumented build only: clock ticks per phase, summed over the groups
+  unsigned long long* phase;
 };
 
-// LDS floats per lane group for the MLP inputs and hidden layers
-constexpr int MLP_LDS_FLOATS = 144 + 128 + 128;
+// LDS floats per lane group staging one training-log row (the inputs)
+constexpr int MLP_LOG_FLOATS = 144;
+// the wavefront's network area: 16 rows of bf16 inputs (stride MLP_XS: 144
+// used, zero to 160 for the 32-deep MFMA slices) and two hidden layers
+// (stride MLP_HS); the strides are padded so that the 16 rows' 16-byte
+// operand reads fall in different LDS banks
+constexpr int MLP_XS         = 168;
+constexpr int MLP_HS         = 136;
+constexpr int MLP_WAVE_BYTES = 2 * (16 * MLP_XS + 2 * 16 * MLP_HS);
 
-// doubles of a lane group's LDS rows area: the m Jacobian rows (stride m+1)
-// while the Newton matrix is built, the R reaction rates during an RHS
-__host__ __device__ constexpr long group_rows_stride(int m, int R)
+// doubles of a lane group's LDS rows area: the m Jacobian rows (stride m+1,
+// entries of row_bytes: 8 in the FP64 form, 4 in the FP32 form) while the
+// Newton matrix is built, the R reaction rates (doubles) during an RHS
+__host__ __device__ constexpr long group_rows_stride(int m, int R, int row_bytes)
 {
-  return ((long)m * (m + 1) > (long)R) ? (long)m * (m + 1) : (long)R;
+  return ((long)m * (m + 1) * row_bytes + 7) / 8 > (long)R ? ((long)m * (m + 1) * row_bytes + 7) / 8
+                                                          : (long)R;
 }
 
 // dynamic LDS of one workgroup (GPW lane groups): staged rate constants,
-// network tables, the groups' rows areas, and the MLP area when the learned
-// start or the log is on
-inline size_t bucket_lds_bytes(int m, int R, int gpw, bool mlp)
+// network tables, the groups' rows areas, the wavefront's network area with
+// the learned start and the log-row staging with the training log
+inline size_t bucket_lds_bytes(int m, int R, int gpw, bool learned, bool log, int row_bytes)
 {
   size_t b = sizeof(double) * (size_t)gpw * 2 * R;
   b += sizeof(int) * (size_t)(8 * R + ((m + 2) & ~1));
-  b += sizeof(double) * (size_t)gpw * group_rows_stride(m, R);
-  if (mlp) { b += sizeof(float) * (size_t)gpw * MLP_LDS_FLOATS; }
+  b += sizeof(double) * (size_t)gpw * group_rows_stride(m, R, row_bytes);
+  if (learned) { b += MLP_WAVE_BYTES; }
+  if (log) { b += sizeof(float) * (size_t)gpw * MLP_LOG_FLOATS; }
   return b;
 }
 

@@ -120,6 +120,12 @@ struct GroupScalars
   double tq[6];
   double l[QMAX + 1];
   double cvals[QMAX + 2];
+#ifdef SUNAMD_BDF_PHASES
+  unsigned long long ph_t;                  // time stamp of the last phase change
+  unsigned long long ph_acc[SUNAMD_BDF_NPHASES]; // clock ticks spent in each phase
+  unsigned int ph_cnt[SUNAMD_BDF_NPHASES];  // entries into each phase
+  int ph_cur;                               // current phase
+#endif
 };
 
 template <int M, typename T>
@@ -155,26 +161,155 @@ struct Grp
   T* prow;     // G entries of LDS
   int* inv;    // G ints of LDS
   GroupScalars* sc;
-  float* mlp; // MLP_LDS_FLOATS of LDS (xs[144] h1[128] h2[128]) or nullptr
-  bool learn; // kernel built with the learned-start / training-log paths
+  float* mlp;         // MLP_LOG_FLOATS of LDS staging a training-log row, or nullptr
+  unsigned short* wx; // the wavefront's network area (MLP_WAVE_BYTES of LDS) or nullptr
+  int row;            // this group's row in the wavefront's network inputs
+  bool learn;         // kernel built with the learned-start / training-log paths
   const BucketArgs* a;
   // the bucket's network, staged in LDS once per workgroup
   const int* idx;  // 4R reaction indices
   const int* soff; // M+1 term-list offsets
   const int* sent; // 4R term codes
   double* rates;   // this group's 2R rate constants (k_f then k_b), staged per system
-  double* jrow;    // this lane's Jacobian-row accumulator (M doubles, LDS)
+  T* jrow;         // this lane's Jacobian-row accumulator (M entries, LDS)
   double* wbuf;    // the group's reaction rates during an RHS (aliases the rows)
   long cell;
   const double* kf; // this system's k_f, k_b (the LDS slot above)
   const double* kb;
 };
 
+// ---- phase clock (instrumented build only) ----------------------------------
+// Built with SUNAMD_BDF_PHASES, every group accumulates the shader clock
+// (s_memtime) between phase changes into its LDS slot, counts the entries into
+// each phase, and adds the totals to BucketArgs::phase when its wavefront
+// leaves the kernel.  A phase change is a scalar clock read and, on the
+// group's first lane, a few LDS updates; the arithmetic of the integrator is
+// untouched.  Each group also times a run of back-to-back phase changes at
+// its start, so the cost of one change can be taken off every phase it
+// entered.  Phase numbers and the output layout: batched_bdf.h.
+#ifdef SUNAMD_BDF_PHASES
+template <int M, typename T>
+__device__ __forceinline__ int ph_set(const Grp<M, T>& g, int p)
+{
+  const unsigned long long t = __builtin_amdgcn_s_memtime();
+  const int prev             = g.sc->ph_cur;
+  if (g.i == 0)
+  {
+    g.sc->ph_acc[prev] += t - g.sc->ph_t;
+    g.sc->ph_cnt[p] += 1u;
+    g.sc->ph_t   = t;
+    g.sc->ph_cur = p;
+  }
+  return prev;
+}
+#define BDF_PH_PUSH(g, p) const int ph_prev_ = ph_set(g, p)
+#define BDF_PH_POP(g) ph_set(g, ph_prev_)
+#define BDF_PH_SET(g, p) ph_set(g, p)
+#else
+#define BDF_PH_PUSH(g, p) ((void)0)
+#define BDF_PH_POP(g) ((void)0)
+#define BDF_PH_SET(g, p) ((void)0)
+#endif
+
+// The product form (single-precision factors) is free of the FP64 form's
+// bit-for-bit contract with the host integrator, so it takes the fast forms:
+// fused multiply-adds and no zero tests in the factorisation, a pivot search
+// that is one unsigned maximum per column, (for groups of 32 or 64 lanes) a
+// single-precision solve, WRMS norms summed as a butterfly
+// instead of in species order, and x^y in the step-ratio formulas through the
+// hardware log2 / exp2 (relative error ~1e-7, immaterial in a ratio that is
+// clamped to [0.1, 10]).  Residuals, updates and the norms themselves stay
+// FP64.  The double-precision form keeps the dense GETRF/GETRS operation
+// order, species-order norms and the correctly rounded x^y.
+template <typename T>
+inline constexpr bool kFastFactor = (sizeof(T) == 4);
+
+// DPP row permutations used by the group reductions
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x)
+{
+  return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_v(double x)
+{
+  const long long u = __double_as_longlong(x);
+  const int lo      = dpp_i<CTRL>((int)(u & 0xffffffffLL));
+  const int hi      = dpp_i<CTRL>((int)(u >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_v(float x)
+{
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(x)));
+}
+
+// lane l's value of x, for a lane index that is the same in the whole wavefront
+__device__ __forceinline__ double readlane_d(double x, int l)
+{
+  const long long v = __double_as_longlong(x);
+  const int lo      = __builtin_amdgcn_readlane((int)(v & 0xffffffffLL), l);
+  const int hi      = __builtin_amdgcn_readlane((int)(v >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+__device__ __forceinline__ float readlane_d(float x, int l)
+{
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// Sum over a G-lane group as a butterfly: quad swaps, then half-row and row
+// mirrors (each lane adds the lane its mirror pairs it with, which holds the
+// other half's partial sum), then the four 16-lane rows by lane reads.  Every
+// addition has the same two operands in both lanes of a pair, so every lane of
+// the group ends with the same bits.
+template <int G>
+__device__ __forceinline__ double g_tree_sum(double v)
+{
+  if constexpr (G >= 2) { v = v + dpp_v<0xB1>(v); }  // quad_perm [1,0,3,2]
+  if constexpr (G >= 4) { v = v + dpp_v<0x4E>(v); }  // quad_perm [2,3,0,1]
+  if constexpr (G >= 8) { v = v + dpp_v<0x141>(v); } // row_half_mirror
+  if constexpr (G >= 16) { v = v + dpp_v<0x140>(v); } // row_mirror
+  if constexpr (G >= 32)
+  {
+    const double r0 = readlane_d(v, 0), r1 = readlane_d(v, 16);
+    const double r2 = readlane_d(v, 32), r3 = readlane_d(v, 48);
+    const double lo = r0 + r1, hi = r2 + r3;
+    if constexpr (G == 64) { v = lo + hi; }
+    else { v = (__lane_id() >= 32) ? hi : lo; }
+  }
+  return v;
+}
+
+// x^y for the step-ratio formulas of the product form: x = 2^e * mant exactly
+// (frexp), log2 of the mantissa and exp2 of the fraction on the hardware's
+// single-precision units, the powers of two exact (ldexp).  x >= 0, 0 < y <= 1.
+__device__ __forceinline__ double fast_pow(double x, double y)
+{
+  if (!(x > 0.0)) { return 0.0; }
+  if (!(x < __builtin_huge_val())) { return x; }
+  int e           = 0;
+  const double mt = frexp(x, &e);
+  const double t  = y * ((double)e + (double)__builtin_amdgcn_logf((float)mt));
+  const double fl = floor(t);
+  const double r  = (double)__builtin_amdgcn_exp2f((float)(t - fl));
+  return ldexp(r, (int)fl);
+}
+
 // ---- group reductions (species order = the serial vector's order) --------
 template <int M, typename T>
 __device__ __forceinline__ double g_wrms(const Grp<M, T>& g, double x, double w)
 {
+  BDF_PH_PUSH(g, SUNAMD_PH_NORM);
   double p = x * w;
+  if constexpr (kFastFactor<T>)
+  {
+    const double s = g_tree_sum<pow2ceil(M)>(g.valid ? p * p : 0.0);
+    BDF_PH_POP(g);
+    return sqrt0(s / (double)M);
+  }
   gsync();
   if (g.valid) { g.red[g.i] = p * p; }
   gsync();
@@ -182,6 +317,7 @@ __device__ __forceinline__ double g_wrms(const Grp<M, T>& g, double x, double w)
 #pragma unroll
   for (int k = 0; k < M; k++) { s += g.red[k]; }
   gsync();
+  BDF_PH_POP(g);
   return sqrt0(s / (double)M);
 }
 
@@ -210,6 +346,7 @@ __device__ __forceinline__ double g_maxabs(const Grp<M, T>& g, double x)
 template <int M, typename T>
 __device__ __forceinline__ double g_rhs(const Grp<M, T>& g, double yi)
 {
+  BDF_PH_PUSH(g, SUNAMD_PH_RHS);
   constexpr int G = pow2ceil(M);
   gsync();
   if (g.valid) { g.red[g.i] = yi; }
@@ -235,6 +372,7 @@ __device__ __forceinline__ double g_rhs(const Grp<M, T>& g, double yi)
     }
   }
   gsync();
+  BDF_PH_POP(g);
   return f;
 }
 
@@ -248,13 +386,14 @@ __device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, d
   if (g.valid) { g.red[g.i] = yj; }
   gsync();
   // the row is accumulated in LDS so that a term's column is a plain address
-  // (the same additions in the same order as a register row)
-  double* row    = g.jrow;
+  // (the same additions in the same order as a register row); the FP32 form
+  // accumulates in single precision (half the LDS, more wavefronts per CU)
+  T* row         = g.jrow;
   const int* idx = g.idx;
   if (g.valid)
   {
 #pragma unroll
-    for (int j = 0; j < M; j++) { row[j] = 0.0; }
+    for (int j = 0; j < M; j++) { row[j] = (T)0; }
     const int e1 = g.soff[g.i + 1];
     for (int e = g.soff[g.i]; e < e1; e++)
     {
@@ -267,8 +406,8 @@ __device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, d
 #pragma unroll
       for (int q = 0; q < 4; q++)
       {
-        const double v = dw[q];
-        row[ix[q]]     = (t < 2) ? (row[ix[q]] - v) : (row[ix[q]] + v);
+        const T v  = (T)dw[q];
+        row[ix[q]] = (t < 2) ? (row[ix[q]] - v) : (row[ix[q]] + v);
       }
     }
   }
@@ -278,24 +417,10 @@ __device__ __forceinline__ void g_newton_matrix(const Grp<M, T>& g, double yj, d
 #pragma unroll
   for (int j = 0; j < M; j++)
   {
-    double v = g.valid ? row[j] * c : 0.0;
+    double v = g.valid ? (double)row[j] * c : 0.0;
     if (j == g.i) { v = v + 1.0; }
     a[j] = (T)v;
   }
-}
-
-// lane l's value of x, for a lane index that is the same in the whole wavefront
-__device__ __forceinline__ double readlane_d(double x, int l)
-{
-  const long long v = __double_as_longlong(x);
-  const int lo      = __builtin_amdgcn_readlane((int)(v & 0xffffffffLL), l);
-  const int hi      = __builtin_amdgcn_readlane((int)(v >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
-}
-
-__device__ __forceinline__ float readlane_d(float x, int l)
-{
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
 // b of lane base+k of this lane's group, for groups of 64 or 32 lanes
@@ -326,26 +451,6 @@ __device__ __forceinline__ float g_lane_value(float b, int k)
 // ---- pivot search: largest |a_ik| over the group, ties to the smallest row
 // (the dense GETRF's first maximum), as a reduction on (value, row) pairs:
 // DPP exchanges inside rows of 16 lanes, lane reads across rows --------------
-template <int CTRL>
-__device__ __forceinline__ int dpp_i(int x)
-{
-  return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
-}
-
-template <int CTRL>
-__device__ __forceinline__ double dpp_v(double x)
-{
-  const long long u = __double_as_longlong(x);
-  const int lo      = dpp_i<CTRL>((int)(u & 0xffffffffLL));
-  const int hi      = dpp_i<CTRL>((int)(u >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
-}
-
-template <int CTRL>
-__device__ __forceinline__ float dpp_v(float x)
-{
-  return __int_as_float(dpp_i<CTRL>(__float_as_int(x)));
-}
 
 // (ov, op) replaces (v, p) when ov is larger, or equal with a smaller row;
 // evaluated without short-circuit so it compiles to compares and selects
@@ -390,11 +495,28 @@ __device__ __forceinline__ void g_argmax(V& v, int& p)
   }
 }
 
-// Single-precision factors take the fast forms: fused multiply-adds, no zero
-// tests, and (for groups of 32 or 64 lanes) a single-precision solve; the
-// double-precision factorisation keeps the dense GETRF/GETRS operation order.
-template <typename T>
-inline constexpr bool kFastFactor = (sizeof(T) == 4);
+
+// unsigned maximum over a G-lane group (every lane ends with it)
+template <int G>
+__device__ __forceinline__ unsigned g_max_u32(unsigned v)
+{
+  auto mx = [](unsigned a, unsigned b) { return a > b ? a : b; };
+  if constexpr (G >= 2) { v = mx(v, (unsigned)dpp_i<0xB1>((int)v)); }
+  if constexpr (G >= 4) { v = mx(v, (unsigned)dpp_i<0x4E>((int)v)); }
+  if constexpr (G >= 8) { v = mx(v, (unsigned)dpp_i<0x141>((int)v)); }
+  if constexpr (G >= 16) { v = mx(v, (unsigned)dpp_i<0x140>((int)v)); }
+  if constexpr (G >= 32)
+  {
+    const unsigned r0 = (unsigned)__builtin_amdgcn_readlane((int)v, 0);
+    const unsigned r1 = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
+    const unsigned r2 = (unsigned)__builtin_amdgcn_readlane((int)v, 32);
+    const unsigned r3 = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
+    const unsigned lo = mx(r0, r1), hi = mx(r2, r3);
+    if constexpr (G == 64) { v = mx(lo, hi); }
+    else { v = (__lane_id() >= 32) ? hi : lo; }
+  }
+  return v;
+}
 
 // ---- LU of the group's matrix (rows in lanes, implicit pivoting) ---------
 template <int M, typename T>
@@ -406,9 +528,25 @@ __device__ __forceinline__ int g_getrf(const Grp<M, T>& g, Sys<M, T>& S)
 #pragma unroll
   for (int k = 0; k < M; k++)
   {
-    T v    = (g.valid && pos >= k) ? (T)fabs((double)S.lu[k]) : (T)-1;
-    int bp = g.valid ? pos : INT_MAX;
-    g_argmax<G>(v, bp);
+    T v;
+    int bp;
+    if constexpr (kFastFactor<T>)
+    {
+      // one unsigned maximum on |a_ik|'s bits with the low six replaced by
+      // 63 - row: pivots within 2^-17 of the largest tie to the smallest row
+      const unsigned key = (g.valid && pos >= k)
+                             ? ((__float_as_uint(fabsf((float)S.lu[k])) & ~63u) | (unsigned)(63 - pos))
+                             : 0u;
+      const unsigned mk = g_max_u32<G>(key);
+      v  = (T)__uint_as_float(mk & ~63u);
+      bp = 63 - (int)(mk & 63u);
+    }
+    else
+    {
+      v  = (g.valid && pos >= k) ? (T)fabs((double)S.lu[k]) : (T)-1;
+      bp = g.valid ? pos : INT_MAX;
+      g_argmax<G>(v, bp);
+    }
     if (v == (T)0 && fail == 0) { fail = k + 1; }
     if (pos == k) { pos = bp; }
     else if (pos == bp) { pos = k; }
@@ -846,13 +984,16 @@ __device__ __forceinline__ int g_lsetup(const Grp<M, T>& g, Sys<M, T>& S, bool j
     S.ysav = S.y;
   }
   // the Newton-matrix row is built in the factor registers themselves
+  BDF_PH_PUSH(g, SUNAMD_PH_JAC);
   g_newton_matrix(g, S.ysav, S.gamma, S.lu);
   if (jcur)
   {
     S.nje++;
     S.nstlj = S.nst;
   }
+  BDF_PH_SET(g, SUNAMD_PH_GETRF);
   int fail = g_getrf(g, S);
+  BDF_PH_POP(g);
   S.nsetups++;
   S.nls_jcur = jcur;
   S.gamrat   = 1.0;
@@ -865,7 +1006,9 @@ __device__ __forceinline__ int g_lsetup(const Grp<M, T>& g, Sys<M, T>& S, bool j
 template <int M, typename T>
 __device__ __forceinline__ void g_lsolve(const Grp<M, T>& g, Sys<M, T>& S)
 {
+  BDF_PH_PUSH(g, SUNAMD_PH_GETRS);
   double x = g_getrs(g, S, S.delta);
+  BDF_PH_POP(g);
   S.delta  = x;
   if (S.gamrat != 1.0) { S.delta = (2.0 / (1.0 + S.gamrat)) * S.delta; }
 }
@@ -991,42 +1134,110 @@ __device__ __forceinline__ void g_features(const Grp<M, T>& g, const Sys<M, T>& 
   gsync();
 }
 
-// forward pass of the group's MLP on xs; returns this lane's output
-// (species g.i), in error-weight units
-template <int M, typename T>
-__device__ __forceinline__ float g_mlp(const Grp<M, T>& g, float* xs)
+// bf16 bits of f, rounded to nearest even (the training kernels' rounding)
+__device__ __forceinline__ unsigned short bf16_bits(float f)
 {
-  constexpr int G             = pow2ceil(M);
-  const unsigned short* W1    = g.a->mlp_w;
-  const unsigned short* W2    = W1 + 128 * 144;
-  const unsigned short* W3    = W2 + 128 * 128;
-  const float* b1             = g.a->mlp_b;
-  const float* b2             = b1 + 128;
-  const float* b3             = b2 + 128;
-  float* h1                   = xs + 144;
-  float* h2                   = h1 + 128;
-  for (int j = g.i; j < 144; j += G) { xs[j] = bf16_round(xs[j]); }
-  gsync();
-  for (int o = g.i; o < 128; o += G)
-  {
-    float acc = b1[o];
-    for (int k = 0; k < 129; k++) { acc += bf16_at(W1, (long)o * 144 + k) * xs[k]; }
-    h1[o] = bf16_round(acc / (1.0f + expf(-acc)));
-  }
-  gsync();
-  for (int o = g.i; o < 128; o += G)
-  {
-    float acc = b2[o];
-    for (int k = 0; k < 128; k++) { acc += bf16_at(W2, (long)o * 128 + k) * h1[k]; }
-    h2[o] = bf16_round(acc / (1.0f + expf(-acc)));
-  }
-  gsync();
-  float out = 0.0f;
+  unsigned int u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) { return (unsigned short)(u >> 16); }
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// the group's network inputs, rounded to bf16, into its row of the
+// wavefront's input block (columns 144.. stay zero)
+template <int M, typename T>
+__device__ __forceinline__ void g_features_bf16(const Grp<M, T>& g, const Sys<M, T>& S,
+                                                unsigned short* xrow)
+{
+  constexpr int G = pow2ceil(M);
   if (g.valid)
   {
-    out = b3[g.i];
-    for (int k = 0; k < 128; k++) { out += bf16_at(W3, (long)g.i * 128 + k) * h2[k]; }
+    const double a   = fabs(S.zn[0]);
+    xrow[g.i]        = bf16_bits((float)(log10(a > 1e-30 ? a : 1e-30) * 0.1));
+    xrow[65 + g.i]   = bf16_bits(S.ratef);
   }
+  for (int j = g.i; j < 144; j += G)
+  {
+    const bool feat = (j < M) || (j >= 65 && j < 65 + M);
+    if (j == 64) { xrow[64] = bf16_bits((float)(log10(S.gamma) * 0.1)); }
+    else if (!feat) { xrow[j] = 0; }
+  }
+}
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// out = act(in W^T + b) for the wavefront's 16 rows.  in: [16 x K] bf16 in
+// LDS (row stride ls); W: [N x kw] bf16 row-major in global memory (k >= kw
+// reads as zero); b: [N] FP32.  ACT: SiLU, rounded to bf16 into out_h (row
+// stride lh); otherwise the FP32 sums into out_f ([16 x 64]).  One
+// 16 x 16 x 32 bf16 MFMA per (16 output columns, 32-deep slice): lane l
+// supplies row l%16 of `in` and row n0 + l%16 of W for the 8 consecutive k
+// of its quarter l/16 (one 16-byte load each) and receives rows
+// 4(l/16) .. 4(l/16)+3 of column n0 + l%16; FP32 accumulation.
+template <int K, int N, bool ACT>
+__device__ __forceinline__ void w_layer(const unsigned short* in, int ls, const unsigned short* W,
+                                        int kw, const float* b, unsigned short* out_h, int lh,
+                                        float* out_f)
+{
+  const int l = __lane_id();
+  const int r = l & 15, q = l >> 4;
+#pragma unroll
+  for (int n0 = 0; n0 < N; n0 += 16)
+  {
+    f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 32)
+    {
+      const int k      = k0 + 8 * q;
+      const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(in + r * ls + k);
+      bf16x8_t wv      = {};
+      if (k < kw) { wv = *reinterpret_cast<const bf16x8_t*>(W + (long)(n0 + r) * kw + k); }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wv, acc, 0, 0, 0);
+    }
+    const float bias = b[n0 + r];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+      const float z = acc[i] + bias;
+      if constexpr (ACT) { out_h[(4 * q + i) * lh + n0 + r] = bf16_bits(z / (1.0f + expf(-z))); }
+      else { out_f[(4 * q + i) * 64 + n0 + r] = z; }
+    }
+  }
+}
+
+// The network on the 16 input rows of X (a wavefront's MLP_WAVE_BYTES network
+// area in LDS; called with all 64 lanes active): 144 inputs (zero-padded to
+// 160) -> 128 -> 128 -> 64, bf16 weights read as MFMA operands straight from
+// the (L2-resident) weight block W (W1 W2 W3, then the FP32 biases), FP32
+// accumulation, SiLU on the hidden layers.  Returns the [16 x 64] FP32
+// outputs, in the area of the first hidden layer.
+__device__ __forceinline__ const float* w_forward16(const unsigned short* W, const float* b,
+                                                    unsigned short* X)
+{
+  const unsigned short* W2 = W + 128 * 144;
+  const unsigned short* W3 = W2 + 128 * 128;
+  unsigned short* H1       = X + 16 * MLP_XS;
+  unsigned short* H2       = H1 + 16 * MLP_HS;
+  float* Y                 = reinterpret_cast<float*>(H1); // layer 1's outputs are dead by then
+  gsync();
+  w_layer<160, 128, true>(X, MLP_XS, W, 144, b, H1, MLP_HS, nullptr);
+  gsync();
+  w_layer<128, 128, true>(H1, MLP_HS, W2, 128, b + 128, H2, MLP_HS, nullptr);
+  gsync();
+  w_layer<128, 64, false>(H2, MLP_HS, W3, 128, b + 256, nullptr, 0, Y);
+  gsync();
+  return Y;
+}
+
+// The wavefront's network evaluation on the rows its groups wrote (a group
+// that did not write its row ignores its output); this lane's output
+// (species g.i of its group), in error-weight units.
+template <int M, typename T>
+__device__ __forceinline__ float w_network(const Grp<M, T>& g, bool want)
+{
+  const float* Y  = w_forward16(g.a->mlp_w, g.a->mlp_b, g.wx);
+  const float out = (want && g.valid) ? Y[g.row * 64 + g.i] : 0.0f;
   gsync();
   return out;
 }
@@ -1054,24 +1265,27 @@ __device__ __forceinline__ void g_log_pair(const Grp<M, T>& g, const Sys<M, T>& 
   }
 }
 
+// have_guess: the wavefront's network proposed guess (error-weight units) for
+// this step's first attempt
 template <int M, typename T>
-__device__ __forceinline__ int g_nls(const Grp<M, T>& g, Sys<M, T>& S, int nflag)
+__device__ __forceinline__ int g_nls(const Grp<M, T>& g, Sys<M, T>& S, int nflag, bool have_guess,
+                                     double guess)
 {
   S.convfail = ((nflag == FIRST_CALL) || (nflag == PREV_ERR_FAIL)) ? CV_NO_FAILURES : CV_FAIL_OTHER;
   bool callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (S.nst == 0) ||
                    (S.nst >= S.nstlp + MSBP) || (fabs(S.gamrat - 1.0) > DGMAX_LSETUP);
   S.acor        = 0.0;
   bool have_res = false, learned = false;
-  if (g.learn && g.a->mlp_w != nullptr && nflag == FIRST_CALL && S.nst > 0)
+  if (have_guess && nflag == FIRST_CALL && S.nst > 0)
   {
     // learned start, kept only if its residual is smaller than the zero
     // start's (the plain predictor is the fallback)
-    g_features(g, S, g.mlp);
-    const double guess = (double)g_mlp(g, g.mlp) / S.ewt;
+    BDF_PH_PUSH(g, SUNAMD_PH_LEARN);
+    const double gs = guess / S.ewt;
     g_residual(g, S);
     const double r0  = g_wrms(g, S.delta, S.ewt);
     const double d0 = S.delta, y0 = S.y, f0 = S.ftemp;
-    S.acor           = g.valid ? guess : 0.0;
+    S.acor           = g.valid ? gs : 0.0;
     g_residual(g, S);
     const double r1 = g_wrms(g, S.delta, S.ewt);
     S.ltried++;
@@ -1088,6 +1302,7 @@ __device__ __forceinline__ int g_nls(const Grp<M, T>& g, Sys<M, T>& S, int nflag
       S.ftemp = f0;
     }
     have_res = true;
+    BDF_PH_POP(g);
   }
   int flag = g_newton(g, S, callSetup, have_res, learned);
   if (flag != CV_SUCCESS) { return flag; }
@@ -1120,6 +1335,16 @@ __device__ __forceinline__ int g_handle_nflag(const Grp<M, T>& g, Sys<M, T>& S, 
   return PREDICT_AGAIN;
 }
 
+// x^y of the eta formulas (SUNRpowerR)
+template <int M, typename T>
+__device__ __forceinline__ double g_pow(const Grp<M, T>& g, double x, double y)
+{
+  BDF_PH_PUSH(g, SUNAMD_PH_POW);
+  const double r = kFastFactor<T> ? fast_pow(x, y) : sunamd_pow(x, y);
+  BDF_PH_POP(g);
+  return r;
+}
+
 template <int M, typename T>
 __device__ __forceinline__ int g_error_test(const Grp<M, T>& g, Sys<M, T>& S, int& nflag, double saved_t,
                             int& nef, double& dsm_out)
@@ -1135,7 +1360,7 @@ __device__ __forceinline__ int g_error_test(const Grp<M, T>& g, Sys<M, T>& S, in
   S.etamax = 1.0;
   if (nef <= MXNEF1)
   {
-    S.eta = 1.0 / (sunamd_pow(BIAS2 * dsm, 1.0 / S.L) + ADDON);
+    S.eta = 1.0 / (g_pow(g, BIAS2 * dsm, 1.0 / S.L) + ADDON);
     double e2 = (S.eta > 0.0) ? S.eta : 0.0;
     S.eta     = (ETA_MIN_EF > e2) ? ETA_MIN_EF : e2;
     if (nef >= SMALL_NEF) { S.eta = (S.eta < ETA_MAX_EF) ? S.eta : ETA_MAX_EF; }
@@ -1231,7 +1456,7 @@ __device__ __forceinline__ void g_prepare_next(const Grp<M, T>& g, Sys<M, T>& S,
     S.eta    = 1.0;
     return;
   }
-  S.etaq = 1.0 / (sunamd_pow(BIAS2 * dsm, 1.0 / S.L) + ADDON);
+  S.etaq = 1.0 / (g_pow(g, BIAS2 * dsm, 1.0 / S.L) + ADDON);
   if (S.qwait != 0)
   {
     S.eta    = S.etaq;
@@ -1245,7 +1470,7 @@ __device__ __forceinline__ void g_prepare_next(const Grp<M, T>& g, Sys<M, T>& S,
   if (S.q > 1)
   {
     double ddn = g_wrms(g, zget(S, S.q), S.ewt) * g.sc->tq[1];
-    S.etaqm1   = 1.0 / (sunamd_pow(BIAS1 * ddn, 1.0 / S.q) + ADDON);
+    S.etaqm1   = 1.0 / (g_pow(g, BIAS1 * ddn, 1.0 / S.q) + ADDON);
   }
   // cvComputeEtaqp1
   S.etaqp1 = 0.0;
@@ -1257,7 +1482,7 @@ __device__ __forceinline__ void g_prepare_next(const Grp<M, T>& g, Sys<M, T>& S,
     double cquot = (g.sc->tq[5] / S.saved_tq5) * pw;
     S.tempv      = lsum(-cquot, S.zn[QMAX], 1.0, S.acor);
     double dup   = g_wrms(g, S.tempv, S.ewt) * g.sc->tq[3];
-    S.etaqp1     = 1.0 / (sunamd_pow(BIAS3 * dup, 1.0 / (S.L + 1)) + ADDON);
+    S.etaqp1     = 1.0 / (g_pow(g, BIAS3 * dup, 1.0 / (S.L + 1)) + ADDON);
   }
   // cvChooseEta
   double em   = (S.etaq > S.etaqp1) ? S.etaq : S.etaqp1;
@@ -1284,44 +1509,6 @@ __device__ __forceinline__ void g_prepare_next(const Grp<M, T>& g, Sys<M, T>& S,
     S.zn[QMAX] = S.acor;
   }
   g_set_eta(S);
-}
-
-template <int M, typename T>
-__device__ __forceinline__ int g_step(const Grp<M, T>& g, Sys<M, T>& S)
-{
-  int ncf = 0, nef = 0;
-  double dsm = 0.0;
-  if ((S.nst > 0) && (S.hprime != S.h))
-  {
-    if (S.qprime != S.q)
-    {
-      g_adjust_order(g, S, S.qprime - S.q);
-      S.q     = S.qprime;
-      S.L     = S.q + 1;
-      S.qwait = S.L;
-    }
-    g_rescale(g, S);
-  }
-  const double saved_t = S.tn;
-  int nflag            = FIRST_CALL;
-  for (;;)
-  {
-    g_predict(S);
-    g_set(g, S, S.nst);
-    nflag     = g_nls(g, S, nflag);
-    int kflag = g_handle_nflag(g, S, nflag, saved_t, ncf);
-    if (kflag == PREDICT_AGAIN) { continue; }
-    if (kflag != DO_ERROR_TEST) { return kflag; }
-    int eflag = g_error_test(g, S, nflag, saved_t, nef, dsm);
-    if (eflag == TRY_AGAIN) { continue; }
-    if (eflag != CV_SUCCESS) { return eflag; }
-    break;
-  }
-  g_complete_step(g, S);
-  g_prepare_next(g, S, dsm);
-  S.etamax = (S.nst <= SMALL_NST) ? ETA_MAX_ES : ETA_MAX_GS;
-  S.acor   = g.sc->tq[2] * S.acor;
-  return CV_SUCCESS;
 }
 
 // initial step (cvHin with cvUpperBoundH0 and cvYddNorm)
@@ -1502,12 +1689,22 @@ __device__ __forceinline__ bool g_acquire(Grp<M, T>& g, Sys<M, T>& S, double tou
   }
 }
 
-// One pass of CVode's main loop (CV_NORMAL): error weights, the work and
-// accuracy guards, one step, and the tout check with interpolation.
+// what the two halves of one pass of CVode's main loop share
+struct StepCtx
+{
+  double saved_t, dsm;
+  int ncf, nef, nflag;
+};
+
+// One pass of CVode's main loop (CV_NORMAL), first half: error weights, the
+// work and accuracy guards, then cvStep up to its first prediction -- the
+// order / step-size change, the Nordsieck predictor and the BDF coefficients.
 // Returns true when the system is finished (yout and flag set).
 template <int M, typename T>
-__device__ __forceinline__ bool g_advance(const Grp<M, T>& g, Sys<M, T>& S, double tout, double& yout, int& flag)
+__device__ __forceinline__ bool g_advance_head(const Grp<M, T>& g, Sys<M, T>& S, double& yout,
+                                               int& flag, StepCtx& c)
 {
+  BDF_PH_SET(g, SUNAMD_PH_GUARD);
   if (S.nst > 0) { S.ewt = g_ewt(g, S.zn[0]); }
   // one CVode call: the steps taken in it are nst; mxstep < 0 disables the check
   if ((g.a->mxstep > 0) && (S.nst >= g.a->mxstep))
@@ -1523,13 +1720,71 @@ __device__ __forceinline__ bool g_advance(const Grp<M, T>& g, Sys<M, T>& S, doub
     yout = S.zn[0];
     return true;
   }
-  int kflag = g_step(g, S);
+  c.ncf = c.nef = 0;
+  c.dsm         = 0.0;
+  BDF_PH_SET(g, SUNAMD_PH_PREDICT);
+  if ((S.nst > 0) && (S.hprime != S.h))
+  {
+    if (S.qprime != S.q)
+    {
+      g_adjust_order(g, S, S.qprime - S.q);
+      S.q     = S.qprime;
+      S.L     = S.q + 1;
+      S.qwait = S.L;
+    }
+    g_rescale(g, S);
+  }
+  c.saved_t = S.tn;
+  c.nflag   = FIRST_CALL;
+  g_predict(S);
+  g_set(g, S, S.nst);
+  return false;
+}
+
+// Second half: the nonlinear solve (from the learned guess when one is
+// given), convergence-failure handling, the error test and the retries
+// (each a new prediction), the completed step with the next step size and
+// order, and the tout check with interpolation.
+template <int M, typename T>
+__device__ __forceinline__ bool g_advance_tail(const Grp<M, T>& g, Sys<M, T>& S, double tout,
+                                               double& yout, int& flag, StepCtx& c,
+                                               bool have_guess, double guess)
+{
+  int kflag  = CV_SUCCESS;
+  bool again = false;
+  for (;;)
+  {
+    if (again)
+    {
+      BDF_PH_SET(g, SUNAMD_PH_PREDICT);
+      g_predict(S);
+      g_set(g, S, S.nst);
+    }
+    again = true;
+    BDF_PH_SET(g, SUNAMD_PH_NEWTON);
+    c.nflag = g_nls(g, S, c.nflag, have_guess, guess);
+    BDF_PH_SET(g, SUNAMD_PH_ERRTEST);
+    kflag = g_handle_nflag(g, S, c.nflag, c.saved_t, c.ncf);
+    if (kflag == PREDICT_AGAIN) { continue; }
+    if (kflag != DO_ERROR_TEST) { break; }
+    kflag = g_error_test(g, S, c.nflag, c.saved_t, c.nef, c.dsm);
+    if (kflag == TRY_AGAIN) { continue; }
+    break;
+  }
   if (kflag != CV_SUCCESS)
   {
+    BDF_PH_SET(g, SUNAMD_PH_OUTPUT);
     flag = kflag;
     yout = S.zn[0];
     return true;
   }
+  BDF_PH_SET(g, SUNAMD_PH_COMPLETE);
+  g_complete_step(g, S);
+  BDF_PH_SET(g, SUNAMD_PH_PREPARE);
+  g_prepare_next(g, S, c.dsm);
+  S.etamax = (S.nst <= SMALL_NST) ? ETA_MAX_ES : ETA_MAX_GS;
+  S.acor   = g.sc->tq[2] * S.acor;
+  BDF_PH_SET(g, SUNAMD_PH_OUTPUT);
   if ((S.tn - tout) * S.h >= 0.0)
   {
     // CVodeGetDky(tout, 0) through the unfused LinearCombination:
@@ -1538,10 +1793,10 @@ __device__ __forceinline__ bool g_advance(const Grp<M, T>& g, Sys<M, T>& S, doub
     double z = 0.0;
     for (int j = S.q; j >= 0; j--)
     {
-      double c = 1.0;
-      for (int i = 0; i < j; i++) { c *= s; }
+      double cj = 1.0;
+      for (int i = 0; i < j; i++) { cj *= s; }
       double zj = zget(S, j);
-      z         = (j == S.q) ? c * zj : lsum(c, zj, 1.0, z);
+      z         = (j == S.q) ? cj * zj : lsum(cj, zj, 1.0, z);
     }
     yout = z;
     flag = CV_SUCCESS;
@@ -1569,16 +1824,20 @@ __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
   __shared__ GroupScalars s_sc[GPW];
   // dynamic LDS: per-group rate constants (GPW x 2R doubles), the network
   // tables (4R + 4R + M+1 ints), per-group Jacobian rows / reaction rates
-  // (GPW x rows_stride doubles), then GPW x MLP_LDS_FLOATS when learned or
-  // logging; 16-byte alignment for the int4 reads of the reaction table
+  // (GPW x rows_stride doubles), then the wavefront's network area
+  // (MLP_WAVE_BYTES) with the learned start, then GPW x MLP_LOG_FLOATS with
+  // the training log; 16-byte alignment for the int4 reads of the reaction
+  // table and the network's 16-byte operand reads
   extern __shared__ __align__(16) double s_dyn[];
-  const long JS   = group_rows_stride(M, A.R);
+  const long JS   = group_rows_stride(M, A.R, (int)sizeof(T));
   double* s_rates = s_dyn;
   int* s_idx      = (int*)(s_dyn + (long)GPW * 2 * A.R);
   int* s_sent     = s_idx + 4 * A.R;
   int* s_soff     = s_sent + 4 * A.R;
   double* s_jrow  = (double*)(s_soff + ((M + 2) & ~1));
-  float* s_mlp    = (float*)(s_jrow + (long)GPW * JS);
+  const bool learned = LEARN && A.mlp_w != nullptr;
+  unsigned short* s_wx = (unsigned short*)(s_jrow + (long)GPW * JS);
+  float* s_log         = (float*)((char*)s_wx + (learned ? MLP_WAVE_BYTES : 0));
 
   const int lane = threadIdx.x;
   for (int t = lane; t < 4 * A.R; t += 64)
@@ -1597,15 +1856,21 @@ __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
   g.prow  = s_prow + g.base;
   g.inv   = s_inv + g.base;
   g.sc    = &s_sc[grp];
-  g.mlp   = (LEARN && (A.mlp_w != nullptr || A.log_x != nullptr)) ? s_mlp + grp * MLP_LDS_FLOATS
-                                                                   : nullptr;
+  g.mlp   = (LEARN && A.log_x != nullptr) ? s_log + grp * MLP_LOG_FLOATS : nullptr;
+  g.wx    = learned ? s_wx : nullptr;
+  g.row   = grp;
+  if (learned)
+  {
+    for (int t = lane; t < MLP_WAVE_BYTES / 2; t += 64) { s_wx[t] = 0; }
+    __syncthreads();
+  }
   g.learn = LEARN;
   g.a     = &A;
   g.idx   = s_idx;
   g.soff  = s_soff;
   g.sent  = s_sent;
   g.rates = s_rates + (long)grp * 2 * A.R;
-  g.jrow  = s_jrow + (long)grp * JS + (long)(g.valid ? g.i : 0) * (M + 1);
+  g.jrow  = (T*)(s_jrow + (long)grp * JS) + (long)(g.valid ? g.i : 0) * (M + 1);
   g.wbuf  = s_jrow + (long)grp * JS;
   g.cell  = 0;
   // the rate constants are always read from the group's LDS slot (a pointer
@@ -1613,18 +1878,100 @@ __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
   g.kf    = g.rates;
   g.kb    = g.rates + A.R;
   Sys<M, T> S;
+#ifdef SUNAMD_BDF_PHASES
+  constexpr int CAL_MARKS = 32;
+  if (g.i == 0)
+  {
+    for (int p = 0; p < SUNAMD_BDF_NPHASES; p++)
+    {
+      g.sc->ph_acc[p] = 0;
+      g.sc->ph_cnt[p] = 0;
+    }
+    g.sc->ph_cur = SUNAMD_PH_IDLE;
+    g.sc->ph_t   = __builtin_amdgcn_s_memtime();
+  }
+  gsync();
+  // calibration: CAL_MARKS back-to-back changes, timed into the idle slot
+  for (int c = 0; c < CAL_MARKS; c++) { ph_set(g, SUNAMD_PH_IDLE); }
+  unsigned long long cal_ticks = 0;
+  if (g.i == 0)
+  {
+    cal_ticks           = g.sc->ph_acc[SUNAMD_PH_IDLE];
+    g.sc->ph_acc[SUNAMD_PH_IDLE] = 0;
+    g.sc->ph_cnt[SUNAMD_PH_IDLE] = 0;
+  }
+  ph_set(g, SUNAMD_PH_ACQUIRE);
+#endif
 
   bool busy = g_acquire(g, S, A.tout);
-  while (busy)
+  if (!busy) { BDF_PH_SET(g, SUNAMD_PH_IDLE); }
+  if (!learned)
   {
-    double yout = 0.0;
-    int flag    = 0;
-    if (g_advance(g, S, A.tout, yout, flag))
+    while (busy)
     {
-      g_store(g, S, yout, flag);
-      busy = g_acquire(g, S, A.tout);
+      double yout = 0.0;
+      int flag    = 0;
+      StepCtx c;
+      bool fin = g_advance_head(g, S, yout, flag, c);
+      if (!fin) { fin = g_advance_tail(g, S, A.tout, yout, flag, c, false, 0.0); }
+      if (fin)
+      {
+        g_store(g, S, yout, flag);
+        BDF_PH_SET(g, SUNAMD_PH_ACQUIRE);
+        busy = g_acquire(g, S, A.tout);
+        // a group without a system waits for its wavefront's other groups
+        if (!busy) { BDF_PH_SET(g, SUNAMD_PH_IDLE); }
+      }
     }
   }
+  else
+  {
+    // the network runs once per pass for the whole wavefront, between the
+    // halves of every group's pass: the loop is wavefront-uniform, so all 64
+    // lanes are active at the evaluation (its MFMA operands come from every
+    // lane)
+    while (__any(busy))
+    {
+      double yout = 0.0;
+      int flag    = 0;
+      StepCtx c;
+      bool fin = false, want = false;
+      if (busy)
+      {
+        fin  = g_advance_head(g, S, yout, flag, c);
+        want = !fin && S.nst > 0;
+        if (want) { g_features_bf16(g, S, g.wx + g.row * MLP_XS); }
+      }
+      double guess = 0.0;
+      if (__any(want))
+      {
+        BDF_PH_PUSH(g, SUNAMD_PH_LEARN);
+        guess = (double)w_network(g, want);
+        BDF_PH_POP(g);
+      }
+      if (busy && !fin) { fin = g_advance_tail(g, S, A.tout, yout, flag, c, want, guess); }
+      if (busy && fin)
+      {
+        g_store(g, S, yout, flag);
+        BDF_PH_SET(g, SUNAMD_PH_ACQUIRE);
+        busy = g_acquire(g, S, A.tout);
+        if (!busy) { BDF_PH_SET(g, SUNAMD_PH_IDLE); }
+      }
+    }
+  }
+#ifdef SUNAMD_BDF_PHASES
+  BDF_PH_SET(g, SUNAMD_PH_IDLE);
+  if (g.i == 0 && A.phase != nullptr)
+  {
+    for (int p = 0; p < SUNAMD_BDF_NPHASES; p++)
+    {
+      atomicAdd(A.phase + p, g.sc->ph_acc[p]);
+      atomicAdd(A.phase + SUNAMD_BDF_NPHASES + p, (unsigned long long)g.sc->ph_cnt[p]);
+    }
+    atomicAdd(A.phase + 2 * SUNAMD_BDF_NPHASES, cal_ticks);
+    atomicAdd(A.phase + 2 * SUNAMD_BDF_NPHASES + 1, (unsigned long long)CAL_MARKS);
+  }
+#endif
 }
 
 
@@ -1634,7 +1981,14 @@ void launch(const BucketArgs& A, int grid, hipStream_t stream, int waves)
 {
   constexpr int GPW = 64 / pow2ceil(M);
   const bool learn  = A.mlp_w != nullptr || A.log_x != nullptr;
-  const size_t lds  = bucket_lds_bytes(M, A.R, GPW, learn);
+  const size_t lds  = bucket_lds_bytes(M, A.R, GPW, A.mlp_w != nullptr, A.log_x != nullptr,
+                                       (int)sizeof(T));
+#ifdef SUNAMD_BDF_PHASES
+  // instrumented build: the two-wavefront register budget only
+  (void)waves;
+  if (learn) { k_batched_bdf<M, T, 2, true><<<grid, 64, lds, stream>>>(A); }
+  else { k_batched_bdf<M, T, 2, false><<<grid, 64, lds, stream>>>(A); }
+#else
   if (waves >= 4) { k_batched_bdf<M, T, 4, true><<<grid, 64, lds, stream>>>(A); }
   else if (waves == 3 && !learn) { k_batched_bdf<M, T, 3, false><<<grid, 64, lds, stream>>>(A); }
   else if (waves >= 2)
@@ -1647,6 +2001,7 @@ void launch(const BucketArgs& A, int grid, hipStream_t stream, int waves)
     if (learn) { k_batched_bdf<M, T, 1, true><<<grid, 64, lds, stream>>>(A); }
     else { k_batched_bdf<M, T, 1, false><<<grid, 64, lds, stream>>>(A); }
   }
+#endif
 }
 
 } // namespace sunamd_bdf

@@ -122,6 +122,7 @@ typedef struct ARKodeMemRec
   SUNContext sunctx;
   ARKRhsFn fe, fi;
   void* user_data;
+  ARKStagePredictFn stage_predict; /* user stage predictor or NULL */
   /* tolerances */
   sunrealtype reltol, Sabstol;
   /* vectors */
@@ -579,6 +580,14 @@ int ARKodeSetStopTime(void* arkode_mem, sunrealtype tstop)
   if (ark->nst > 0 && (tstop - ark->tcur) * ark->h < ZERO) { return ARK_ILL_INPUT; }
   ark->tstop    = tstop;
   ark->tstopset = SUNTRUE;
+  return ARK_SUCCESS;
+}
+
+int ARKodeSetStagePredictFn(void* arkode_mem, ARKStagePredictFn PredictStage)
+{
+  ARKMEM_OR_RETURN();
+  /* arkode_arkstep_io.c:1143-1160 */
+  ark->stage_predict = PredictStage;
   return ARK_SUCCESS;
 }
 
@@ -1241,6 +1250,16 @@ static int take_step(ARKodeMem ark, sunrealtype* dsmPtr, int* nflagPtr)
     {
       /* arkStep_Predict, trivial predictor: the step's initial state */
       N_VScale(ONE, ark->yn, ark->zpred);
+      /* the user's stage predictor after the built-in one
+         (arkode_arkstep.c:1943-1955): it may refine zpred; a positive
+         return retries the step attempt (the caller's convergence-failure
+         path with nflag unchanged), a negative one fails the step */
+      if (ark->stage_predict)
+      {
+        retval = ark->stage_predict(ark->tcur, ark->zpred, ark->user_data);
+        if (retval < 0) { return ARK_USER_PREDICT_FAIL; }
+        if (retval > 0) { return TRY_AGAIN; }
+      }
     }
     retval = stage_setup(ark, implicit_stage);
     if (retval != ARK_SUCCESS) { return retval; }

@@ -124,9 +124,54 @@ __global__ void k_fill_uniform(double* __restrict__ v, long n, unsigned long lon
   }
 }
 
+// Copy ceiling: z = x in the vector kernels' chunked form (each lane's U
+// 128-bit non-temporal loads issued before its stores), B threads per
+// workgroup, at most cap workgroups per CU.
+template <int U, int B>
+__global__ void __launch_bounds__(B) k_copy_chunks(const double* __restrict__ x,
+                                                   double* __restrict__ z, long np)
+{
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2* __restrict__ xs = reinterpret_cast<const d2*>(x);
+  d2* __restrict__ zs       = reinterpret_cast<d2*>(z);
+  const long chunk          = (long)U * B;
+  for (long c = (long)blockIdx.x * chunk; c < np; c += (long)gridDim.x * chunk)
+  {
+    const long i0 = c + threadIdx.x;
+    if (c + chunk <= np)
+    {
+      d2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) { v[u] = __builtin_nontemporal_load(xs + i0 + (long)u * B); }
+#pragma unroll
+      for (int u = 0; u < U; u++) { __builtin_nontemporal_store(v[u], zs + i0 + (long)u * B); }
+    }
+    else
+    {
+      for (long i = i0; i < np; i += B) { __builtin_nontemporal_store(__builtin_nontemporal_load(xs + i), zs + i); }
+    }
+  }
+}
+
+template <int U, int B>
+static void launch_copy(const double* x, double* z, long n, int cap_per_cu, hipStream_t s)
+{
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+  {
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  }
+  const long np = n / 2, chunk = (long)U * B;
+  long g        = (np + chunk - 1) / chunk;
+  const long cap = (long)cus * cap_per_cu;
+  if (g > cap) { g = cap; }
+  if (g < 1) { g = 1; }
+  k_copy_chunks<U, B><<<(int)g, B, 0, s>>>(x, z, np);
+}
+
 extern "C" int SUNAMD_BenchVectorOps(sunindextype n, int nvec, int reps, double* out_ms)
 {
-  if (n <= 0 || nvec < 1 || nvec > 32 || reps < 1) { return -1; }
+  if (n <= 0 || (n & 1) || nvec < 1 || nvec > 32 || reps < 1) { return -1; }
   SUNContext ctx = NULL;
   if (SUNContext_Create(SUN_COMM_NULL, &ctx)) { return -1; }
   int rc = -1;
@@ -155,7 +200,9 @@ extern "C" int SUNAMD_BenchVectorOps(sunindextype n, int nvec, int reps, double*
     for (int k = 0; k < nvec; k++) { fill(Y[k], 10 + k, -1.0, 1.0); }
     if (hipStreamSynchronize(s) != hipSuccess) { goto done; }
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { goto done; }
-    for (int op = 0; op < 5; op++)
+    double* xd = N_VGetDeviceArrayPointer(x);
+    double* zd = N_VGetDeviceArrayPointer(z);
+    for (int op = 0; op < 8; op++)
     {
       auto run = [&]() {
         switch (op)
@@ -165,6 +212,9 @@ extern "C" int SUNAMD_BenchVectorOps(sunindextype n, int nvec, int reps, double*
         case 2: (void)N_VDotProdMulti(nvec, x, Y.data(), d); break;
         case 3: N_VLinearSum(1.0, x, 1.0, y, y); break;
         case 4: (void)N_VDotProd(x, y); break;
+        case 5: launch_copy<8, 256>(xd, zd, (long)n, 32, s); break;
+        case 6: launch_copy<8, 1024>(xd, zd, (long)n, 8, s); break;
+        case 7: (void)hipMemcpyAsync(zd, xd, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s); break;
         }
       };
       run(); // warm-up

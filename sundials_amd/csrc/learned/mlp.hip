@@ -1,8 +1,8 @@
 // -----------------------------------------------------------------------------
 // MLP for the learned Newton initial iterate: forward, backward and Adam on
 // the MI355X.  The batched products run on MFMA bf16 tiles
-// (v_mfma_f32_16x16x16_bf16, FP32 accumulation); bias, SiLU, loss and the
-// optimiser are elementwise kernels.  Weights are kept in FP32 (master copy
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation) staged through LDS; bias,
+// SiLU, loss and the optimiser are elementwise kernels.  Weights are kept in FP32 (master copy
 // and Adam moments) and re-rounded to BF16 after every update.
 // -----------------------------------------------------------------------------
 #include <hip/hip_runtime.h>
@@ -34,7 +34,6 @@ static_assert(NP == SUNAMD_MLP_NPARAM, "parameter layout");
 // device weight block used by the integrator: bf16 W1 W2 W3, then FP32 biases
 constexpr long NW_BF16 = (long)HID * IN + (long)HID * HID + (long)OUT * HID;
 
-typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline uint16_t to_bf16(float f)
@@ -63,44 +62,129 @@ __device__ inline float dsilu(float z)
 }
 
 // C[i][j] = sum_k A(i,k) B(k,j), A(i,k) = A[i*lai + k*lak], B(k,j) = B[k*lbk + j*lbj],
-// bf16 operands, FP32 result.  One wavefront per 16x16 tile of C, four per
-// workgroup; M, N, K multiples of 16.  Operand lanes (16x16x16 bf16 MFMA):
-// lane l supplies A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16], j < 4, and holds
-// C[4(l/16)+j][l%16].
+// bf16 operands, FP32 result; M and N multiples of 16, any K.
+//
+// A workgroup of four wavefronts owns a 64 x 64 block of C and walks its K
+// range (all of K, or one of `splits` consecutive 256-deep ranges) in
+// 32-deep slices: the slice of A (64 x 32) and of B (32 x 64, stored
+// transposed, k contiguous) is staged in LDS with bounds-checked strided
+// loads, then each wavefront runs the 2 x 2 MFMA tiles of its 32 x 32
+// quadrant with gfx950's v_mfma_f32_16x16x32_bf16: lane l reads the 8
+// consecutive k of its quarter l/16 for row l%16 of its A and B fragments
+// (16 bytes each) and holds rows 4(l/16)..4(l/16)+3 of column l%16 of the
+// tile.  Each A and B element is loaded from memory once per workgroup
+// instead of once per 16 x 16 tile.  With splits > 1 every range writes its
+// partial block to part[split][M][N] and k_splitsum adds the ranges in
+// order (deterministic).
+constexpr int GT  = 64;      // block tile (M and N)
+constexpr int GK  = 32;      // K slice
+constexpr int GKP = GK + 8;  // LDS row stride (bf16): 16-byte aligned, rows in distinct banks
+constexpr int GSPLIT_K = 256; // K range per split
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
 __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const uint16_t* __restrict__ A,
                                               long lai, long lak, const uint16_t* __restrict__ B,
-                                              long lbk, long lbj, float* __restrict__ C, long ldc)
+                                              long lbk, long lbj, float* __restrict__ C, long ldc,
+                                              int splits, float* __restrict__ part)
 {
-  const int wave    = threadIdx.x >> 6;
-  const int lane    = threadIdx.x & 63;
-  const int tiles_n = N >> 4;
-  const long tile   = (long)blockIdx.x * 4 + wave;
-  if (tile >= (long)(M >> 4) * tiles_n) { return; } // whole wavefront
-  const long ti = tile / tiles_n, tj = tile % tiles_n;
+  __shared__ __align__(16) uint16_t As[GT * GKP];
+  __shared__ __align__(16) uint16_t Bs[GT * GKP];
+  const int tiles_n = (N + GT - 1) / GT;
+  const int bi = blockIdx.x / tiles_n, bj = blockIdx.x % tiles_n;
+  const int split = blockIdx.y;
+  const int k_lo  = (splits > 1) ? split * GSPLIT_K : 0;
+  const int k_hi  = (splits > 1) ? min(K, k_lo + GSPLIT_K) : K;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int r = lane & 15, q = lane >> 4;
-  const long row = ti * 16 + r, col = tj * 16 + r;
-  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int k0 = 0; k0 < K; k0 += 16)
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
   {
-    s16x4 a, b;
 #pragma unroll
-    for (int j = 0; j < 4; j++)
-    {
-      const long k = k0 + 4 * q + j;
-      a[j]         = (short)A[row * lai + k * lak];
-      b[j]         = (short)B[k * lbk + col * lbj];
-    }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    for (int b = 0; b < 2; b++) { acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f}; }
   }
+  for (int k0 = k_lo; k0 < k_hi; k0 += GK)
+  {
+    // stage: 64 x 32 of A and of B^T, 8 elements per thread each
 #pragma unroll
-  for (int j = 0; j < 4; j++) { C[(ti * 16 + 4 * q + j) * ldc + col] = acc[j]; }
+    for (int e = 0; e < 8; e++)
+    {
+      const int idx = t + 256 * e;      // 0 .. 2047
+      const int row = idx >> 5, kk = idx & 31;
+      const long gi = (long)bi * GT + row, gj = (long)bj * GT + row;
+      const long k  = k0 + kk;
+      As[row * GKP + kk] = (gi < M && k < k_hi) ? A[gi * lai + k * lak] : (uint16_t)0;
+      Bs[row * GKP + kk] = (gj < N && k < k_hi) ? B[k * lbk + gj * lbj] : (uint16_t)0;
+    }
+    __syncthreads();
+    bf16x8_t af[2], bf[2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+    {
+      af[a] = *reinterpret_cast<const bf16x8_t*>(&As[(wm + 16 * a + r) * GKP + 8 * q]);
+      bf[a] = *reinterpret_cast<const bf16x8_t*>(&Bs[(wn + 16 * a + r) * GKP + 8 * q]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+    {
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+      {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bf[b], acc[a][b], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  float* out   = (splits > 1) ? part + (long)split * M * N : C;
+  const long ld = (splits > 1) ? (long)N : ldc;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+  {
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+    {
+      const long col = (long)bj * GT + wn + 16 * b + r;
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+      {
+        const long row = (long)bi * GT + wm + 16 * a + 4 * q + i;
+        if (row < M && col < N) { out[row * ld + col] = acc[a][b][i]; }
+      }
+    }
+  }
 }
+
+// C = sum over the splits of part[split] (in split order)
+__global__ void k_splitsum(const float* __restrict__ part, int splits, int M, int N,
+                           float* __restrict__ C, long ldc)
+{
+  const long n = (long)M * N;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+  {
+    float s = part[i];
+    for (int p = 1; p < splits; p++) { s += part[(long)p * n + i]; }
+    C[(i / N) * ldc + i % N] = s;
+  }
+}
+
+float* g_gemm_part     = nullptr; // split-K partials of the calling network (set per call)
+long g_gemm_part_elems = 0;
 
 void gemm(hipStream_t s, int M, int N, int K, const uint16_t* A, long lai, long lak,
           const uint16_t* B, long lbk, long lbj, float* C, long ldc)
 {
-  long tiles = (long)(M / 16) * (N / 16);
-  k_gemm<<<(unsigned)((tiles + 3) / 4), 256, 0, s>>>(M, N, K, A, lai, lak, B, lbk, lbj, C, ldc);
+  const int tiles = ((M + GT - 1) / GT) * ((N + GT - 1) / GT);
+  int splits      = (K > 2 * GSPLIT_K && tiles < 64) ? (K + GSPLIT_K - 1) / GSPLIT_K : 1;
+  if ((long)splits * M * N > g_gemm_part_elems) { splits = 1; }
+  k_gemm<<<dim3((unsigned)tiles, (unsigned)splits), 256, 0, s>>>(M, N, K, A, lai, lak, B, lbk, lbj,
+                                                                  C, ldc, splits, g_gemm_part);
+  if (splits > 1)
+  {
+    const long n = (long)M * N;
+    k_splitsum<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g_gemm_part, splits, M, N, C, ldc);
+  }
 }
 
 __global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, long n)
@@ -235,6 +319,8 @@ struct SUNAMD_MLP_
   uint16_t *xb, *a1b, *a2b, *dyb, *dz2b, *dz1b;
   float *z1, *z2, *y, *dy, *da2, *dz2, *da1, *dz1, *acc;
   float *hx, *ht, *hm; // staging for the host-array entry points (device)
+  float* gpart;        // split-K partials of the weight-gradient products
+  long gpart_elems;
 };
 
 static int mfail(hipError_t e, const char* what)
@@ -290,6 +376,10 @@ extern "C" SUNAMD_MLP* SUNAMD_MLPCreate(uint64_t seed, int max_batch)
   alloc((void**)&M->hx, sizeof(float) * B * IN);
   alloc((void**)&M->ht, sizeof(float) * B * OUT);
   alloc((void**)&M->hm, sizeof(float) * B * OUT);
+  // the longest K is the batch: at most ceil(B / 256) ranges of the largest
+  // weight block (HID x IN)
+  M->gpart_elems = ((B + 255) / 256) * (long)HID * IN;
+  alloc((void**)&M->gpart, sizeof(float) * M->gpart_elems);
   if (!ok)
   {
     SUNAMD_MLPFree(M);
@@ -337,7 +427,7 @@ extern "C" void SUNAMD_MLPFree(SUNAMD_MLP* M)
   if (M->stream) { (void)hipStreamSynchronize(M->stream); }
   void* ptrs[] = {M->p,   M->g,   M->m,   M->v,   M->wblock, M->xb,  M->a1b, M->a2b,
                   M->dyb, M->dz2b, M->dz1b, M->z1, M->z2,  M->y,   M->dy,  M->da2,
-                  M->dz2, M->da1, M->dz1, M->acc, M->hx,     M->ht,  M->hm};
+                  M->dz2, M->da1, M->dz1, M->acc, M->hx,     M->ht,  M->hm, M->gpart};
   for (void* q : ptrs)
   {
     if (q) { (void)hipFree(q); }
@@ -371,6 +461,8 @@ extern "C" const void* SUNAMD_MLPDeviceWeights(SUNAMD_MLP* M) { return M->wblock
 static void forward_dev(SUNAMD_MLP* M, const float* x, int B)
 {
   hipStream_t s = M->stream;
+  g_gemm_part       = M->gpart;
+  g_gemm_part_elems = M->gpart_elems;
   k_to_bf16<<<blocks_for((long)B * IN), 256, 0, s>>>(x, M->xb, (long)B * IN);
   gemm(s, B, HID, IN, M->xb, IN, 1, M->w1, 1, IN, M->z1, HID);
   k_bias_act<<<blocks_for((long)B * HID), 256, 0, s>>>(M->z1, M->bias, B, HID, M->a1b, 1);

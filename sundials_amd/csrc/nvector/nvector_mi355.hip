@@ -210,12 +210,13 @@ static void exec_release(Mi355Exec* ex)
 // launch geometry
 // ----------------------------------------------------------------------------
 // Element pairs a lane handles per chunk for a kernel that issues `loads`
-// 128-bit loads per pair: about 16 loads in flight per lane (a measured sweep
-// over 6.3M-2^28 elements: 1 pair per lane per iteration leaves ~20 % of the
-// bandwidth unused), fewer pairs when each pair already reads many vectors.
+// 128-bit loads per pair (measured sweeps over 6.3M-2^28 elements): 8 for one
+// or two input vectors (1 pair per lane per iteration leaves ~20 % of the
+// bandwidth unused), 4 for more -- at 2^28 a nine-vector dot product moves
+// 6.7 TB/s with 4 pairs per lane against 6.2-6.4 TB/s with 1.
 __host__ __device__ constexpr int unroll_for(int loads)
 {
-  return loads <= 2 ? 8 : (loads <= 4 ? 4 : (loads <= 8 ? 2 : 1));
+  return loads <= 2 ? 8 : 4;
 }
 
 // streaming: chunks of U * NV_BLOCK pairs per workgroup, grid capped at 32

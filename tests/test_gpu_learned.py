@@ -60,6 +60,33 @@ def test_mlp_forward_matches_reference(hip):
         hip.SUNAMD_MLPFree(M)
 
 
+def test_integrator_network_matches_reference(hip):
+    """The network as the batched integrator evaluates it inside its kernels
+    (one wavefront per 16 rows, bf16 weights as 16 x 16 x 32 MFMA operands)
+    against the float64 reference that rounds at the same points, and
+    against the training-side forward pass; 40 rows, so the last wavefront's
+    block is partly empty."""
+    M = hip.SUNAMD_MLPCreate(11, 256)
+    assert M
+    try:
+        rng = np.random.default_rng(4)
+        x = _inputs(rng, 48)
+        yw = np.zeros((48, OUT), dtype=np.float32)
+        assert hip.SUNAMD_MLPForwardWave(hip.SUNAMD_MLPDeviceWeights(M), x[:40].ctypes.data_as(PF),
+                                         40, yw.ctypes.data_as(PF)) == 0
+        y_ref, _ = forward(_params(M).astype(np.float64), x)
+        scale = np.abs(y_ref).max()
+        err = np.abs(yw[:40] - y_ref[:40]).max()
+        print("in-kernel network vs reference: max |dy| %.3g (scale %.3g)" % (err, scale))
+        assert err <= 2e-3 * scale
+        yf = np.zeros((48, OUT), dtype=np.float32)
+        assert hip.SUNAMD_MLPForward(M, x.ctypes.data_as(PF), 48, yf.ctypes.data_as(PF)) == 0
+        assert np.abs(yw[:40] - yf[:40]).max() <= 2e-3 * scale
+        assert np.all(yw[40:] == 0.0)  # rows beyond the input are not written
+    finally:
+        hip.SUNAMD_MLPFree(M)
+
+
 def test_mlp_gradient_matches_reference(hip):
     M = hip.SUNAMD_MLPCreate(11, 256)
     assert M
@@ -204,6 +231,105 @@ def test_learned_predictor_end_to_end(hip):
               "steps plain %d learned %d" % (nni_p, nni_l, kept.value, tried.value,
                                              c64[:, 0].sum(), cl[:, 0].sum()))
         assert tried.value > 0
+    finally:
+        hip.SUNAMD_MLPFree(M)
+        free_batch(train)
+        free_batch(test)
+
+
+def test_learned_start_seven_size_mix(hip):
+    """Config 5 at test scale over the whole size mix (m in {4, 8, 16, 24, 32,
+    48, 64}, the benchmark's weights): the network is trained on pairs logged
+    by plain FP64 runs of one batch and evaluated on a held-out batch in the
+    product form (FP32 Newton-matrix LU) with the learned start on in every
+    bucket, beside the same form without it.  Printed (the measured gain):
+    Newton iterations per step, steps, wall time and guesses kept per bucket.
+    Asserted: every system succeeds; the guess is tried in every bucket and
+    kept in some steps; Newton iterations per step stay within 5 % of the run
+    without it (the zero start is the fallback whenever the guess's residual
+    is larger, so a poor network cannot make the iteration diverge); accuracy
+    against a 10^4x tighter reference is distributed like the run without
+    it."""
+    import torch
+
+    train = make_batch(2048, 7007)
+    test = make_batch(1400, 8008)
+    M = hip.SUNAMD_MLPCreate(19, 1024)
+    assert M
+    try:
+        x, t, m = _log_pairs(hip, train, 1200000, 1)
+        n = x.shape[0]
+        perm = torch.randperm(n, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+        x, t, m = x[perm].contiguous(), t[perm].contiguous(), m[perm].contiguous()
+        torch.cuda.synchronize()
+        loss = C.c_float()
+        assert hip.SUNAMD_MLPTrain(M, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
+                                   C.c_void_p(m.data_ptr()), n, 1024, 4000, 1e-3,
+                                   C.byref(loss)) == 0
+        del x, t, m
+        print("training pairs %d, final loss %.4g" % (n, loss.value))
+        bc = test.contents
+        nb = bc.nbuckets
+        ms = [bc.bucket[k].contents.m for k in range(nb)]
+        sb = np.array([bc.sys_bucket[s_] for s_ in range(bc.nsys)])
+
+        def run(mask):
+            B = hip.SUNAMD_BatchedBDFCreate(test, RTOL, ATOL, 1)
+            assert B
+            try:
+                hip.SUNAMD_BatchedBDFSetMaxNumSteps(B, MXSTEP)
+                hip.SUNAMD_BatchedBDFSetOccupancy(B, (C.c_int * nb)(*([2] * nb)))
+                if mask:
+                    assert hip.SUNAMD_BatchedBDFSetLearnedPredictor(
+                        B, hip.SUNAMD_MLPDeviceWeights(M), mask) == 0
+                wall = C.c_double()
+                assert hip.SUNAMD_BatchedBDFRun(B, 1.0, C.byref(wall)) == 0  # warm-up
+                assert hip.SUNAMD_BatchedBDFRun(B, 1.0, C.byref(wall)) == 0
+                y = np.zeros(bc.ytotal)
+                c = np.zeros((bc.nsys, _lib.BS_COUNT), dtype=np.int64)
+                assert hip.SUNAMD_BatchedBDFGetResults(B, y.ctypes.data_as(C.POINTER(C.c_double)),
+                                                       c.ctypes.data_as(C.POINTER(C.c_long))) == 0
+                tried = np.zeros(nb, dtype=np.int64)
+                kept = np.zeros(nb, dtype=np.int64)
+                assert hip.SUNAMD_BatchedBDFGetLearnedStatsByBucket(
+                    B, tried.ctypes.data_as(C.POINTER(C.c_long)),
+                    kept.ctypes.data_as(C.POINTER(C.c_long))) == nb
+                return y, c, wall.value, tried, kept
+            finally:
+                hip.SUNAMD_BatchedBDFFree(B)
+
+        yp, cp, wp, _, _ = run(0)
+        yl, cl, wl, tried, kept = run((1 << nb) - 1)
+        assert np.all(cp[:, 9] == 0) and np.all(cl[:, 9] == 0)
+        for k in range(nb):
+            sel = sb == k
+            print("m=%2d: Newton iters/step plain %.4f learned %.4f, steps %d / %d, guess kept "
+                  "%d of %d" % (ms[k], cp[sel, 4].sum() / cp[sel, 0].sum(),
+                                cl[sel, 4].sum() / cl[sel, 0].sum(), cp[sel, 0].sum(),
+                                cl[sel, 0].sum(), kept[k], tried[k]))
+        nni_p = cp[:, 4].sum() / cp[:, 0].sum()
+        nni_l = cl[:, 4].sum() / cl[:, 0].sum()
+        print("all sizes: Newton iters/step plain %.4f learned %.4f; wall plain %.2f ms learned "
+              "%.2f ms; guess kept %d of %d" % (nni_p, nni_l, 1e3 * wp, 1e3 * wl, kept.sum(),
+                                               tried.sum()))
+        assert np.all(tried > 0) and kept.sum() > 0
+        assert nni_l <= nni_p * 1.05
+        y_tight, _, rc, _ = run_oracle(test, nthreads=16, rtol=1e-10, atol=1e-14)
+        assert rc == 0
+
+        def errs(y):
+            out = []
+            for (k, c_, mm, off) in system_slices(test):
+                d = y[off:off + mm] - y_tight[off:off + mm]
+                w = 1.0 / (RTOL * np.abs(y_tight[off:off + mm]) + ATOL)
+                out.append(np.sqrt(np.mean((d * w) ** 2)))
+            return np.array(out)
+
+        el, ep = errs(yl), errs(yp)
+        print("error vs tight reference: plain median %.3g p95 %.3g, learned median %.3g p95 %.3g"
+              % (np.median(ep), np.percentile(ep, 95), np.median(el), np.percentile(el, 95)))
+        assert np.median(el) <= 1.5 * np.median(ep) + 0.05
+        assert np.percentile(el, 95) <= 1.5 * np.percentile(ep, 95) + 0.1
     finally:
         hip.SUNAMD_MLPFree(M)
         free_batch(train)
