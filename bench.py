@@ -152,6 +152,43 @@ def vector_microbench(args, hip):
     return micro
 
 
+def bench_blocklu(args):
+    """--workload blocklu: the block LU factorisation of 2^20 blocks of 32 x 32
+    and 2^18 of 64 x 64 (8 GiB of FP64 blocks each) by the FP64 and FP32
+    register-row kernels and the FP32 blocked kernel with MFMA trailing
+    updates.  value = the fastest FP32 variant's GB/s at m = 32."""
+    from sundials_amd import _lib
+
+    hip = _lib.hip()
+    out = {}
+    for m, nblocks in ((32, 1 << 20), (64, 1 << 18)):
+        row = {}
+        for variant, name in ((0, "fp64_register"), (1, "fp32_register"), (2, "fp32_mfma")):
+            ms = C.c_double()
+            if hip.SUNAMD_BenchBlockLU(nblocks, m, variant, 10, C.byref(ms)) != 0:
+                print("[bench] LU benchmark failed", file=sys.stderr)
+                sys.exit(1)
+            wbytes = 8 if variant == 0 else 4
+            traffic = nblocks * (8 * m * m + wbytes * m * m + 4 * m)
+            row[name] = {"ms": round(ms.value, 4),
+                         "gbs": round(traffic / (ms.value * 1e-3) / 1e9, 1),
+                         "blocks_per_s": nblocks / (ms.value * 1e-3)}
+            print("[bench] LU m=%d %s: %.3f ms, %.0f GB/s" % (m, name, ms.value, row[name]["gbs"]),
+                  file=sys.stderr, flush=True)
+        out["m%d" % m] = dict(row, blocks=nblocks)
+    best = max(out["m32"]["fp32_register"]["gbs"], out["m32"]["fp32_mfma"]["gbs"])
+    return {"metric": "block LU factorisation GB/s (algorithmic bytes), FP32 factors",
+            "value": best, "unit": "GB/s", "n_gpus": 1, "steps": 10, "warmup": 1,
+            "ms_per_step": min(out["m32"]["fp32_register"]["ms"], out["m32"]["fp32_mfma"]["ms"]),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": "batched block LU, 2^20 blocks of 32 x 32 and 2^18 of "
+                                   "64 x 64, U(-1,1)/m + I"},
+            "roofline": {"bound": "hbm", "achieved": best, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(best / HBM_PEAK_GBS, 4), "traffic": None},
+            "cpu_baseline": None, "results": out}
+
+
 def bench_vector(args):
     """--workload vector: the N_Vector microbench alone (LinearSum, WrmsNorm,
     DotProdMulti(8), in-place LinearSum, DotProd at N = 2^28 FP64) against the
@@ -415,20 +452,26 @@ def train_predictor(args, core, hip):
 
 
 def autotune(core, hip, prec, args_systems, weights=None):
-    """Per-bucket choices made on an independent calibration batch of the
-    measured batch's size: the kernel's register budget (1, 2 or 4 wavefronts per SIMD,
-    i.e. occupancy against spilling) and, given a trained network, whether
-    the learned start pays off.  Each choice keeps the bucket's shorter
-    kernel time."""
+    """Per-bucket launch choices made on an independent calibration batch of
+    the measured batch's size (a bucket's choice only shows its effect when
+    the bucket fills the GPU as it does in the measured batch):
+      1. the kernel's register budget (1, 2, 3 or 4 wavefronts per SIMD:
+         occupancy against spilling);
+      2. the launch size (2 .. 32 workgroups per CU: how many systems each
+         lane group takes from the bucket's queue, i.e. load balance inside
+         a wavefront against wavefronts for the GPU to schedule);
+      3. given a trained network, whether the learned start pays off.
+    Buckets share the GPU, so per-bucket minima do not add up: a per-bucket
+    mix is kept only if the whole batch runs faster with it than with the
+    best uniform choice.  Returns (waves, caps, learned mask, info)."""
     nb = len(BATCH_M)
-    # full size: a bucket's occupancy choice only shows its benefit when the
-    # bucket fills the GPU as it does in the measured batch
     bcal = _batch(core, args_systems, CALIB_SEED)
 
-    def kernel_ms(waves, mask):
+    def run_cfg(waves, caps, mask):
         B = hip.SUNAMD_BatchedBDFCreate(bcal, RTOL, ATOL, prec)
         hip.SUNAMD_BatchedBDFSetMaxNumSteps(B, BATCH_MXSTEP)
         hip.SUNAMD_BatchedBDFSetOccupancy(B, (C.c_int * nb)(*waves))
+        hip.SUNAMD_BatchedBDFSetGridCap(B, (C.c_int * nb)(*caps))
         if weights is not None:
             hip.SUNAMD_BatchedBDFSetLearnedPredictor(B, weights, mask)
         wall = C.c_double()
@@ -440,29 +483,36 @@ def autotune(core, hip, prec, args_systems, weights=None):
         hip.SUNAMD_BatchedBDFFree(B)
         return kms, wall.value
 
-    runs = {w: kernel_ms([w] * nb, 0) for w in (1, 2, 3, 4)}
-    table = {w: runs[w][0] for w in runs}
-    best_w = [min((1, 2, 3, 4), key=lambda w: table[w][k]) for k in range(nb)]
-    # buckets share the GPU, so the per-bucket mix is kept only if the whole
-    # batch is faster with it than with the best uniform budget
-    mix_ms, mix_wall = kernel_ms(best_w, 0)
-    w_uni = min(runs, key=lambda w: runs[w][1])
-    if runs[w_uni][1] < mix_wall:
-        best_w, mix_ms = [w_uni] * nb, table[w_uni]
-    best = np.array(mix_ms)
-    info = {"waves": best_w, "ms_by_waves": {str(w): [round(v, 2) for v in table[w]]
-                                            for w in table},
-            "wall_ms_by_waves": {str(w): round(runs[w][1] * 1e3, 2) for w in runs},
-            "wall_ms_mix": round(mix_wall * 1e3, 2)}
+    def choose(options, make_cfg):
+        runs = {o: run_cfg(*make_cfg([o] * nb)) for o in options}
+        per = [min(options, key=lambda o: runs[o][0][k]) for k in range(nb)]
+        uni = min(options, key=lambda o: runs[o][1])
+        mix_ms, mix_wall = run_cfg(*make_cfg(per))
+        if runs[uni][1] <= mix_wall:
+            return [uni] * nb, runs[uni], runs, mix_wall
+        return per, (mix_ms, mix_wall), runs, mix_wall
+
+    caps0 = [16] * nb
+    best_w, _, wruns, wmix = choose((1, 2, 3, 4), lambda w: (w, caps0, 0))
+    best_c, (best_ms, best_wall), cruns, cmix = choose((2, 4, 8, 16, 32),
+                                                        lambda c: (best_w, c, 0))
+    info = {"waves": best_w, "grid_cap_per_cu": best_c,
+            "ms_by_waves": {str(w): [round(v, 2) for v in wruns[w][0]] for w in wruns},
+            "wall_ms_by_waves": {str(w): round(wruns[w][1] * 1e3, 2) for w in wruns},
+            "wall_ms_waves_mix": round(wmix * 1e3, 2),
+            "ms_by_grid_cap": {str(c): [round(v, 2) for v in cruns[c][0]] for c in cruns},
+            "wall_ms_by_grid_cap": {str(c): round(cruns[c][1] * 1e3, 2) for c in cruns},
+            "wall_ms_grid_cap_mix": round(cmix * 1e3, 2),
+            "wall_ms_chosen": round(best_wall * 1e3, 2)}
     mask = 0
     if weights is not None:
-        lms, _ = kernel_ms(best_w, (1 << nb) - 1)
+        lms, _ = run_cfg(best_w, best_c, (1 << nb) - 1)
         for k in range(nb):
-            if lms[k] < best[k]:
+            if lms[k] < best_ms[k]:
                 mask |= 1 << k
         info["learned_ms"] = [round(v, 2) for v in lms]
     core.SUNAMD_BatchFree(bcal)
-    return best_w, mask, info
+    return best_w, best_c, mask, info
 
 
 IMEX_SEED = 1003
@@ -492,7 +542,7 @@ def stream_traffic(workload, kernel, n):
 
 # the kernel source the MEASURED_TRAFFIC figures were taken at (first 16 hex digits of the
 # SHA-256 of bdf_device.h followed by bdf_args.h); a changed kernel reports traffic null
-MEASURED_TRAFFIC_SOURCE = "fa969dec90e61d87"
+MEASURED_TRAFFIC_SOURCE = "ecca45b33cc880d6"
 
 
 def batched_kernel_source_hash():
@@ -505,14 +555,18 @@ def batched_kernel_source_hash():
     return h.hexdigest()[:16]
 
 
+# HBM bytes per launch of the FP32-LU kernels at three wavefronts per SIMD (the budget the
+# autotuner picks), rocprofv3 --pmc FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (the factors a
+# plain 256 MiB copy gives) on the 65,536-system batch; almost all of it is register-spill
+# stores (the algorithmic traffic, states and rate constants, is ~20 MB per launch)
 MEASURED_TRAFFIC = {
-    "k_batched_bdf<4, float, 2, false>": 8.66e6,
-    "k_batched_bdf<8, float, 2, false>": 12.28e6,
-    "k_batched_bdf<16, float, 2, false>": 13.03e6,
-    "k_batched_bdf<24, float, 2, false>": 12.21e6,
-    "k_batched_bdf<32, float, 2, false>": 16.19e6,
-    "k_batched_bdf<48, float, 2, false>": 18.65e6,
-    "k_batched_bdf<64, float, 2, false>": 87.34e6,
+    "k_batched_bdf<4, float, 3, false>": 19.32e6,
+    "k_batched_bdf<8, float, 3, false>": 34.18e6,
+    "k_batched_bdf<16, float, 3, false>": 343.8e6,
+    "k_batched_bdf<24, float, 3, false>": 146.8e6,
+    "k_batched_bdf<32, float, 3, false>": 1136.0e6,
+    "k_batched_bdf<48, float, 3, false>": 3347.6e6,
+    "k_batched_bdf<64, float, 3, false>": 6282.4e6,
 }
 
 
@@ -873,13 +927,14 @@ def bench_batched(args):
     sb = np.ctypeslib.as_array(bc.sys_bucket, shape=(bc.nsys,)).copy()
     msys, rsys = bm[sb], bR[sb]
 
-    def make(prec, waves):
+    def make(prec, waves, caps):
         B = hip.SUNAMD_BatchedBDFCreate(b, RTOL, ATOL, prec)
         if not B:
             print("[bench] device setup failed", file=sys.stderr)
             sys.exit(1)
         hip.SUNAMD_BatchedBDFSetMaxNumSteps(B, BATCH_MXSTEP)
         hip.SUNAMD_BatchedBDFSetOccupancy(B, (C.c_int * nb)(*waves))
+        hip.SUNAMD_BatchedBDFSetGridCap(B, (C.c_int * nb)(*caps))
         return B
 
     def results(B):
@@ -902,12 +957,15 @@ def bench_batched(args):
 
     # plain FP64 form (register budget tuned per bucket): warm up, then time one integration
     fixed = None if args.waves == "auto" else [int(args.waves)] * nb
+    fixed_cap = [args.grid_cap] * nb if args.grid_cap > 0 else [16] * nb
     if fixed is None:
-        w64, _, tune64 = autotune(core, hip, LU_FP64, args.systems)
+        w64, c64cap, _, tune64 = autotune(core, hip, LU_FP64, args.systems)
     else:
-        w64, tune64 = fixed, {"waves": fixed, "fixed": True}
-    print("[bench] fp64 waves per bucket: %s" % w64, file=sys.stderr, flush=True)
-    B64 = make(LU_FP64, w64)
+        w64, c64cap, tune64 = fixed, fixed_cap, {"waves": fixed, "grid_cap_per_cu": fixed_cap,
+                                                 "fixed": True}
+    print("[bench] fp64 waves per bucket: %s, workgroups per CU %s" % (w64, c64cap),
+          file=sys.stderr, flush=True)
+    B64 = make(LU_FP64, w64, c64cap)
     run(B64, "fp64 warmup")
     t64 = run(B64, "fp64")
     y64, c64 = results(B64)
@@ -924,15 +982,17 @@ def bench_batched(args):
         mlp, learned = train_predictor(args, core, hip)
         weights = hip.SUNAMD_MLPDeviceWeights(mlp)
     if fixed is None:
-        w32, mask, tune32 = autotune(core, hip, LU_FP32, args.systems, weights)
+        w32, c32cap, mask, tune32 = autotune(core, hip, LU_FP32, args.systems, weights)
     else:
-        w32, mask, tune32 = fixed, 0, {"waves": fixed, "fixed": True}
-    print("[bench] fp32 waves per bucket: %s, learned mask %d" % (w32, mask), file=sys.stderr,
-          flush=True)
+        w32, c32cap, mask, tune32 = fixed, fixed_cap, 0, {"waves": fixed,
+                                                          "grid_cap_per_cu": fixed_cap,
+                                                          "fixed": True}
+    print("[bench] fp32 waves per bucket: %s, workgroups per CU %s, learned mask %d"
+          % (w32, c32cap, mask), file=sys.stderr, flush=True)
     if learned is not None:
         learned["bucket_mask"] = mask
         learned["buckets_on"] = [int(BATCH_M[k]) for k in range(len(BATCH_M)) if mask >> k & 1]
-    B = make(LU_FP32, w32)
+    B = make(LU_FP32, w32, c32cap)
     if mlp is not None:
         hip.SUNAMD_BatchedBDFSetLearnedPredictor(B, hip.SUNAMD_MLPDeviceWeights(mlp), mask)
     for w in range(args.warmup):
@@ -960,7 +1020,7 @@ def bench_batched(args):
     # Newton iterations per step and guesses kept
     learned_all, y_la = None, None
     if mlp is not None:
-        BL = make(LU_FP32, w32)
+        BL = make(LU_FP32, w32, c32cap)
         hip.SUNAMD_BatchedBDFSetLearnedPredictor(BL, hip.SUNAMD_MLPDeviceWeights(mlp),
                                                  (1 << nb) - 1)
         run(BL, "learned warmup")
@@ -1201,7 +1261,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["batched", "coupled", "imex", "kinsol", "kinsol-newton",
-                             "phases", "vector"],
+                             "phases", "vector", "blocklu"],
                     default="batched")
     ap.add_argument("--kin-cells", type=int, default=1 << 21, help="kinsol: cells (3 unknowns)")
     ap.add_argument("--kin-cpu-cells", type=int, default=1 << 21, help="kinsol: CPU-oracle cells")
@@ -1224,6 +1284,8 @@ def main():
     ap.add_argument("--waves", choices=["auto", "1", "2", "3", "4"], default="auto",
                     help="wavefronts per SIMD for every bucket (auto: calibrated per bucket; "
                          "fixed values also keep the learned start off)")
+    ap.add_argument("--grid-cap", type=int, default=0,
+                    help="with fixed --waves: workgroups per CU for every bucket (0: 16)")
     ap.add_argument("--train-systems", type=int, default=32768,
                     help="systems of the independent batch whose plain runs are logged")
     ap.add_argument("--train-pairs", type=int, default=10000000,
@@ -1256,6 +1318,8 @@ def main():
         line = bench_phases(args)
     elif args.workload == "vector":
         line = bench_vector(args)
+    elif args.workload == "blocklu":
+        line = bench_blocklu(args)
     else:
         line = bench_coupled(args)
     print(json.dumps(line))

@@ -88,6 +88,13 @@ int SUNAMD_BatchedBDFGetResults(SUNAMD_BatchedBDF* B, double* y, long int* count
    3 applies to buckets without the learned start, which otherwise use 2) --
    the occupancy-versus-spilling choice an autotuner makes per size bucket */
 int SUNAMD_BatchedBDFSetOccupancy(SUNAMD_BatchedBDF* B, const int* waves);
+/* launch size of each bucket: at most per_cu[k] workgroups (one wavefront
+   each) per CU, 1..64 (default 16).  Fewer wavefronts than the bucket's
+   systems make each lane group take several systems from the bucket's work
+   queue, which evens out systems of different step counts inside a
+   wavefront; more give the GPU more wavefronts to schedule -- the second
+   launch-shape choice an autotuner makes per size bucket */
+int SUNAMD_BatchedBDFSetGridCap(SUNAMD_BatchedBDF* B, const int* per_cu);
 /* Learned initial Newton iterate (no counterpart in the reference, whose
    Newton solve always starts from a zero correction, cvode.c:3095-3109):
    weights = SUNAMD_MLPDeviceWeights of a trained network (NULL: off),

@@ -48,6 +48,13 @@ int SUNAMD_ProfilerGet(int family, double* total_ms, long int* count);
    holds 8 values; n must be even. */
 int SUNAMD_BenchVectorOps(sunindextype n, int nvec_multi, int reps, double* out_ms);
 
+/* block LU factorisation benchmark: nblocks blocks of m x m (m = 32 or 64,
+   entries U(-1,1)/m plus the identity, generated on the device); average ms
+   per factorisation of the whole batch over reps launches after a warm-up.
+   variant 0: FP64 register-row kernel, 1: FP32 register-row kernel, 2: FP32
+   blocked kernel with MFMA trailing updates. */
+int SUNAMD_BenchBlockLU(long int nblocks, int m, int variant, int reps, double* ms_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,13 @@ SUNErrCode SUNMatrix_Mi355BlockDense_CopyFromDevice(SUNMatrix A);
 
 SUNLinearSolver SUNLinSol_Mi355BlockLU(N_Vector y, SUNMatrix A, int precision,
                                        SUNContext sunctx);
+/* FP32 factors of 32 x 32 and 64 x 64 blocks by the blocked kernel whose
+   trailing updates run on MFMA (v_mfma_f32_16x16x4f32; panels of 16 columns,
+   the block staged in LDS) instead of the register-row kernel; no
+   counterpart in the reference (MAGMA's batched GETRF plays this part for
+   sunlinsol_magmadense.cpp:339).  Same pivots and factor layout, so the FP32
+   solve is shared.  SUN_ERR_ARG_INCOMPATIBLE for FP64 solvers or other m. */
+SUNErrCode SUNLinSol_Mi355BlockLUSetMFMA(SUNLinearSolver S, int onoff);
 /* device pivot array (int32, nblocks*m, pivot row of step k of block b at b*m+k) */
 int* SUNLinSol_Mi355BlockLU_DevicePivots(SUNLinearSolver S);
 /* copy the pivots to a host array of nblocks*m ints (synchronous) */

@@ -299,6 +299,7 @@ def _batched_sigs(lib):
     _sig(lib, "SUNAMD_BatchedBDFGetKernelTimes", C.c_int, [c_void_p, P(c_real)])
     _sig(lib, "SUNAMD_BatchedBDFSetLearnedPredictor", C.c_int, [c_void_p, c_void_p, C.c_int])
     _sig(lib, "SUNAMD_BatchedBDFSetOccupancy", C.c_int, [c_void_p, P(C.c_int)])
+    _sig(lib, "SUNAMD_BatchedBDFSetGridCap", C.c_int, [c_void_p, P(C.c_int)])
     _sig(lib, "SUNAMD_BatchedBDFSetTrainingLog", C.c_int,
          [c_void_p, c_void_p, c_void_p, c_void_p, C.c_long, C.c_int])
     _sig(lib, "SUNAMD_BatchedBDFGetTrainingLogCount", C.c_long, [c_void_p])
@@ -379,6 +380,8 @@ def hip():
     _sig(lib, "SUNAMD_ProfilerReset", None, [])
     _sig(lib, "SUNAMD_ProfilerGet", C.c_int, [C.c_int, P(C.c_double), P(C.c_long)])
     _sig(lib, "SUNAMD_BenchVectorOps", C.c_int, [c_index, C.c_int, C.c_int, P(c_real)])
+    _sig(lib, "SUNAMD_BenchBlockLU", C.c_int, [C.c_long, C.c_int, C.c_int, C.c_int, P(c_real)])
+    _sig(lib, "SUNLinSol_Mi355BlockLUSetMFMA", C.c_int, [c_void_p, C.c_int])
     _sig(lib, "SUNAMD_ClassicalGS_Mi355", C.c_int, [P(c_void_p), P(P(c_real)), C.c_int, C.c_int,
                                                     P(c_real), P(c_real), P(c_void_p)])
     _sig(lib, "SUNAMD_TridiagATimes_Mi355", C.c_int, [c_void_p, c_void_p, c_void_p])

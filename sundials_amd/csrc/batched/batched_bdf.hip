@@ -59,6 +59,7 @@ struct SUNAMD_BatchedBDF_
   const void* mlp;      // device weight block (SUNAMD_MLPDeviceWeights) or NULL
   int learned_mask;     // bit k: bucket k uses it
   int waves[SUNAMD_BATCH_MAXB]; // register budget of bucket k's kernel (1, 2 or 4 waves/SIMD)
+  int grid_cap[SUNAMD_BATCH_MAXB]; // bucket k's launch: at most this many workgroups per CU
   float *log_x, *log_t, *log_m;
   long log_cap;
   int log_stride;
@@ -147,7 +148,11 @@ extern "C" SUNAMD_BatchedBDF* SUNAMD_BatchedBDFCreate(const SUNAMD_Batch* batch,
   B->atol = atol;
   B->prec = lu_precision;
   B->mxstep = MXSTEP_DEFAULT;
-  for (int k = 0; k < SUNAMD_BATCH_MAXB; k++) { B->waves[k] = 1; }
+  for (int k = 0; k < SUNAMD_BATCH_MAXB; k++)
+  {
+    B->waves[k]    = 1;
+    B->grid_cap[k] = 16;
+  }
   if (hipMalloc((void**)&B->dcount, sizeof(unsigned long long) * (1 + 2 * SUNAMD_BATCH_MAXB)) !=
         hipSuccess ||
       hipMemset(B->dcount, 0, sizeof(unsigned long long) * (1 + 2 * SUNAMD_BATCH_MAXB)) != hipSuccess)
@@ -321,7 +326,7 @@ extern "C" int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* w
     if (e != hipSuccess) { return hfail(e, "queue reset"); }
     const int gpw = 64 / pow2ceil(d.m);
     long want     = (d.ncells + gpw - 1) / gpw;
-    long cap      = (long)B->num_cus * 16;
+    long cap      = (long)B->num_cus * B->grid_cap[order[n]];
     int grid      = (int)(want < cap ? want : cap);
     e = hipEventRecord(d.ev0, d.stream);
     if (e != hipSuccess) { return hfail(e, "event record"); }
@@ -407,6 +412,17 @@ extern "C" int SUNAMD_BatchedBDFSetOccupancy(SUNAMD_BatchedBDF* B, const int* wa
     if (waves[k] < 1 || waves[k] > 4) { return -1; }
   }
   for (int k = 0; k < B->nb; k++) { B->waves[k] = waves[k]; }
+  return 0;
+}
+
+extern "C" int SUNAMD_BatchedBDFSetGridCap(SUNAMD_BatchedBDF* B, const int* per_cu)
+{
+  if (!B || !per_cu) { return -1; }
+  for (int k = 0; k < B->nb; k++)
+  {
+    if (per_cu[k] < 1 || per_cu[k] > 64) { return -1; }
+  }
+  for (int k = 0; k < B->nb; k++) { B->grid_cap[k] = per_cu[k]; }
   return 0;
 }
 

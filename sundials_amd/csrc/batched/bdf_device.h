@@ -191,7 +191,10 @@ struct Grp
 template <int M, typename T>
 __device__ __forceinline__ int ph_set(const Grp<M, T>& g, int p)
 {
+  // no instruction is scheduled across the clock read
+  __builtin_amdgcn_sched_barrier(0);
   const unsigned long long t = __builtin_amdgcn_s_memtime();
+  __builtin_amdgcn_sched_barrier(0);
   const int prev             = g.sc->ph_cur;
   if (g.i == 0)
   {
@@ -205,10 +208,14 @@ __device__ __forceinline__ int ph_set(const Grp<M, T>& g, int p)
 #define BDF_PH_PUSH(g, p) const int ph_prev_ = ph_set(g, p)
 #define BDF_PH_POP(g) ph_set(g, ph_prev_)
 #define BDF_PH_SET(g, p) ph_set(g, p)
+// a phase's result is complete before the clock is read at its end (the
+// compiler may otherwise sink the work past the read)
+#define BDF_PH_FENCE(v) asm volatile("" ::"v"(v))
 #else
 #define BDF_PH_PUSH(g, p) ((void)0)
 #define BDF_PH_POP(g) ((void)0)
 #define BDF_PH_SET(g, p) ((void)0)
+#define BDF_PH_FENCE(v) ((void)0)
 #endif
 
 // The product form (single-precision factors) is free of the FP64 form's
@@ -307,6 +314,7 @@ __device__ __forceinline__ double g_wrms(const Grp<M, T>& g, double x, double w)
   if constexpr (kFastFactor<T>)
   {
     const double s = g_tree_sum<pow2ceil(M)>(g.valid ? p * p : 0.0);
+    BDF_PH_FENCE(s);
     BDF_PH_POP(g);
     return sqrt0(s / (double)M);
   }
@@ -317,6 +325,7 @@ __device__ __forceinline__ double g_wrms(const Grp<M, T>& g, double x, double w)
 #pragma unroll
   for (int k = 0; k < M; k++) { s += g.red[k]; }
   gsync();
+  BDF_PH_FENCE(s);
   BDF_PH_POP(g);
   return sqrt0(s / (double)M);
 }
@@ -363,15 +372,26 @@ __device__ __forceinline__ double g_rhs(const Grp<M, T>& g, double yi)
   double f = 0.0;
   if (g.valid)
   {
+    // four terms per pass: their codes, then their rates, are read together
+    // (independent LDS reads in flight), then added in term order
     const int e1 = g.soff[g.i + 1];
-    for (int e = g.soff[g.i]; e < e1; e++)
+    for (int e = g.soff[g.i]; e < e1; e += 4)
     {
-      const int code = g.sent[e];
-      const double w = g.wbuf[code >> 2];
-      f              = ((code & 3) < 2) ? f - w : f + w;
+      int code[4];
+      double w[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) { code[q] = (e + q < e1) ? g.sent[e + q] : 0; }
+#pragma unroll
+      for (int q = 0; q < 4; q++) { w[q] = g.wbuf[code[q] >> 2]; }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+      {
+        if (e + q < e1) { f = ((code[q] & 3) < 2) ? f - w[q] : f + w[q]; }
+      }
     }
   }
   gsync();
+  BDF_PH_FENCE(f);
   BDF_PH_POP(g);
   return f;
 }
@@ -986,6 +1006,8 @@ __device__ __forceinline__ int g_lsetup(const Grp<M, T>& g, Sys<M, T>& S, bool j
   // the Newton-matrix row is built in the factor registers themselves
   BDF_PH_PUSH(g, SUNAMD_PH_JAC);
   g_newton_matrix(g, S.ysav, S.gamma, S.lu);
+#pragma unroll
+  for (int j = 0; j < M; j++) { BDF_PH_FENCE(S.lu[j]); }
   if (jcur)
   {
     S.nje++;
@@ -993,6 +1015,9 @@ __device__ __forceinline__ int g_lsetup(const Grp<M, T>& g, Sys<M, T>& S, bool j
   }
   BDF_PH_SET(g, SUNAMD_PH_GETRF);
   int fail = g_getrf(g, S);
+#pragma unroll
+  for (int j = 0; j < M; j++) { BDF_PH_FENCE(S.lu[j]); }
+  BDF_PH_FENCE(fail);
   BDF_PH_POP(g);
   S.nsetups++;
   S.nls_jcur = jcur;
@@ -1008,6 +1033,7 @@ __device__ __forceinline__ void g_lsolve(const Grp<M, T>& g, Sys<M, T>& S)
 {
   BDF_PH_PUSH(g, SUNAMD_PH_GETRS);
   double x = g_getrs(g, S, S.delta);
+  BDF_PH_FENCE(x);
   BDF_PH_POP(g);
   S.delta  = x;
   if (S.gamrat != 1.0) { S.delta = (2.0 / (1.0 + S.gamrat)) * S.delta; }
@@ -1341,6 +1367,7 @@ __device__ __forceinline__ double g_pow(const Grp<M, T>& g, double x, double y)
 {
   BDF_PH_PUSH(g, SUNAMD_PH_POW);
   const double r = kFastFactor<T> ? fast_pow(x, y) : sunamd_pow(x, y);
+  BDF_PH_FENCE(r);
   BDF_PH_POP(g);
   return r;
 }

@@ -382,6 +382,121 @@ __global__ void __launch_bounds__(64) k_getrs_lds(const T* __restrict__ F, const
 __global__ void k_info_reset(unsigned long long* info) { *info = ULLONG_MAX; }
 
 // ----------------------------------------------------------------------------
+// FP32 blocked LU with an MFMA trailing update (blocks of 32 and 64)
+// ----------------------------------------------------------------------------
+// One block per wavefront, staged in LDS row-major (stride M + 1 floats).
+// Right-looking with panels of 16 columns: the panel is factored column by
+// column (partial pivoting with physical row swaps across the whole block,
+// the reference's choice of the first largest |a_ik|), the panel's pivot
+// rows get their trailing columns by forward substitution with the panel's
+// unit lower triangle (U12), and the trailing block is updated
+// A22 -= L21 U12 with v_mfma_f32_16x16x4f32 tiles: lane l supplies
+// L21[row l%16][k + l/16] and U12[k + l/16][column l%16] and holds rows
+// 4(l/16) .. 4(l/16)+3 of column l%16 of each 16 x 16 tile.  Factors and
+// pivots are stored as the register kernel stores them, so the FP32 solve is
+// shared.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+template <int M>
+__global__ void __launch_bounds__(64) k_getrf_mfma(const double* __restrict__ A,
+                                                   float* __restrict__ F, int* __restrict__ piv,
+                                                   long nblocks, unsigned long long* info)
+{
+  constexpr int NB = 16, LD = M + 1;
+  __shared__ float S[M * LD];
+  const int t    = threadIdx.x;
+  const long blk = blockIdx.x;
+  if (blk >= nblocks) { return; } // whole workgroup
+  const double* Ab = A + blk * (long)M * M;
+  for (int e = t; e < M * M; e += 64) { S[(e % M) * LD + e / M] = (float)Ab[e]; }
+  __syncthreads();
+  int fail = 0;
+  int* pb  = piv + blk * M;
+  for (int k0 = 0; k0 < M; k0 += NB)
+  {
+    // panel: columns k0 .. k0+NB-1, lane t = row t
+    for (int k = k0; k < k0 + NB; k++)
+    {
+      const bool cand = (t < M) && (t >= k);
+      float v         = cand ? fabsf(S[t * LD + k]) : -1.0f;
+      int p           = cand ? t : INT_MAX;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1)
+      {
+        const float ov = __shfl_xor(v, off, 64);
+        const int op   = __shfl_xor(p, off, 64);
+        if (ov > v || (ov == v && op < p))
+        {
+          v = ov;
+          p = op;
+        }
+      }
+      if (p < k || p >= M) { p = k; } // no comparable candidate (NaN column): no swap
+      if (t == 0) { pb[k] = p; }
+      if (!(v > 0.0f) && fail == 0) { fail = k + 1; }
+      __syncthreads();
+      if (p != k && t < M)
+      {
+        const float a = S[k * LD + t], b = S[p * LD + t];
+        S[k * LD + t] = b;
+        S[p * LD + t] = a;
+      }
+      __syncthreads();
+      if (fail == 0 && t > k && t < M)
+      {
+        const float l = S[t * LD + k] * (1.0f / S[k * LD + k]);
+        S[t * LD + k] = l;
+        for (int j = k + 1; j < k0 + NB; j++) { S[t * LD + j] = fmaf(-l, S[k * LD + j], S[t * LD + j]); }
+      }
+      __syncthreads();
+    }
+    if (k0 + NB >= M) { break; }
+    // U12: the pivot rows' trailing columns (lane t = column k0 + NB + t)
+    const int j = k0 + NB + t;
+    if (fail == 0 && j < M)
+    {
+      for (int k = k0; k < k0 + NB; k++)
+      {
+        const float ukj = S[k * LD + j];
+        for (int i = k + 1; i < k0 + NB; i++) { S[i * LD + j] = fmaf(-S[i * LD + k], ukj, S[i * LD + j]); }
+      }
+    }
+    __syncthreads();
+    // A22 -= L21 U12, one 16 x 16 tile at a time
+    if (fail == 0)
+    {
+      const int r = t & 15, q = t >> 4;
+      const int n2 = (M - k0 - NB) / 16;
+      for (int ti = 0; ti < n2; ti++)
+      {
+        for (int tj = 0; tj < n2; tj++)
+        {
+          const int i0 = k0 + NB + 16 * ti, j0 = k0 + NB + 16 * tj;
+          f32x4_t acc  = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int kk = 0; kk < NB; kk += 4)
+          {
+            const float a = S[(i0 + r) * LD + k0 + kk + q];
+            const float b = S[(k0 + kk + q) * LD + j0 + r];
+            acc           = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int e = 0; e < 4; e++)
+          {
+            float* d = &S[(i0 + 4 * q + e) * LD + j0 + r];
+            *d       = *d - acc[e];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float* Fb = F + blk * (long)M * M;
+  for (int e = t; e < M * M; e += 64) { Fb[e] = S[(e % M) * LD + e / M]; }
+  if (t == 0 && fail != 0) { atomicMin(info, (unsigned long long)(blk * M + fail)); }
+}
+
+// ----------------------------------------------------------------------------
 // launch dispatch over block sizes with register-resident kernels
 // ----------------------------------------------------------------------------
 #define LU_SIZES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(12) X(16) X(24) X(32) X(48) X(64)
@@ -615,6 +730,7 @@ struct BlockLU
 {
   sunindextype nblocks, m;
   int precision;
+  int mfma;                  // FP32 factors by the MFMA blocked kernel (m = 32, 64)
   int* piv;                  // device, nblocks*m
   float* f32;                // device FP32 factors (FP32 mode)
   unsigned long long* dinfo; // device
@@ -650,7 +766,21 @@ static int lu_setup(SUNLinearSolver S, SUNMatrix A)
   k_info_reset<<<1, 1, 0, c->stream>>>(c->dinfo);
   bool ok;
   sunamd_prof_begin(SUNAMD_K_GETRF, c->stream);
-  if (c->precision == SUNAMD_LU_FP32)
+  if (c->precision == SUNAMD_LU_FP32 && c->mfma)
+  {
+    ok = true;
+    if (c->m == 32)
+    {
+      k_getrf_mfma<32><<<(unsigned)c->nblocks, 64, 0, c->stream>>>(BM(A)->dev, c->f32, c->piv,
+                                                                   (long)c->nblocks, c->dinfo);
+    }
+    else
+    {
+      k_getrf_mfma<64><<<(unsigned)c->nblocks, 64, 0, c->stream>>>(BM(A)->dev, c->f32, c->piv,
+                                                                   (long)c->nblocks, c->dinfo);
+    }
+  }
+  else if (c->precision == SUNAMD_LU_FP32)
   {
     ok = launch_getrf<float>((int)c->m, BM(A)->dev, c->f32, c->piv, (long)c->nblocks, c->dinfo,
                              c->stream);
@@ -801,7 +931,101 @@ extern "C" SUNLinearSolver SUNLinSol_Mi355BlockLU(N_Vector y, SUNMatrix A, int p
   return S;
 }
 
+extern "C" SUNErrCode SUNLinSol_Mi355BlockLUSetMFMA(SUNLinearSolver S, int onoff)
+{
+  if (!S || !S->content) { return SUN_ERR_ARG_CORRUPT; }
+  BlockLU* c = LU(S);
+  if (onoff && (c->precision != SUNAMD_LU_FP32 || (c->m != 32 && c->m != 64)))
+  {
+    return SUN_ERR_ARG_INCOMPATIBLE;
+  }
+  if (onoff && c->nblocks > 0x7fffffffL) { return SUN_ERR_ARG_OUTOFRANGE; }
+  c->mfma = onoff ? 1 : 0;
+  return SUN_SUCCESS;
+}
+
 extern "C" int* SUNLinSol_Mi355BlockLU_DevicePivots(SUNLinearSolver S) { return LU(S)->piv; }
+
+// ----------------------------------------------------------------------------
+// factorisation benchmark: nblocks blocks of m x m (m = 32 or 64), entries
+// U(-1, 1)/m plus the identity, generated on the device; average ms per
+// factorisation over reps launches after a warm-up (HIP events).  variant 0:
+// FP64 register kernel (into a separate factor array), 1: FP32 register
+// kernel, 2: FP32 MFMA blocked kernel.
+// ----------------------------------------------------------------------------
+__global__ void k_bench_fill(double* __restrict__ a, long nblocks, int m)
+{
+  const long n = nblocks * m * m;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+  {
+    unsigned long long z = (unsigned long long)i * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E5FULL;
+    z                    = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z                    = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    const double u   = ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    const long inb   = i % ((long)m * m);
+    const bool diag  = (inb / m) == (inb % m);
+    a[i]             = (2.0 * u - 1.0) / m + (diag ? 1.0 : 0.0);
+  }
+}
+
+extern "C" int SUNAMD_BenchBlockLU(long nblocks, int m, int variant, int reps, double* ms_out)
+{
+  if (nblocks < 1 || (m != 32 && m != 64) || variant < 0 || variant > 2 || reps < 1 ||
+      nblocks > 0x7fffffffL)
+  {
+    return -1;
+  }
+  double *A = nullptr, *F64 = nullptr;
+  float* F32              = nullptr;
+  int* piv                = nullptr;
+  unsigned long long* inf = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const size_t nel = (size_t)nblocks * m * m;
+  hipError_t e     = hipMalloc((void**)&A, sizeof(double) * nel);
+  if (e == hipSuccess && variant == 0) { e = hipMalloc((void**)&F64, sizeof(double) * nel); }
+  if (e == hipSuccess && variant > 0) { e = hipMalloc((void**)&F32, sizeof(float) * nel); }
+  if (e == hipSuccess) { e = hipMalloc((void**)&piv, sizeof(int) * (size_t)nblocks * m); }
+  if (e == hipSuccess) { e = hipMalloc((void**)&inf, sizeof(unsigned long long)); }
+  if (e == hipSuccess) { e = hipEventCreate(&e0); }
+  if (e == hipSuccess) { e = hipEventCreate(&e1); }
+  if (e == hipSuccess)
+  {
+    k_bench_fill<<<4096, 256>>>(A, nblocks, m);
+    auto run = [&]() {
+      if (variant == 2)
+      {
+        if (m == 32) { k_getrf_mfma<32><<<(unsigned)nblocks, 64>>>(A, F32, piv, nblocks, inf); }
+        else { k_getrf_mfma<64><<<(unsigned)nblocks, 64>>>(A, F32, piv, nblocks, inf); }
+      }
+      else if (variant == 1) { (void)launch_getrf<float>(m, A, F32, piv, nblocks, inf, 0); }
+      else { (void)launch_getrf<double>(m, A, F64, piv, nblocks, inf, 0); }
+    };
+    run();
+    e = hipDeviceSynchronize();
+    if (e == hipSuccess) { e = hipEventRecord(e0, 0); }
+    for (int r = 0; r < reps && e == hipSuccess; r++) { run(); }
+    if (e == hipSuccess) { e = hipEventRecord(e1, 0); }
+    if (e == hipSuccess) { e = hipEventSynchronize(e1); }
+    float ms = 0.0f;
+    if (e == hipSuccess) { e = hipEventElapsedTime(&ms, e0, e1); }
+    if (e == hipSuccess) { e = hipGetLastError(); }
+    *ms_out = (double)ms / reps;
+  }
+  if (e0) { (void)hipEventDestroy(e0); }
+  if (e1) { (void)hipEventDestroy(e1); }
+  (void)hipFree(A);
+  (void)hipFree(F64);
+  (void)hipFree(F32);
+  (void)hipFree(piv);
+  (void)hipFree(inf);
+  if (e != hipSuccess)
+  {
+    fprintf(stderr, "[sundials_amd blockdense] LU benchmark: %s\n", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
 
 extern "C" SUNErrCode SUNLinSol_Mi355BlockLU_GetPivots(SUNLinearSolver S, int* host_out)
 {

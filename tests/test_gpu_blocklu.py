@@ -138,6 +138,59 @@ def test_fp32_solve_residual(libs, ctx, m):
         core.N_VDestroy(v)
 
 
+@pytest.mark.parametrize("m", [32, 64])
+def test_fp32_mfma_factorisation(libs, ctx, m):
+    """The blocked FP32 factorisation with MFMA trailing updates
+    (SUNLinSol_Mi355BlockLUSetMFMA) against the register-row FP32 kernel on
+    the same matrices (forced pivoting, no near-ties): the same pivot rows,
+    factors that agree to single precision, and solutions at the same
+    single-precision level; a zero pivot is reported the same way."""
+    core, hip, orc = libs
+    nb = 97
+    rng = np.random.default_rng(100 + m)
+    A = _blocks(rng, nb, m, ties=False)
+    n = nb * m
+    xt = rng.uniform(-1, 1, n)
+    b = np.einsum("kij,kj->ki", A, xt.reshape(nb, m)).ravel()
+    y = gpu_vec(ctx, np.zeros(n))
+    M = _gpu_matrix(hip, ctx, y, A)
+    out = {}
+    for mf in (0, 1):
+        LS = hip.SUNLinSol_Mi355BlockLU(y, M, 1, ctx)
+        assert hip.SUNLinSol_Mi355BlockLUSetMFMA(LS, mf) == 0
+        assert core.SUNLinSolSetup(LS, M) == 0
+        piv = np.zeros(n, dtype=np.int32)
+        assert hip.SUNLinSol_Mi355BlockLU_GetPivots(LS, piv.ctypes.data_as(C.POINTER(C.c_int))) == 0
+        gb, gx = gpu_vec(ctx, b), gpu_vec(ctx, np.zeros(n))
+        assert core.SUNLinSolSolve(LS, M, gx, gb, 0.0) == 0
+        out[mf] = (piv, gpu_get(gx))
+        core.SUNLinSolFree(LS)
+        for v in (gb, gx):
+            core.N_VDestroy(v)
+    assert np.array_equal(out[0][0], out[1][0])
+    err_reg = np.abs(out[0][1] - xt).max()
+    err_mfma = np.abs(out[1][1] - xt).max()
+    print("m=%d: max |x - x_true| register %.3g, MFMA %.3g" % (m, err_reg, err_mfma))
+    assert err_mfma < 2e-4 * m
+    assert np.abs(out[1][1] - out[0][1]).max() < 2e-4 * m
+    # the switch applies to FP32 solvers of m = 32 or 64 only
+    LS64 = hip.SUNLinSol_Mi355BlockLU(y, M, 0, ctx)
+    assert hip.SUNLinSol_Mi355BlockLUSetMFMA(LS64, 1) != 0
+    core.SUNLinSolFree(LS64)
+    # zero pivot: the first failing global column, as the register kernel reports it
+    A[5, :, 3] = 0.0
+    A[9, :, 1] = 0.0
+    M2 = _gpu_matrix(hip, ctx, y, A)
+    LS = hip.SUNLinSol_Mi355BlockLU(y, M2, 1, ctx)
+    assert hip.SUNLinSol_Mi355BlockLUSetMFMA(LS, 1) == 0
+    assert core.SUNLinSolSetup(LS, M2) == SUNLS_LUFACT_FAIL
+    assert core.SUNLinSolLastFlag(LS) == 5 * m + 3 + 1
+    core.SUNLinSolFree(LS)
+    core.SUNMatDestroy(M2)
+    core.SUNMatDestroy(M)
+    core.N_VDestroy(y)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_zero_pivot_reported(libs, ctx, prec):
     core, hip, orc = libs
