@@ -274,14 +274,25 @@ def bench_coupled(args):
                 "avg_launch_us": round(dk["avg_us"], 2)}
 
     # the same integration with the integrator's fused kernels (error weights,
-    # residual, Nordsieck predictor / undo): bit-identical results, fewer passes
+    # residual, Nordsieck predictor / undo, and the Newton-matrix setup as one
+    # pass: copy + ScaleAddI + factorisation): bit-identical results, fewer passes
     fused = None
+    y_plain = np.zeros(ncells * m)
+    hip.SUNAMD_CoupledGetState_Mi355(run, y_plain.ctypes.data_as(C.POINTER(C.c_double)))
+    plain_counts = (st.nst, st.nni, st.nsetups, st.nje)
     if hip.SUNAMD_CoupledSetFusedKernels_Mi355(run, 1) == 0:
         rc = hip.SUNAMD_CoupledRun_Mi355(run, TOUT, C.byref(st))
         if rc == 0:
+            y_fused = np.zeros(ncells * m)
+            hip.SUNAMD_CoupledGetState_Mi355(run, y_fused.ctypes.data_as(C.POINTER(C.c_double)))
             fused = {"ms": round(1e3 * st.wall_s, 3), "nst": st.nst,
-                     "value": ncells * st.nst / st.wall_s}
+                     "value": ncells * st.nst / st.wall_s,
+                     "bitwise_equal_to_unfused": bool(
+                         np.array_equal(y_plain, y_fused)
+                         and plain_counts == (st.nst, st.nni, st.nsetups, st.nje))}
+            del y_fused
         hip.SUNAMD_CoupledSetFusedKernels_Mi355(run, 0)
+    del y_plain
 
     micro = None if args.no_micro else vector_microbench(args, hip)
 

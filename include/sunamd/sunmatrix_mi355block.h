@@ -58,6 +58,20 @@ SUNErrCode SUNLinSol_Mi355BlockLUSetMFMA(SUNLinearSolver S, int onoff);
 int* SUNLinSol_Mi355BlockLU_DevicePivots(SUNLinearSolver S);
 /* copy the pivots to a host array of nblocks*m ints (synchronous) */
 SUNErrCode SUNLinSol_Mi355BlockLU_GetPivots(SUNLinearSolver S, int* host_out);
+/* The Newton-matrix setup in one pass over the blocks, in place of the three
+   calls of cvode_ls.c:1311-1395 (cvLsLinSys + cvLsSetup):
+     save_to_J == 0:  SUNMatCopy(J, A); SUNMatScaleAddI(c, A); SUNLinSolSetup(S, A)
+     save_to_J != 0:  SUNMatCopy(A, J); SUNMatScaleAddI(c, A); SUNLinSolSetup(S, A)
+   with the same per-element roundings, so the factors, pivots and what A
+   holds afterwards equal the three calls' bit for bit.  Returns what
+   SUNLinSolSetup returns (0 or SUNLS_LUFACT_FAIL), or SUN_ERR_ARG_INCOMPATIBLE
+   without doing anything when the three calls must run instead: S not a
+   block LU of A's shape, J not a block matrix of that shape or J == A, a
+   matrix on another stream, m outside 1-10, 12, 16, 24, 32, 48, 64, or the
+   MFMA factorisation switched on.  No counterpart in the reference, whose
+   matrix and solver ops are separate calls. */
+int SUNLinSol_Mi355BlockLUSetupScaled(SUNLinearSolver S, SUNMatrix A, SUNMatrix J, sunrealtype c,
+                                      int save_to_J);
 
 #ifdef __cplusplus
 }

@@ -218,6 +218,9 @@ typedef struct CVodeMemRec
   int (*fused_resid)(sunrealtype, sunrealtype, N_Vector, N_Vector, N_Vector, N_Vector);
   int (*fused_predict)(int, N_Vector*); /* Nordsieck predictor / its undo in one pass */
   int (*fused_restore)(int, N_Vector*);
+  /* copy + ScaleAddI + factorisation of the Newton matrix in one pass (NULL:
+     the library has none); declines with SUN_ERR_ARG_INCOMPATIBLE */
+  int (*fused_lsetup)(SUNLinearSolver, SUNMatrix, SUNMatrix, sunrealtype, int);
 
   sunbooleantype MallocDone;
 } CVodeMemRec;
@@ -570,6 +573,8 @@ int CVodeSetUseIntegratorFusedKernels(void* cvode_mem, sunbooleantype onoff)
                               N_Vector))sv;
   cv->fused_resid  = (int (*)(sunrealtype, sunrealtype, N_Vector, N_Vector, N_Vector,
                               N_Vector))rs;
+  cv->fused_lsetup = (int (*)(SUNLinearSolver, SUNMatrix, SUNMatrix, sunrealtype,
+                              int))dlsym(RTLD_DEFAULT, "SUNLinSol_Mi355BlockLUSetupScaled");
   cv->usefused     = SUNTRUE;
   return CV_SUCCESS;
 }
@@ -658,13 +663,28 @@ static int cvLsInitialize(CVodeMem cv)
 static int cvLsLinSys(CVodeMem cv, sunrealtype t, N_Vector y, N_Vector fy,
                       SUNMatrix A, sunbooleantype jok, sunbooleantype* jcur,
                       sunrealtype gamma, N_Vector vtemp1, N_Vector vtemp2,
-                      N_Vector vtemp3)
+                      N_Vector vtemp3, sunbooleantype* fused)
 {
   LsState* ls = &cv->ls;
   int retval;
+  /* with the fused kernels on, a solver that offers it forms and factors
+     I - gamma*J in one pass from the saved (jok) or the fresh Jacobian;
+     *fused receives SUNTRUE and the setup's flag lands in ls->last_flag */
+  const sunbooleantype try_fused = cv->usefused && cv->fused_lsetup && fused;
+  if (fused) { *fused = SUNFALSE; }
   if (jok)
   {
-    *jcur  = SUNFALSE;
+    *jcur = SUNFALSE;
+    if (try_fused)
+    {
+      retval = cv->fused_lsetup(ls->LS, A, ls->savedJ, -gamma, 0);
+      if (retval != SUN_ERR_ARG_INCOMPATIBLE)
+      {
+        *fused        = SUNTRUE;
+        ls->last_flag = retval;
+        return CVLS_SUCCESS;
+      }
+    }
     retval = SUNMatCopy(ls->savedJ, A);
     if (retval)
     {
@@ -694,6 +714,16 @@ static int cvLsLinSys(CVodeMem cv, sunrealtype t, N_Vector y, N_Vector fy,
     {
       ls->last_flag = CVLS_JACFUNC_RECVR;
       return 1;
+    }
+    if (try_fused)
+    {
+      retval = cv->fused_lsetup(ls->LS, A, ls->savedJ, -gamma, 1);
+      if (retval != SUN_ERR_ARG_INCOMPATIBLE)
+      {
+        *fused        = SUNTRUE;
+        ls->last_flag = retval;
+        return CVLS_SUCCESS;
+      }
     }
     retval = SUNMatCopy(A, ls->savedJ);
     if (retval)
@@ -726,10 +756,11 @@ static int cvLsSetup(CVodeMem cv, int convfail, N_Vector ypred, N_Vector fpred,
   ls->jbad = (cv->nst == 0) || (cv->nst >= ls->nstlj + ls->msbj) ||
              ((convfail == CV_FAIL_BAD_J) && (dgamma < ls->dgmax_jbad)) ||
              (convfail == CV_FAIL_OTHER);
+  sunbooleantype fused = SUNFALSE;
   if (ls->A != NULL)
   {
     retval = cvLsLinSys(cv, cv->tn, ypred, fpred, ls->A, !(ls->jbad), jcurPtr,
-                        cv->gamma, vtemp1, vtemp2, vtemp3);
+                        cv->gamma, vtemp1, vtemp2, vtemp3, &fused);
     if (*jcurPtr)
     {
       ls->nje++;
@@ -739,7 +770,7 @@ static int cvLsSetup(CVodeMem cv, int convfail, N_Vector ypred, N_Vector fpred,
     if (retval != CVLS_SUCCESS) { return retval; }
   }
   else { *jcurPtr = ls->jbad; }
-  ls->last_flag = SUNLinSolSetup(ls->LS, ls->A);
+  if (!fused) { ls->last_flag = SUNLinSolSetup(ls->LS, ls->A); }
   return ls->last_flag;
 }
 

@@ -35,7 +35,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
+#include "../core/lane_ops.h"
 #include "../core/profiler_internal.h"
 #include "sunamd/nvector_mi355.h"
 #include "sunamd/sunamd_profiler.h"
@@ -141,28 +143,72 @@ __device__ __forceinline__ T to_t(double v)
   return (T)v;
 }
 
+// How k_getrf forms the matrix it factors from its source blocks A:
+//   SRC_PLAIN  the blocks themselves;
+//   SRC_SCALED c*A + I (SUNMatScaleAddI's per-element a*c, then +1 on the
+//              diagonal), A read-only -- the Newton matrix from a saved
+//              Jacobian without the copy and the scaling pass;
+//   SRC_SAVE   as SRC_SCALED, and A's blocks are also written to `save` first
+//              (a fresh Jacobian kept for later setups).
+// With SRC_SCALED / SRC_SAVE and FP32 factors, `scaled` receives c*A + I in
+// FP64 -- the matrix the separate copy / scale / factor calls leave in the
+// solver's matrix; with FP64 factors the factors themselves go there.
+enum GetrfSource
+{
+  SRC_PLAIN  = 0,
+  SRC_SCALED = 1,
+  SRC_SAVE   = 2
+};
+
+// wavefronts per workgroup of the register-row factorisation (each wavefront
+// factors its own blocks; workgroups of several wavefronts put more blocks in
+// flight per CU than one-wavefront workgroups)
+#define LU_WAVES 4
+
 // Factor nblocks m x m blocks in place (T = double) or into an FP32 factor
 // array (T = float).  info receives min over failing blocks of
-// (block*m + k + 1) for the first zero pivot k.
-template <int M, typename T>
-__global__ void __launch_bounds__(64) k_getrf(const double* A, T* F /* may alias A */,
-                                              int* __restrict__ piv, long nblocks,
-                                              unsigned long long* info)
+// (block*m + k + 1) for the first zero pivot k.  Lane r of a block's group of
+// G = pow2ceil(m) lanes holds row r; the row swaps exchange position tags
+// only, and the rows are written to their final positions at the end.
+template <int M, typename T, int SRC = SRC_PLAIN>
+__global__ void __launch_bounds__(64 * LU_WAVES) k_getrf(const double* A, T* F /* may alias A */,
+                                                         int* __restrict__ piv, long nblocks,
+                                                         unsigned long long* info, double c = 0.0,
+                                                         double* save = nullptr,
+                                                         double* scaled = nullptr)
 {
+  using namespace sunamd_lane;
   constexpr int G   = pow2ceil(M);
   constexpr int BPW = 64 / G;
-  __shared__ T prow[BPW][M];
+  __shared__ T prow[LU_WAVES * BPW][M];
 
-  const int lane   = threadIdx.x;
-  const int grp    = lane / G;
+  const int lane   = threadIdx.x & 63;
+  const int slot   = (threadIdx.x >> 6) * BPW + lane / G;
   const int r      = lane % G;
-  const long blk   = (long)blockIdx.x * BPW + grp;
+  const long blk   = (long)blockIdx.x * (LU_WAVES * BPW) + slot;
   const bool valid = (blk < nblocks) && (r < M);
 
   T a[M];
   const double* Ab = A + blk * (long)M * M;
+  if (SRC == SRC_PLAIN)
+  {
 #pragma unroll
-  for (int j = 0; j < M; j++) { a[j] = valid ? to_t<T>(Ab[j * M + r]) : (T)0; }
+    for (int j = 0; j < M; j++) { a[j] = valid ? to_t<T>(Ab[j * M + r]) : (T)0; }
+  }
+  else
+  {
+    const long o0 = blk * (long)M * M + r;
+#pragma unroll
+    for (int j = 0; j < M; j++)
+    {
+      double v = valid ? Ab[j * M + r] : 0.0;
+      if (SRC == SRC_SAVE && valid) { save[o0 + (long)j * M] = v; }
+      v = v * c;
+      if (j == r) { v = v + 1.0; }
+      if (!std::is_same<T, double>::value && valid) { scaled[o0 + (long)j * M] = v; }
+      a[j] = valid ? to_t<T>(v) : (T)0;
+    }
+  }
 
   int pos  = r; // physical row currently held by this lane
   int myp  = 0; // pivot row chosen at step k == r
@@ -174,17 +220,7 @@ __global__ void __launch_bounds__(64) k_getrf(const double* A, T* F /* may alias
     // arg-max of |a_rk| over rows at positions >= k; ties -> lowest position
     T v    = (valid && pos >= k) ? (T)fabs((double)a[k]) : (T)-1;
     int bp = valid ? pos : INT_MAX;
-#pragma unroll
-    for (int off = G / 2; off >= 1; off >>= 1)
-    {
-      T ov   = __shfl_xor(v, off, 64);
-      int op = __shfl_xor(bp, off, 64);
-      if (ov > v || (ov == v && op < bp))
-      {
-        v  = ov;
-        bp = op;
-      }
-    }
+    g_argmax<G>(v, bp);
     if (r == k) { myp = bp; }
     if (v == (T)0 && fail == 0) { fail = k + 1; }
 
@@ -192,26 +228,51 @@ __global__ void __launch_bounds__(64) k_getrf(const double* A, T* F /* may alias
     if (pos == k) { pos = bp; }
     else if (pos == bp) { pos = k; }
 
-    // the pivot row (now at position k) publishes columns k..M-1
-    if (valid && pos == k)
+    const bool up = valid && pos > k && fail == 0;
+    if constexpr (G == 64)
     {
-#pragma unroll
-      for (int j = k; j < M; j++) { prow[grp][j] = a[j]; }
-    }
-    __syncthreads();
-    if (valid && pos > k && fail == 0)
-    {
-      const T mult = (T)1 / prow[grp][k];
-      const T l    = a[k] * mult;
-      a[k]         = l;
+      // one block per wavefront: the pivot row's lane is wavefront-uniform,
+      // its entries are read straight from that lane's registers
+      const unsigned long long holder = __ballot(valid && pos == k);
+      const int pl = __builtin_amdgcn_readfirstlane(__ffsll((long long)holder) - 1) & 63;
+      const T pk   = readlane_d(a[k], pl);
+      T l          = (T)0;
+      if (up)
+      {
+        const T mult = (T)1 / pk;
+        l            = a[k] * mult;
+        a[k]         = l;
+      }
 #pragma unroll
       for (int j = k + 1; j < M; j++)
       {
-        const T pj = prow[grp][j];
-        if (pj != (T)0) { a[j] = a[j] - pj * l; }
+        const T pj = readlane_d(a[j], pl);
+        if (up && pj != (T)0) { a[j] = a[j] - pj * l; }
       }
     }
-    __syncthreads();
+    else
+    {
+      // the pivot row (now at position k) publishes columns k..M-1
+      wave_sync();
+      if (valid && pos == k)
+      {
+#pragma unroll
+        for (int j = k; j < M; j++) { prow[slot][j] = a[j]; }
+      }
+      wave_sync();
+      if (up)
+      {
+        const T mult = (T)1 / prow[slot][k];
+        const T l    = a[k] * mult;
+        a[k]         = l;
+#pragma unroll
+        for (int j = k + 1; j < M; j++)
+        {
+          const T pj = prow[slot][j];
+          if (pj != (T)0) { a[j] = a[j] - pj * l; }
+        }
+      }
+    }
   }
 
   if (valid)
@@ -510,9 +571,9 @@ static bool launch_getrf(int m, const double* A, T* F, int* piv, long nblocks,
 #define GETRF_CASE(MM)                                                                     \
   case MM:                                                                                 \
   {                                                                                        \
-    constexpr int BPW = 64 / pow2ceil(MM);                                                 \
-    long g            = (nblocks + BPW - 1) / BPW;                                         \
-    k_getrf<MM, T><<<(unsigned)g, 64, 0, s>>>(A, F, piv, nblocks, info);                   \
+    constexpr int BPG = LU_WAVES * (64 / pow2ceil(MM));                                    \
+    long g            = (nblocks + BPG - 1) / BPG;                                         \
+    k_getrf<MM, T><<<(unsigned)g, 64 * LU_WAVES, 0, s>>>(A, F, piv, nblocks, info);        \
     return true;                                                                           \
   }
     LU_SIZES(GETRF_CASE)
@@ -522,6 +583,40 @@ static bool launch_getrf(int m, const double* A, T* F, int* piv, long nblocks,
   if (m > 64) { return false; }
   k_getrf_lds<T><<<(unsigned)nblocks, 64, sizeof(T) * m * m, s>>>(A, F, piv, nblocks, m, info);
   return true;
+}
+
+// The register-resident sizes factor c*src + I in one pass (SRC_SCALED, or
+// SRC_SAVE with src also copied to save); false for other sizes, which take
+// the separate copy / scale / factor calls.
+template <typename T>
+static bool launch_getrf_scaled(int m, bool save_src, const double* src, double c, double* save,
+                                double* scaled, T* F, int* piv, long nblocks,
+                                unsigned long long* info, hipStream_t s)
+{
+  switch (m)
+  {
+#define GETRF_SC_CASE(MM)                                                                  \
+  case MM:                                                                                 \
+  {                                                                                        \
+    constexpr int BPG = LU_WAVES * (64 / pow2ceil(MM));                                    \
+    long g            = (nblocks + BPG - 1) / BPG;                                         \
+    if (save_src)                                                                          \
+    {                                                                                      \
+      k_getrf<MM, T, SRC_SAVE><<<(unsigned)g, 64 * LU_WAVES, 0, s>>>(                      \
+        src, F, piv, nblocks, info, c, save, scaled);                                      \
+    }                                                                                      \
+    else                                                                                   \
+    {                                                                                      \
+      k_getrf<MM, T, SRC_SCALED><<<(unsigned)g, 64 * LU_WAVES, 0, s>>>(                    \
+        src, F, piv, nblocks, info, c, save, scaled);                                      \
+    }                                                                                      \
+    return true;                                                                           \
+  }
+    LU_SIZES(GETRF_SC_CASE)
+#undef GETRF_SC_CASE
+  default: break;
+  }
+  return false;
 }
 
 template <typename T>
@@ -756,6 +851,8 @@ static SUNErrCode lu_initialize(SUNLinearSolver S)
   return SUN_SUCCESS;
 }
 
+static int lu_finish(SUNLinearSolver S);
+
 static int lu_setup(SUNLinearSolver S, SUNMatrix A)
 {
   BlockLU* c = LU(S);
@@ -792,6 +889,13 @@ static int lu_setup(SUNLinearSolver S, SUNMatrix A)
   }
   sunamd_prof_end(SUNAMD_K_GETRF, c->stream);
   if (!ok) { return SUN_ERR_ARG_OUTOFRANGE; }
+  return lu_finish(S);
+}
+
+// the end of a setup: fetch the factorisation's zero-pivot report
+static int lu_finish(SUNLinearSolver S)
+{
+  BlockLU* c   = LU(S);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess)
   {
@@ -945,6 +1049,47 @@ extern "C" SUNErrCode SUNLinSol_Mi355BlockLUSetMFMA(SUNLinearSolver S, int onoff
 }
 
 extern "C" int* SUNLinSol_Mi355BlockLU_DevicePivots(SUNLinearSolver S) { return LU(S)->piv; }
+
+// One pass instead of three: SUNMatCopy + SUNMatScaleAddI(c) + SUNLinSolSetup
+// (cvode_ls.c:1311-1395 issues them one after the other).  Declines with
+// SUN_ERR_ARG_INCOMPATIBLE, before touching anything, where the separate
+// calls must run instead.
+extern "C" int SUNLinSol_Mi355BlockLUSetupScaled(SUNLinearSolver S, SUNMatrix A, SUNMatrix J,
+                                                 sunrealtype c, int save_to_J)
+{
+  if (!S || !S->content || !A || !J || A == J || S->ops->setup != lu_setup ||
+      A->ops->getid != bm_getid || J->ops->getid != bm_getid)
+  {
+    return SUN_ERR_ARG_INCOMPATIBLE;
+  }
+  BlockLU* lc = LU(S);
+  if (!compatible(A, J) || BM(A)->nblocks != lc->nblocks || BM(A)->m != lc->m || lc->mfma ||
+      BM(A)->stream != lc->stream || BM(J)->stream != lc->stream)
+  {
+    return SUN_ERR_ARG_INCOMPATIBLE;
+  }
+  const int m = (int)lc->m;
+  if (m > 10 && m != 12 && m != 16 && m != 24 && m != 32 && m != 48 && m != 64)
+  {
+    return SUN_ERR_ARG_INCOMPATIBLE;
+  }
+  const double* src = save_to_J ? BM(A)->dev : BM(J)->dev;
+  double* save      = save_to_J ? BM(J)->dev : nullptr;
+  k_info_reset<<<1, 1, 0, lc->stream>>>(lc->dinfo);
+  sunamd_prof_begin(SUNAMD_K_GETRF, lc->stream);
+  if (lc->precision == SUNAMD_LU_FP32)
+  {
+    launch_getrf_scaled<float>(m, save_to_J != 0, src, c, save, BM(A)->dev, lc->f32, lc->piv,
+                               (long)lc->nblocks, lc->dinfo, lc->stream);
+  }
+  else
+  {
+    launch_getrf_scaled<double>(m, save_to_J != 0, src, c, save, nullptr, BM(A)->dev, lc->piv,
+                                (long)lc->nblocks, lc->dinfo, lc->stream);
+  }
+  sunamd_prof_end(SUNAMD_K_GETRF, lc->stream);
+  return lu_finish(S);
+}
 
 // ----------------------------------------------------------------------------
 // factorisation benchmark: nblocks blocks of m x m (m = 32 or 64), entries

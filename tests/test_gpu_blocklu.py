@@ -112,6 +112,97 @@ def test_fp64_factor_and_solve_bitwise(libs, ctx, m, nb):
         core.N_VDestroy(v)
 
 
+def _setup_run(core, hip, ctx, J, prec, save_to_J, fused, c):
+    """The Newton-matrix setup from J: fused (one call) or as the three
+    separate calls; returns (flag, what A holds, pivots, solution of one
+    right-hand side, what the saved-Jacobian matrix holds)."""
+    nb, m, _ = J.shape
+    n = nb * m
+    y = gpu_vec(ctx, np.zeros(n))
+    # save_to_J: A holds the fresh Jacobian; else the saved matrix holds it
+    MA = _gpu_matrix(hip, ctx, y, J if save_to_J else np.zeros_like(J))
+    MJ = _gpu_matrix(hip, ctx, y, np.zeros_like(J) if save_to_J else J)
+    LS = hip.SUNLinSol_Mi355BlockLU(y, MA, prec, ctx)
+    assert core.SUNLinSolInitialize(LS) == 0
+    if fused:
+        flag = hip.SUNLinSol_Mi355BlockLUSetupScaled(LS, MA, MJ, c, save_to_J)
+    else:
+        assert core.SUNMatCopy(MA, MJ) == 0 if save_to_J else core.SUNMatCopy(MJ, MA) == 0
+        assert core.SUNMatScaleAddI(c, MA) == 0
+        flag = core.SUNLinSolSetup(LS, MA)
+    a = _gpu_matrix_values(hip, MA, nb, m)
+    j = _gpu_matrix_values(hip, MJ, nb, m)
+    piv = np.zeros(n, np.int32)
+    assert hip.SUNLinSol_Mi355BlockLU_GetPivots(LS, piv.ctypes.data_as(C.POINTER(C.c_int))) == 0
+    x = None
+    if flag == 0:
+        b = np.random.default_rng(5).uniform(-1, 1, n)
+        gb, gx = gpu_vec(ctx, b), gpu_vec(ctx, np.zeros(n))
+        assert core.SUNLinSolSolve(LS, MA, gx, gb, 0.0) == 0
+        x = gpu_get(gx)
+        for v in (gb, gx):
+            core.N_VDestroy(v)
+    core.SUNLinSolFree(LS)
+    for M in (MA, MJ):
+        core.SUNMatDestroy(M)
+    core.N_VDestroy(y)
+    return flag, a, piv, x, j
+
+
+@pytest.mark.parametrize("m", [1, 3, 4, 8, 10, 16, 24, 32, 48, 64])
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("save_to_J", [0, 1])
+def test_fused_newton_matrix_setup(libs, ctx, m, prec, save_to_J):
+    """SUNLinSol_Mi355BlockLUSetupScaled (copy + ScaleAddI + factorisation in
+    one pass) leaves what the three separate calls leave, bit for bit: the
+    solver matrix (FP64: the factors; FP32: c J + I), the saved Jacobian,
+    the pivots and the solution of a right-hand side; in FP64 that is also
+    the oracle's GETRF of c J + I block by block."""
+    core, hip, orc = libs
+    nb = 67
+    rng = np.random.default_rng(17 * m + 3 * prec + save_to_J)
+    J = rng.uniform(-1, 1, (nb, m, m)) * 40.0 / m
+    c = -0.0371
+    ref = _setup_run(core, hip, ctx, J, prec, save_to_J, False, c)
+    got = _setup_run(core, hip, ctx, J, prec, save_to_J, True, c)
+    assert ref[0] == 0 and got[0] == 0
+    for k, (r, g) in enumerate(zip(ref[1:], got[1:])):
+        assert np.array_equal(r, g), ("A", "pivots", "solution", "saved J")[k]
+    assert np.array_equal(got[4], _to_blockmajor(J))
+    if prec == 0:
+        for k in range(0, nb, 11):
+            a = _to_blockmajor(J[k:k + 1]).copy() * c
+            a[::m + 1] += 1.0
+            p = np.zeros(m, np.int64)
+            assert orc.oracle_block_getrf(dp(a), m, p.ctypes.data_as(C.POINTER(C.c_int64))) == 0
+            assert np.array_equal(got[1][k * m * m:(k + 1) * m * m], a), k
+
+
+def test_fused_setup_declines(libs, ctx):
+    """Sizes without a register kernel, and J == A, decline before doing
+    anything (SUN_ERR_ARG_INCOMPATIBLE); a zero pivot reports like setup."""
+    core, hip, orc = libs
+    SUN_ERR_ARG_INCOMPATIBLE = -9998  # SUN_ERR_MINIMUM (-10000) + 2
+    for m in (11, 20):
+        J = np.ones((3, m, m))
+        y = gpu_vec(ctx, np.zeros(3 * m))
+        MA, MJ = _gpu_matrix(hip, ctx, y, J), _gpu_matrix(hip, ctx, y, 2 * J)
+        LS = hip.SUNLinSol_Mi355BlockLU(y, MA, 0, ctx)
+        assert hip.SUNLinSol_Mi355BlockLUSetupScaled(LS, MA, MJ, 1.0, 0) == SUN_ERR_ARG_INCOMPATIBLE
+        assert hip.SUNLinSol_Mi355BlockLUSetupScaled(LS, MA, MA, 1.0, 0) == SUN_ERR_ARG_INCOMPATIBLE
+        assert np.array_equal(_gpu_matrix_values(hip, MA, 3, m), _to_blockmajor(J))
+        core.SUNLinSolFree(LS)
+        for M in (MA, MJ):
+            core.SUNMatDestroy(M)
+        core.N_VDestroy(y)
+    # c J + I singular in block 1: c = -1, J = I there
+    m = 8
+    J = np.tile(np.eye(m) * 3.0, (4, 1, 1))
+    J[1] = np.eye(m)
+    flag, a, piv, x, j = _setup_run(core, hip, ctx, J, 0, 0, True, -1.0)
+    assert flag == SUNLS_LUFACT_FAIL
+
+
 @pytest.mark.parametrize("m", [3, 8, 16, 32, 48, 64])
 def test_fp32_solve_residual(libs, ctx, m):
     core, hip, orc = libs
