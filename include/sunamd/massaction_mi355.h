@@ -66,6 +66,27 @@ void SUNAMD_CoupledFree_Mi355(SUNAMD_CoupledRun* r);
 /* switch the integrator onto the fused error-weight / residual kernels
    (CVodeSetUseIntegratorFusedKernels); results are unchanged bit for bit */
 int SUNAMD_CoupledSetFusedKernels_Mi355(SUNAMD_CoupledRun* r, int onoff);
+/* Device stepping: SUNAMD_CoupledRun_Mi355 then integrates with one
+   cooperative launch in which every decision of the integrator -- Newton
+   convergence, Jacobian reuse, the error test, step size and order -- is
+   taken on the device, with no host synchronisation between steps (the
+   step / Newton / error-test logic of cvode.c cvStep, cvode_nls.c and
+   sunnonlinsol_newton.c restated per workgroup; each WRMS norm a grid-wide
+   reduction in a fixed order).  Per cell it streams the state through one
+   fused pass per Newton iteration and one per accepted step.  With
+   SUNAMD_REDUCE_SEQUENTIAL the norms are summed in element order and the
+   run equals the host-stepped one bit for bit (counters, step sizes, final
+   state); with SUNAMD_REDUCE_TREE the order is fixed by the launch shape.
+   The saved Jacobian is kept as the state it was taken at; the block
+   matrix's storage holds the factors and the solver's pivot array the row
+   permutations.  Supported for m in 1-8, 10, 12, 16, 24, 32 (other m:
+   SUNAMD_CoupledRun_Mi355 returns -100).  No counterpart in the reference,
+   whose integrators decide on the host after every norm. */
+int SUNAMD_CoupledSetDeviceStepping_Mi355(SUNAMD_CoupledRun* r, int onoff);
+/* launch shape of the last device-stepped run: workgroups of the
+   cooperative grid, threads per workgroup, dynamic LDS bytes per workgroup
+   (-1 before any run) */
+int SUNAMD_CoupledDeviceLastShape(int* workgroups, int* threads, int* lds_bytes);
 
 #ifdef __cplusplus
 }

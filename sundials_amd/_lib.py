@@ -382,6 +382,8 @@ def hip():
     _sig(lib, "SUNAMD_BenchVectorOps", C.c_int, [c_index, C.c_int, C.c_int, P(c_real)])
     _sig(lib, "SUNAMD_BenchBlockLU", C.c_int, [C.c_long, C.c_int, C.c_int, C.c_int, P(c_real)])
     _sig(lib, "SUNLinSol_Mi355BlockLUSetMFMA", C.c_int, [c_void_p, C.c_int])
+    _sig(lib, "SUNAMD_CoupledSetDeviceStepping_Mi355", C.c_int, [c_void_p, C.c_int])
+    _sig(lib, "SUNAMD_CoupledDeviceLastShape", C.c_int, [P(C.c_int), P(C.c_int), P(C.c_int)])
     _sig(lib, "SUNLinSol_Mi355BlockLUSetupScaled", C.c_int,
          [c_void_p, c_void_p, c_void_p, c_real, C.c_int])
     _sig(lib, "SUNAMD_ClassicalGS_Mi355", C.c_int, [P(c_void_p), P(P(c_real)), C.c_int, C.c_int,

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "../batched/coupled_device.h"
 #include "../core/profiler_internal.h"
 #include "sunamd/cvode_sunamd.h"
 #include "sunamd/sunamd_profiler.h"
@@ -259,7 +260,19 @@ struct SUNAMD_CoupledRun_
   SUNMatrix A;
   SUNLinearSolver LS;
   void* cv;
+  // the device-stepped integrator (coupled_device.hip) and what it needs
+  int devstep;
+  void* dstate;
+  double rtol, atol;
+  int lu_precision, reduction_order;
 };
+
+extern "C" int SUNAMD_CoupledSetDeviceStepping_Mi355(SUNAMD_CoupledRun* r, int onoff)
+{
+  if (!r) { return -1; }
+  r->devstep = onoff ? 1 : 0;
+  return 0;
+}
 
 extern "C" int SUNAMD_CoupledSetFusedKernels_Mi355(SUNAMD_CoupledRun* r, int onoff)
 {
@@ -270,6 +283,7 @@ extern "C" int SUNAMD_CoupledSetFusedKernels_Mi355(SUNAMD_CoupledRun* r, int ono
 extern "C" void SUNAMD_CoupledFree_Mi355(SUNAMD_CoupledRun* r)
 {
   if (!r) { return; }
+  if (r->dstate) { sunamd_coupled_device_free(r->dstate); }
   if (r->cv) { CVodeFree(&r->cv); }
   if (r->LS) { SUNLinSolFree(r->LS); }
   if (r->A) { SUNMatDestroy(r->A); }
@@ -289,6 +303,10 @@ extern "C" SUNAMD_CoupledRun* SUNAMD_CoupledCreate_Mi355(const SUNAMD_MassAction
   if (!r) { return NULL; }
   bool ok = SUNContext_Create(SUN_COMM_NULL, &r->ctx) == 0;
   r->N    = (sunindextype)p->ncells * p->m;
+  r->rtol = rtol;
+  r->atol = atol;
+  r->lu_precision    = lu_precision;
+  r->reduction_order = reduction_order;
   if (ok) { ok = (r->y = N_VNew_Mi355(r->N, r->ctx)) != NULL; }
   if (ok) { ok = N_VSetReductionOrder_Mi355(r->y, reduction_order) == 0; }
   if (ok) { ok = (r->y0 = N_VClone(r->y)) != NULL; }
@@ -317,9 +335,43 @@ extern "C" SUNAMD_CoupledRun* SUNAMD_CoupledCreate_Mi355(const SUNAMD_MassAction
   return r;
 }
 
+static int coupled_run_device(SUNAMD_CoupledRun* r, double tout, SUNAMD_SolveStats* st)
+{
+  if (hipDeviceSynchronize() != hipSuccess) { return -1; }
+  CoupledDeviceJob job;
+  job.m            = r->dev->m;
+  job.R            = r->dev->R;
+  job.ncells       = r->dev->ncells;
+  job.idx          = r->dev->idx;
+  job.kf_rm        = r->dev->kf;
+  job.kb_rm        = r->dev->kb;
+  job.y0           = N_VGetDeviceArrayPointer(r->y0);
+  job.yout         = N_VGetDeviceArrayPointer(r->y);
+  job.lu           = SUNMatrix_Mi355BlockDense_DeviceData(r->A);
+  job.piv          = SUNLinSol_Mi355BlockLU_DevicePivots(r->LS);
+  job.rtol         = r->rtol;
+  job.atol         = r->atol;
+  job.tout         = tout;
+  job.mxstep       = 1000000;
+  job.lu_precision = r->lu_precision;
+  job.sequential   = r->reduction_order == SUNAMD_REDUCE_SEQUENTIAL;
+  job.stream       = (hipStream_t)N_VGetStream_Mi355(r->y);
+  SUNAMD_SolveStats s;
+  memset(&s, 0, sizeof(s));
+  auto t0  = std::chrono::steady_clock::now();
+  int flag = sunamd_coupled_device_run(job, &r->dstate, &s);
+  if (hipDeviceSynchronize() != hipSuccess) { return -1; }
+  auto t1 = std::chrono::steady_clock::now();
+  s.wall_s = std::chrono::duration<double>(t1 - t0).count();
+  s.flag   = flag;
+  if (st) { *st = s; }
+  return (flag < 0) ? flag : 0;
+}
+
 extern "C" int SUNAMD_CoupledRun_Mi355(SUNAMD_CoupledRun* r, double tout, SUNAMD_SolveStats* st)
 {
   sunrealtype t = 0.0;
+  if (r->devstep) { return coupled_run_device(r, tout, st); }
   if (hipDeviceSynchronize() != hipSuccess) { return -1; }
   long l0 = N_VGetNumLaunches_Mi355(r->y);
   auto t0 = std::chrono::steady_clock::now();

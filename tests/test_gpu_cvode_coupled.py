@@ -224,3 +224,89 @@ def test_coupled_fused_kernels_change_nothing(libs, order, m, R, ncells):
         assert c1 == tuple(int(v) for v in ost[:8])
         assert h1 == (ost[10], ost[11], ost[13])
         assert np.array_equal(y1, y_cpu)
+
+
+def _run_coupled(hip, p, n, order, prec=0, fused=0, devstep=0, tout=1.0, runs=1):
+    r = hip.SUNAMD_CoupledCreate_Mi355(p, 1e-6, 1e-10, prec, order)
+    assert r
+    if fused:
+        assert hip.SUNAMD_CoupledSetFusedKernels_Mi355(r, 1) == 0
+    if devstep:
+        assert hip.SUNAMD_CoupledSetDeviceStepping_Mi355(r, 1) == 0
+    out = []
+    for _ in range(runs):
+        st = _lib.SolveStats()
+        rc = hip.SUNAMD_CoupledRun_Mi355(r, tout, C.byref(st))
+        y = np.zeros(n)
+        assert hip.SUNAMD_CoupledGetState_Mi355(r, dp(y)) == 0
+        out.append((rc, st, y))
+    hip.SUNAMD_CoupledFree_Mi355(r)
+    return out
+
+
+@pytest.mark.parametrize("m,R,ncells,seed", [(8, 16, 4096, 1001), (3, 4, 33, 5),
+                                              (32, 48, 128, 1002), (16, 24, 300, 9),
+                                              (24, 36, 100, 7)])
+def test_device_stepping_bitwise_sequential(libs, m, R, ncells, seed):
+    """Device stepping: the whole integration in one cooperative launch, with
+    Newton convergence, Jacobian reuse, the error test and the step-size and
+    order choices decided on the device.  With WRMS norms in the serial
+    order the run equals the host-stepped run and the CPU oracle bit for
+    bit -- every counter, the step sizes and the final state -- and a second
+    run of the same runner repeats it exactly.  (8 x 4096 spreads the cells
+    over 128 workgroups: the norms' partials are read across the grid.)"""
+    core, hip, orc = libs
+    p = _prob(core, m, R, ncells, seed)
+    n = ncells * m
+    (rh, sh, yh), = _run_coupled(hip, p, n, SEQ)
+    d1, d2 = _run_coupled(hip, p, n, SEQ, devstep=1, runs=2)
+    y_cpu, ost = np.zeros(n), np.zeros(16)
+    assert orc.oracle_massaction_solve_coupled(p, 1e-6, 1e-10, 1.0, dp(y_cpu), dp(ost)) == 0
+    core.SUNAMD_MassActionFree(p)
+    assert rh == 0 and sh.flag == 0
+    for rd, sd, yd in (d1, d2):
+        assert rd == 0 and sd.flag == 0
+        assert _counts(sd) == _counts(sh) == tuple(int(v) for v in ost[:8])
+        assert (sd.qlast, sd.qcur) == (sh.qlast, sh.qcur)
+        assert (sd.hinused, sd.hlast, sd.hcur, sd.tcur) == (sh.hinused, sh.hlast, sh.hcur,
+                                                            sh.tcur)
+        assert np.array_equal(yd, yh)
+        assert np.array_equal(yd, y_cpu)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("m,R,ncells,seed", [(8, 16, 1024, 1001), (32, 48, 128, 1002)])
+def test_device_stepping_tree_accuracy(libs, prec, m, R, ncells, seed):
+    """Device stepping with tree-ordered norms (the order is fixed by the
+    launch shape, so runs repeat exactly) and FP64 or FP32 factors: the error
+    against a reference solution at 1000x tighter tolerances is within twice
+    the oracle's own error (+0.1), the bar of the host-stepped tree test."""
+    core, hip, orc = libs
+    p = _prob(core, m, R, ncells, seed)
+    n = ncells * m
+    d1, d2 = _run_coupled(hip, p, n, TREE, prec=prec, devstep=1, runs=2)
+    y_cpu, ost = np.zeros(n), np.zeros(16)
+    assert orc.oracle_massaction_solve_coupled(p, 1e-6, 1e-10, 1.0, dp(y_cpu), dp(ost)) == 0
+    y_ref = np.zeros(n)
+    assert orc.oracle_massaction_solve_coupled(p, 1e-9, 1e-13, 1.0, dp(y_ref), None) == 0
+    core.SUNAMD_MassActionFree(p)
+    (r1, s1, y1), (r2, s2, y2) = d1, d2
+    assert r1 == 0 and r2 == 0
+    assert _counts(s1) == _counts(s2) and np.array_equal(y1, y2)
+    assert abs(s1.nst - ost[0]) <= 0.1 * ost[0]
+    e_dev, e_cpu = _wrms_err(y1, y_ref), _wrms_err(y_cpu, y_ref)
+    assert e_dev <= 2.0 * e_cpu + 0.1, (e_dev, e_cpu)
+
+
+def test_device_stepping_unsupported_size(libs):
+    """Groups of 64 lanes (m = 48, 64) are not offered by the device-stepped
+    integrator: the run returns -100 without launching anything."""
+    core, hip, _ = libs
+    p = _prob(core, 64, 96, 7, 3)
+    r = hip.SUNAMD_CoupledCreate_Mi355(p, 1e-6, 1e-10, 0, TREE)
+    assert r
+    assert hip.SUNAMD_CoupledSetDeviceStepping_Mi355(r, 1) == 0
+    st = _lib.SolveStats()
+    assert hip.SUNAMD_CoupledRun_Mi355(r, 1.0, C.byref(st)) == -100
+    hip.SUNAMD_CoupledFree_Mi355(r)
+    core.SUNAMD_MassActionFree(p)

@@ -11,10 +11,11 @@ depend on size.
   (first, last and spread), and the one-pass Newton-matrix setup equal to the
   three separate calls over the whole 8 GiB.
 - Coupled BDF on 2^20 cells x 32 species (the coupled benchmark's shape),
-  tiled from 128 base cells: cells interact only through the norms, so every
-  tile ends bit-identical to every other, and the step count and the error
-  against a 1000x tighter reference solution of the base cells stay where
-  the oracle's own run of the base cells puts them.
+  tiled from 128 base cells, host-stepped and device-stepped: cells interact
+  only through the norms, so every tile ends bit-identical to every other,
+  and the step count and the error against a 1000x tighter reference
+  solution of the base cells stay where the oracle's own run of the base
+  cells puts them.
 """
 import ctypes as C
 import sys
@@ -173,3 +174,20 @@ def test_coupled_2p20x32_tiled(libs):
     err = lambda v: np.sqrt(np.mean(((v - y_ref) / (1e-6 * np.abs(y_ref) + 1e-10)) ** 2))
     e_gpu, e_cpu = err(tiles[0]), err(y_cpu)
     assert e_cpu <= 20.0 and e_gpu <= 20.0, (e_gpu, e_cpu)
+    # the same with every integrator decision on the device (one launch)
+    _say("coupled: the same, device-stepped")
+    base_again = core.SUNAMD_MassActionCreate(m, R, nbase, 1002, 0.2)
+    big = core.SUNAMD_MassActionTile(base_again, ncells)
+    r = hip.SUNAMD_CoupledCreate_Mi355(big, 1e-6, 1e-10, 0, 0)
+    assert r and hip.SUNAMD_CoupledSetDeviceStepping_Mi355(r, 1) == 0
+    sd = _lib.SolveStats()
+    assert hip.SUNAMD_CoupledRun_Mi355(r, 1.0, C.byref(sd)) == 0
+    assert hip.SUNAMD_CoupledGetState_Mi355(r, dp(y)) == 0
+    hip.SUNAMD_CoupledFree_Mi355(r)
+    core.SUNAMD_MassActionFree(big)
+    core.SUNAMD_MassActionFree(base_again)
+    _say("coupled device-stepped: nst=%d nni=%d" % (sd.nst, sd.nni))
+    tiles = y.reshape(-1, nbase * m)
+    assert all(np.array_equal(tiles[0], t) for t in tiles[1:])
+    assert abs(sd.nst - ost[0]) <= 0.1 * ost[0]
+    assert err(tiles[0]) <= 20.0, err(tiles[0])
