@@ -294,6 +294,35 @@ def bench_coupled(args):
         hip.SUNAMD_CoupledSetFusedKernels_Mi355(run, 0)
     del y_plain
 
+    # the same integration device-stepped (one cooperative launch, every
+    # integrator decision on the device), tree-ordered norms; and with FP32
+    # factors, device-stepped and host-stepped
+    device = None
+    if hip.SUNAMD_CoupledSetDeviceStepping_Mi355(run, 1) == 0:
+        rc = hip.SUNAMD_CoupledRun_Mi355(run, TOUT, C.byref(st))
+        if rc == 0:
+            shp = [C.c_int(), C.c_int(), C.c_int()]
+            hip.SUNAMD_CoupledDeviceLastShape(C.byref(shp[0]), C.byref(shp[1]), C.byref(shp[2]))
+            device = {"fp64": {"ms": round(1e3 * st.wall_s, 3), "nst": st.nst, "nni": st.nni,
+                               "value": ncells * st.nst / st.wall_s},
+                      "launch": {"workgroups": shp[0].value, "threads": shp[1].value,
+                                 "lds_bytes": shp[2].value}}
+        hip.SUNAMD_CoupledSetDeviceStepping_Mi355(run, 0)
+    if args.lu == "fp64":
+        run32 = hip.SUNAMD_CoupledCreate_Mi355(prob, RTOL, ATOL, 1, 0)
+        if run32:
+            hip.SUNAMD_CoupledSetFusedKernels_Mi355(run32, 1)
+            if hip.SUNAMD_CoupledRun_Mi355(run32, TOUT, C.byref(st)) == 0 and device is not None:
+                device["fp32_lu_host_stepped_fused"] = {
+                    "ms": round(1e3 * st.wall_s, 3), "nst": st.nst,
+                    "value": ncells * st.nst / st.wall_s}
+            hip.SUNAMD_CoupledSetFusedKernels_Mi355(run32, 0)
+            if hip.SUNAMD_CoupledSetDeviceStepping_Mi355(run32, 1) == 0 and device is not None:
+                if hip.SUNAMD_CoupledRun_Mi355(run32, TOUT, C.byref(st)) == 0:
+                    device["fp32_lu"] = {"ms": round(1e3 * st.wall_s, 3), "nst": st.nst,
+                                         "value": ncells * st.nst / st.wall_s}
+            hip.SUNAMD_CoupledFree_Mi355(run32)
+
     micro = None if args.no_micro else vector_microbench(args, hip)
 
     hip.SUNAMD_CoupledFree_Mi355(run)
@@ -325,6 +354,7 @@ def bench_coupled(args):
         "cpu_baseline": cpu,
         "kernels": kernels,
         "fused_kernels": fused,
+        "device_stepping": device,
         "microbench": micro,
     }
     return line

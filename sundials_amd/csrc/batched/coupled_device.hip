@@ -383,6 +383,57 @@ __device__ __forceinline__ void add_term(const CplArgs& A, double (&acc)[3], int
   else { acc[k] = acc[k] + t; }
 }
 
+// reactions per lane whose rate constants a pass keeps in registers (larger
+// networks stage every rate constant of the cell through LDS)
+constexpr int RPL = 4;
+
+// The right-hand side with each lane's reactions' rate constants in
+// registers (reaction r = g.i + t G in kr[t], br[t]): the same operations in
+// the same order as g_rhs, which reads them from the group's LDS slot
+template <int M, typename T>
+__device__ __forceinline__ double rhs_regs(const CplArgs& A, const Grp<M, T>& g, double yi,
+                                           const double (&kr)[RPL], const double (&br)[RPL])
+{
+  constexpr int G = pow2ceil(M);
+  gsync();
+  if (g.valid) { g.red[g.i] = yi; }
+  gsync();
+#pragma unroll
+  for (int t = 0; t < RPL; t++)
+  {
+    const int r = g.i + t * G;
+    if (r < A.R)
+    {
+      const int4 q    = *reinterpret_cast<const int4*>(g.idx + 4 * r);
+      const double wf = (kr[t] * g.red[q.x]) * g.red[q.y];
+      const double wb = (br[t] * g.red[q.z]) * g.red[q.w];
+      g.wbuf[r]       = wf - wb;
+    }
+  }
+  gsync();
+  double f = 0.0;
+  if (g.valid)
+  {
+    const int e1 = g.soff[g.i + 1];
+    for (int e = g.soff[g.i]; e < e1; e += 4)
+    {
+      int code[4];
+      double w[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) { code[q] = (e + q < e1) ? g.sent[e + q] : 0; }
+#pragma unroll
+      for (int q = 0; q < 4; q++) { w[q] = g.wbuf[code[q] >> 2]; }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+      {
+        if (e + q < e1) { f = ((code[q] & 3) < 2) ? f - w[q] : f + w[q]; }
+      }
+    }
+  }
+  gsync();
+  return f;
+}
+
 // ---- pass: history updates, then one Newton iteration ----------------------
 // terms: 0 = ||delta||, 1 = ||acor||; maximum: a failed factorisation (1)
 template <int M, typename T>
@@ -392,15 +443,47 @@ __device__ __forceinline__ void pass_newton(const CplArgs& A, const Grp<M, T>& g
   const long N        = A.N;
   const bool zn_ops   = (d.scale1 != 0.0) || d.restore_q || d.save_acor || d.inc_q || d.dec_q ||
                       d.reset1 || d.rescale_q || d.predict_q;
+  constexpr int G  = pow2ceil(M);
+  const bool regs  = A.R <= RPL * G;
   for (long ch = ch0; ch < ch1; ch++)
   {
     const Cell<M, T> c = cell_at(A, g, ch);
-    stage_rates(A, g, c.cc);
-    // the Newton iterations after the first read zn0 and zn1 only
+    // every global operand of the cell is requested before the first wait:
+    // the lane's rate constants, the history columns (zn0 and zn1 only in
+    // the iterations after the first), the correction, the weights and, when
+    // no setup is due, the saved factors
+    double kr[RPL], br[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; t++)
+    {
+      const int r = g.i + t * G;
+      kr[t]       = (regs && r < A.R) ? A.kf[c.cc * A.R + r] : 0.0;
+      br[t]       = (regs && r < A.R) ? A.kb[c.cc * A.R + r] : 0.0;
+    }
     double z[QMAX + 1];
 #pragma unroll
     for (int j = 0; j <= QMAX; j++) { z[j] = (c.lv && (j <= 1 || zn_ops)) ? A.zn[(long)j * N + c.e] : 0.0; }
     double acor = (d.acor_zero && !d.save_acor) ? 0.0 : (c.lv ? A.acor[c.e] : 0.0);
+    const double w = c.lv ? A.ewt[c.e] : 0.0;
+    if (!d.setup) { lu_load(A, g, S, c); }
+    // the Newton matrix and the order-1 restart's RHS read any reaction's
+    // rates: the cell's rates go to the group's LDS slot
+    if (!regs) { stage_rates(A, g, c.cc); }
+    else if (d.setup || d.reset1)
+    {
+      gsync();
+#pragma unroll
+      for (int t = 0; t < RPL; t++)
+      {
+        const int r = g.i + t * G;
+        if (r < A.R)
+        {
+          g.rates[r]       = kr[t];
+          g.rates[A.R + r] = br[t];
+        }
+      }
+      gsync();
+    }
     if (zn_ops)
     {
       if (d.scale1 != 0.0) { z[1] = d.scale1 * z[1]; }
@@ -489,15 +572,13 @@ __device__ __forceinline__ void pass_newton(const CplArgs& A, const Grp<M, T>& g
       if (c.cv && fail) { mx = 1.0; }
       lu_store(A, g, S, c);
     }
-    else { lu_load(A, g, S, c); }
-    const double f = g_rhs(g, y);
+    const double f = regs ? rhs_regs(A, g, y, kr, br) : g_rhs(g, y);
     const double res = lsum(d.rl1, z[1], 1.0, acor);
     double delta     = lsum(-d.gamma, f, 1.0, res);
     delta            = -1.0 * delta;
     delta            = g_getrs(g, S, delta);
     if (d.gscale != 1.0) { delta = d.gscale * delta; }
     acor = lsum(1.0, acor, 1.0, delta);
-    const double w = c.lv ? A.ewt[c.e] : 0.0;
     if (c.lv) { A.acor[c.e] = acor; }
     add_term(A, acc, 0, c.e, c.lv, delta, w, gen);
     add_term(A, acc, 1, c.e, c.lv, acor, w, gen);
