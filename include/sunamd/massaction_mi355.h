@@ -79,7 +79,7 @@ int SUNAMD_CoupledSetFusedKernels_Mi355(SUNAMD_CoupledRun* r, int onoff);
    state); with SUNAMD_REDUCE_TREE the order is fixed by the launch shape.
    The saved Jacobian is kept as the state it was taken at; the block
    matrix's storage holds the factors and the solver's pivot array the row
-   permutations.  Supported for m in 1-8, 10, 12, 16, 24, 32 (other m:
+   permutations.  Supported for m in 1-8, 10, 12, 16, 24, 32, 48, 64 (other m:
    SUNAMD_CoupledRun_Mi355 returns -100).  No counterpart in the reference,
    whose integrators decide on the host after every norm. */
 int SUNAMD_CoupledSetDeviceStepping_Mi355(SUNAMD_CoupledRun* r, int onoff);

@@ -1608,9 +1608,7 @@ int run_sized(const CoupledDeviceJob& job, CplState* st, SUNAMD_SolveStats* out)
 
 } // namespace
 
-// sizes with a kernel; groups of 64 lanes (m = 48, 64) are not offered: a run
-// at m = 64 did not finish and is not understood yet
-#define CPL_SIZES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(16) X(24) X(32)
+#define CPL_SIZES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(16) X(24) X(32) X(48) X(64)
 
 int sunamd_coupled_device_run(const CoupledDeviceJob& job, void** state, SUNAMD_SolveStats* out)
 {

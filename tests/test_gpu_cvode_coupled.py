@@ -246,7 +246,8 @@ def _run_coupled(hip, p, n, order, prec=0, fused=0, devstep=0, tout=1.0, runs=1)
 
 @pytest.mark.parametrize("m,R,ncells,seed", [(8, 16, 4096, 1001), (3, 4, 33, 5),
                                               (32, 48, 128, 1002), (16, 24, 300, 9),
-                                              (24, 36, 100, 7)])
+                                              (24, 36, 100, 7), (64, 96, 7, 3),
+                                              (48, 72, 50, 4)])
 def test_device_stepping_bitwise_sequential(libs, m, R, ncells, seed):
     """Device stepping: the whole integration in one cooperative launch, with
     Newton convergence, Jacobian reuse, the error test and the step-size and
@@ -299,10 +300,10 @@ def test_device_stepping_tree_accuracy(libs, prec, m, R, ncells, seed):
 
 
 def test_device_stepping_unsupported_size(libs):
-    """Groups of 64 lanes (m = 48, 64) are not offered by the device-stepped
-    integrator: the run returns -100 without launching anything."""
+    """Sizes without a register-LU kernel (here m = 9) are not offered by the
+    device-stepped integrator: the run returns -100 without launching."""
     core, hip, _ = libs
-    p = _prob(core, 64, 96, 7, 3)
+    p = _prob(core, 9, 14, 7, 3)
     r = hip.SUNAMD_CoupledCreate_Mi355(p, 1e-6, 1e-10, 0, TREE)
     assert r
     assert hip.SUNAMD_CoupledSetDeviceStepping_Mi355(r, 1) == 0
