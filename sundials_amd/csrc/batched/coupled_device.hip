@@ -52,6 +52,36 @@ __host__ __device__ constexpr int cpl_waves()
 }
 constexpr int CW_MAX = 4;
 
+// LDS of a wavefront.  With FP64 rows, groups of 32 lanes (two per
+// wavefront) share one Newton-matrix rows area, used only by setup passes,
+// and build their matrices one after the other: the 8.4 KiB a group's rows
+// take would otherwise leave room for three workgroups per CU instead of
+// four.  Every group has its own rate constants and reaction-rate buffer.
+// Other cases keep one rows area per group (which the reaction rates alias);
+// FP32 rows are half the size and fit four workgroups either way (sharing
+// measured slower there).  In doubles:
+template <int M, typename T>
+__host__ __device__ constexpr bool cpl_shared_rows()
+{
+  return pow2ceil(M) == 32 && sizeof(T) == 8;
+}
+template <int M, typename T>
+__host__ __device__ inline long cpl_rows_doubles(int R)
+{
+  return ((long)M * (M + 1) * (long)sizeof(T) + 7) / 8;
+}
+template <int M, typename T>
+__host__ __device__ inline long cpl_group_doubles(int R)
+{
+  return cpl_shared_rows<M, T>() ? 3L * R : 2L * R + group_rows_stride(M, R, (int)sizeof(T));
+}
+template <int M, typename T>
+__host__ __device__ inline long cpl_wave_doubles(int R)
+{
+  constexpr int GPW = 64 / pow2ceil(M);
+  return GPW * cpl_group_doubles<M, T>(R) + (cpl_shared_rows<M, T>() ? cpl_rows_doubles<M, T>(R) : 0);
+}
+
 // results written by workgroup 0 at the end
 enum Out
 {
@@ -434,6 +464,57 @@ __device__ __forceinline__ double rhs_regs(const CplArgs& A, const Grp<M, T>& g,
   return f;
 }
 
+// The Newton matrix rows (I - gamma J)_i of each group, with the groups of a
+// wavefront taking turns in the shared rows area: the additions of
+// g_newton_matrix in the same order, so the rows are the same bits
+template <int M, typename T>
+__device__ __forceinline__ void newton_matrix_turns(const Grp<M, T>& g, double yj, double gamma,
+                                                    T (&a)[M])
+{
+  constexpr int GPW = 64 / pow2ceil(M);
+  gsync();
+  if (g.valid) { g.red[g.i] = yj; }
+  gsync();
+  const double c = -gamma;
+  for (int turn = 0; turn < GPW; turn++)
+  {
+    T* row = g.jrow;
+    if (g.row == turn && g.valid)
+    {
+#pragma unroll
+      for (int j = 0; j < M; j++) { row[j] = (T)0; }
+      const int e1 = g.soff[g.i + 1];
+      for (int e = g.soff[g.i]; e < e1; e++)
+      {
+        const int code = g.sent[e];
+        const int r = code >> 2, t = code & 3;
+        int ix[4]    = {g.idx[4 * r], g.idx[4 * r + 1], g.idx[4 * r + 2], g.idx[4 * r + 3]};
+        double kfv   = g.kf[r], kbv = g.kb[r];
+        double dw[4] = {kfv * g.red[ix[1]], kfv * g.red[ix[0]], -(kbv * g.red[ix[3]]),
+                        -(kbv * g.red[ix[2]])};
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+        {
+          const T v  = (T)dw[q];
+          row[ix[q]] = (t < 2) ? (row[ix[q]] - v) : (row[ix[q]] + v);
+        }
+      }
+    }
+    gsync();
+    if (g.row == turn)
+    {
+#pragma unroll
+      for (int j = 0; j < M; j++)
+      {
+        double v = g.valid ? (double)row[j] * c : 0.0;
+        if (j == g.i) { v = v + 1.0; }
+        a[j] = (T)v;
+      }
+    }
+    gsync();
+  }
+}
+
 // ---- pass: history updates, then one Newton iteration ----------------------
 // terms: 0 = ||delta||, 1 = ||acor||; maximum: a failed factorisation (1)
 template <int M, typename T>
@@ -567,7 +648,8 @@ __device__ __forceinline__ void pass_newton(const CplArgs& A, const Grp<M, T>& g
         if (c.lv) { A.ysav[c.e] = ys; }
       }
       else { ys = c.lv ? A.ysav[c.e] : 0.0; }
-      g_newton_matrix(g, ys, d.gamma, S.lu);
+      if constexpr (cpl_shared_rows<M, T>()) { newton_matrix_turns(g, ys, d.gamma, S.lu); }
+      else { g_newton_matrix(g, ys, d.gamma, S.lu); }
       const int fail = g_getrf(g, S);
       if (c.cv && fail) { mx = 1.0; }
       lu_store(A, g, S, c);
@@ -677,7 +759,6 @@ __global__ void __launch_bounds__(64 * cpl_waves<M>(), 2) k_coupled_bdf(const Cp
   __shared__ PassD s_pd;
   __shared__ DoneD s_dd;
   extern __shared__ __align__(16) double s_dyn[];
-  const long JS  = group_rows_stride(M, A.R, (int)sizeof(T));
   int* s_idx     = (int*)s_dyn;
   int* s_sent    = s_idx + 4 * A.R;
   int* s_soff    = s_sent + 4 * A.R;
@@ -708,12 +789,18 @@ __global__ void __launch_bounds__(64 * cpl_waves<M>(), 2) k_coupled_bdf(const Cp
   g.idx   = s_idx;
   g.soff  = s_soff;
   g.sent  = s_sent;
-  double* area = s_grp + (long)(wave * GPW + grp) * (2 * A.R + JS);
+  double* wave_area = s_grp + (long)wave * cpl_wave_doubles<M, T>(A.R);
+  double* area      = wave_area + (long)grp * cpl_group_doubles<M, T>(A.R);
   g.rates = area;
   g.kf    = area;
   g.kb    = area + A.R;
   g.wbuf  = area + 2 * A.R;
-  g.jrow  = (T*)(area + 2 * A.R) + (long)(g.valid ? g.i : 0) * (M + 1);
+  if constexpr (cpl_shared_rows<M, T>())
+  {
+    g.jrow = (T*)(wave_area + (long)GPW * cpl_group_doubles<M, T>(A.R)) +
+             (long)(g.valid ? g.i : 0) * (M + 1);
+  }
+  else { g.jrow = (T*)(area + 2 * A.R) + (long)(g.valid ? g.i : 0) * (M + 1); }
   g.cell  = 0;
   Sys<M, T> S;
   S.pos  = g.i;
@@ -1506,9 +1593,8 @@ int run_sized(const CoupledDeviceJob& job, CplState* st, SUNAMD_SolveStats* out)
   constexpr int CW  = cpl_waves<M>();
   constexpr int CT  = 64 * CW;
   const int R       = job.R;
-  const long JS     = group_rows_stride(M, R, (int)sizeof(T));
   const size_t shm  = sizeof(int) * (size_t)(8 * R + ((M + 2) & ~1)) +
-                     sizeof(double) * (size_t)CW * GPW * (2 * R + JS);
+                     sizeof(double) * (size_t)CW * cpl_wave_doubles<M, T>(R);
   if (shm > 65536) { return -100; }
   auto kern = k_coupled_bdf<M, T>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
