@@ -19,7 +19,10 @@ extern "C" {
 
 typedef struct SUNAMD_MassActionDev_ SUNAMD_MassActionDev;
 
-/* copy the problem to HBM; kernels run on `stream` (a hipStream_t) */
+/* copy the problem to HBM; kernels run on `stream` (a hipStream_t).  NULL for
+ * m outside 1..64, a species index outside 0..m-1, or a network whose
+ * per-cell Jacobian work area (m*m + m + 2R doubles, plus 8R + m + 2 ints of
+ * tables) exceeds 64 KiB of LDS (R up to ~640 at m = 64) */
 SUNAMD_MassActionDev* SUNAMD_MassActionToDevice(const SUNAMD_MassAction* p, void* stream);
 void SUNAMD_MassActionDevFree(SUNAMD_MassActionDev* d);
 

@@ -16,8 +16,8 @@ struct CoupledDeviceJob
   int m, R;
   long ncells;
   const int* idx;      // 4R reaction indices
-  const double* kf_rm; // rate constants, reaction-major (r * ncells + cell)
-  const double* kb_rm;
+  const double* kf;    // rate constants, cell-major (cell * R + r)
+  const double* kb;
   const double* y0;    // ncells * m
   double* yout;        // ncells * m
   void* lu;            // ncells * m * m entries of the factor precision (scratch)

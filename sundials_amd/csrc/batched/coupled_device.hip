@@ -1542,7 +1542,6 @@ struct CplState
   const double* kf_src; // the problem the buffers were built for
   int* soff;
   int* sent;
-  double *kf, *kb;
   double *zn, *ewt, *acor, *ysav, *part, *seq, *out;
   float* rinv;
   unsigned* bar;
@@ -1561,22 +1560,10 @@ int hip_err(hipError_t e, const char* what)
   return -100;
 }
 
-__global__ void k_to_cell_major(const double* __restrict__ src, double* __restrict__ dst, int R,
-                                long ncells)
-{
-  const long n = (long)R * ncells;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
-  {
-    const long cell = t / R;
-    const int r     = (int)(t - cell * R);
-    dst[t]          = src[(long)r * ncells + cell];
-  }
-}
-
 void free_state(CplState* st)
 {
   if (!st) { return; }
-  void* ps[] = {st->soff, st->sent, st->kf, st->kb, st->zn, st->ewt, st->acor, st->ysav,
+  void* ps[] = {st->soff, st->sent, st->zn, st->ewt, st->acor, st->ysav,
                 st->part, st->seq, st->out, st->rinv, st->bar, st->abort, st->args};
   for (void* p : ps)
   {
@@ -1626,8 +1613,8 @@ int run_sized(const CoupledDeviceJob& job, CplState* st, SUNAMD_SolveStats* out)
   A.idx        = job.idx;
   A.soff       = st->soff;
   A.sent       = st->sent;
-  A.kf         = st->kf;
-  A.kb         = st->kb;
+  A.kf         = job.kf;
+  A.kb         = job.kb;
   A.y0         = job.y0;
   A.zn         = st->zn;
   A.ewt        = st->ewt;
@@ -1706,7 +1693,7 @@ int sunamd_coupled_device_run(const CoupledDeviceJob& job, void** state, SUNAMD_
   if (!sized || job.ncells < 1 || job.R < 1) { return -100; }
   const long N  = job.ncells * (long)m;
   CplState* st  = (CplState*)*state;
-  if (st && (st->m != m || st->R != job.R || st->ncells != job.ncells || st->kf_src != job.kf_rm))
+  if (st && (st->m != m || st->R != job.R || st->ncells != job.ncells || st->kf_src != job.kf))
   {
     free_state(st);
     st     = nullptr;
@@ -1720,11 +1707,8 @@ int sunamd_coupled_device_run(const CoupledDeviceJob& job, void** state, SUNAMD_
     st->m      = m;
     st->R      = job.R;
     st->ncells = job.ncells;
-    st->kf_src = job.kf_rm;
-    const size_t nr = (size_t)job.R * (size_t)job.ncells;
-    e = hipMalloc((void**)&st->kf, sizeof(double) * nr);
-    if (e == hipSuccess) { e = hipMalloc((void**)&st->kb, sizeof(double) * nr); }
-    if (e == hipSuccess) { e = hipMalloc((void**)&st->zn, sizeof(double) * (QMAX + 1) * (size_t)N); }
+    st->kf_src = job.kf;
+    e          = hipMalloc((void**)&st->zn, sizeof(double) * (QMAX + 1) * (size_t)N);
     if (e == hipSuccess) { e = hipMalloc((void**)&st->ewt, sizeof(double) * (size_t)N); }
     if (e == hipSuccess) { e = hipMalloc((void**)&st->acor, sizeof(double) * (size_t)N); }
     if (e == hipSuccess) { e = hipMalloc((void**)&st->ysav, sizeof(double) * (size_t)N); }
@@ -1784,12 +1768,6 @@ int sunamd_coupled_device_run(const CoupledDeviceJob& job, void** state, SUNAMD_
     free(off);
     free(ent);
     free(fill);
-    if (e == hipSuccess)
-    {
-      k_to_cell_major<<<1024, 256, 0, job.stream>>>(job.kf_rm, st->kf, job.R, job.ncells);
-      k_to_cell_major<<<1024, 256, 0, job.stream>>>(job.kb_rm, st->kb, job.R, job.ncells);
-      e = hipGetLastError();
-    }
     if (e == hipSuccess) { e = hipStreamSynchronize(job.stream); }
     if (e != hipSuccess)
     {

@@ -3,14 +3,17 @@
 // coupled-system driver (every cell in one N_Vector, block-diagonal Jacobian,
 // BDF + modified Newton + batched block LU).
 //
-// RHS: one lane per cell; the workgroup stages its cells' species through LDS
-// with coalesced loads (row stride m+1 to spread LDS banks), accumulates f in
-// LDS in the CPU's reaction order, and writes f back coalesced.  Rate
-// constants are stored reaction-major (r*ncells + cell) so each reaction's
-// rates are one coalesced load across the workgroup.
-// Jacobian: one group of pow2ceil(m) lanes per cell, lane = row; the block is
-// assembled in LDS (each lane only touches its own row, in the CPU's
-// reaction / target / column order) and streamed out column by column.
+// RHS and Jacobian: one group of G = pow2ceil(m) lanes per cell (lane =
+// species, or Jacobian row), 64 / G cells per wavefront, one-wavefront
+// workgroups striding over the cells.  The RHS evaluates each reaction's net
+// rate once (reactions spread over the group's lanes, into LDS) and each
+// species then adds its terms in reaction / role order -- the order in which
+// the serial evaluation updates f_i -- so f is bit-identical to it.  The
+// Jacobian row of each lane is assembled in LDS from that lane's terms only,
+// in the serial evaluation's reaction / target / column order, and the block
+// is streamed out column by column.  Rate constants are stored cell-major
+// (cell * R + r), so a cell's rates are one coalesced read by its group; the
+// species, rates and results move as whole rows of the cell.
 // -----------------------------------------------------------------------------
 #include <hip/hip_runtime.h>
 
@@ -32,7 +35,9 @@ struct SUNAMD_MassActionDev_
   int m, R;
   long ncells;
   int* idx;   // 4R
-  double* kf; // R*ncells, reaction-major
+  int* soff;  // m+1: species s's terms are sent[soff[s] .. soff[s+1])
+  int* sent;  // 4R term codes 4r + role (role 0, 1: reactants; 2, 3: products)
+  double* kf; // ncells*R, cell-major
   double* kb;
   hipStream_t stream;
 };
@@ -44,106 +49,156 @@ static int hipfail(hipError_t e, const char* what)
   return -1;
 }
 
-// cells per workgroup (one lane per cell): 64, or 32 above 32 species so the
-// two LDS tiles stay within 64 KiB
-static inline int rhs_cpb(int m) { return m <= 32 ? 64 : 32; }
+// LDS of one workgroup: per group its species (m doubles), then the
+// reaction table, the term offsets and the term codes; doubles counted in
+// pairs so the int4 reads of the table are 16-byte aligned
+static inline size_t ma_table_ints(int m, int R) { return (size_t)(8 * R + ((m + 2) & ~1)); }
 
+__device__ __forceinline__ void ma_stage_tables(const int* __restrict__ idx,
+                                                const int* __restrict__ soff,
+                                                const int* __restrict__ sent, int* s_idx,
+                                                int* s_soff, int* s_sent, int m, int R)
+{
+  for (int t = threadIdx.x; t < 4 * R; t += blockDim.x)
+  {
+    s_idx[t]  = idx[t];
+    s_sent[t] = sent[t];
+  }
+  for (int t = threadIdx.x; t <= m; t += blockDim.x) { s_soff[t] = soff[t]; }
+}
+
+// f = RHS(y): per group, y (m) and the net rates w (R) in LDS
 __global__ void __launch_bounds__(64) k_ma_rhs(const double* __restrict__ y,
                                                double* __restrict__ f,
                                                const int* __restrict__ idx,
-                                               const double* __restrict__ kf,
-                                               const double* __restrict__ kb, int m, int R,
-                                               long ncells)
-{
-  extern __shared__ double sm[];
-  const int cpb    = blockDim.x;
-  const int ld     = m + 1;
-  double* ys       = sm;
-  double* fs       = sm + cpb * ld;
-  const long cell0 = (long)blockIdx.x * cpb;
-  const long rem   = ncells - cell0;
-  const int ncb    = rem < cpb ? (int)rem : cpb;
-  const double* yb = y + cell0 * m;
-  for (int e = threadIdx.x; e < ncb * m; e += cpb)
-  {
-    int c                  = e / m;
-    ys[c * ld + e - c * m] = yb[e];
-  }
-  __syncthreads();
-  const int c = threadIdx.x;
-  if (c < ncb)
-  {
-    const double* yc = ys + c * ld;
-    double* fc       = fs + c * ld;
-    for (int s = 0; s < m; s++) { fc[s] = 0.0; }
-    const long cell = cell0 + c;
-    for (int r = 0; r < R; r++)
-    {
-      const int a = idx[4 * r], b = idx[4 * r + 1], cc = idx[4 * r + 2], d = idx[4 * r + 3];
-      double wf = (kf[(long)r * ncells + cell] * yc[a]) * yc[b];
-      double wb = (kb[(long)r * ncells + cell] * yc[cc]) * yc[d];
-      double w  = wf - wb;
-      fc[a]     = fc[a] - w;
-      fc[b]     = fc[b] - w;
-      fc[cc]    = fc[cc] + w;
-      fc[d]     = fc[d] + w;
-    }
-  }
-  __syncthreads();
-  double* fb = f + cell0 * m;
-  for (int e = threadIdx.x; e < ncb * m; e += cpb)
-  {
-    int cc = e / m;
-    fb[e]  = fs[cc * ld + e - cc * m];
-  }
-}
-
-__global__ void __launch_bounds__(64) k_ma_jac(const double* __restrict__ y,
-                                               double* __restrict__ J,
-                                               const int* __restrict__ idx,
+                                               const int* __restrict__ soff,
+                                               const int* __restrict__ sent,
                                                const double* __restrict__ kf,
                                                const double* __restrict__ kb, int m, int G,
                                                int R, long ncells)
 {
-  extern __shared__ double sj[];
-  const int lane   = threadIdx.x;
-  const int grp    = lane / G;
-  const int i      = lane % G;
-  const long cell  = (long)blockIdx.x * (64 / G) + grp;
-  const bool live  = cell < ncells;
-  const bool valid = live && i < m;
-  double* Jl       = sj + (long)grp * m * m;
-  if (live)
-  {
-    for (int e = i; e < m * m; e += G) { Jl[e] = 0.0; }
-  }
+  extern __shared__ __align__(16) double sm[];
+  const int gpw   = 64 / G;
+  const long per  = (long)((m + R + 1) & ~1);
+  const int grp   = threadIdx.x / G;
+  const int i     = threadIdx.x % G;
+  double* ys      = sm + grp * per;
+  double* ws      = ys + m;
+  int* s_idx      = (int*)(sm + gpw * per);
+  int* s_sent     = s_idx + 4 * R;
+  int* s_soff     = s_sent + 4 * R;
+  ma_stage_tables(idx, soff, sent, s_idx, s_soff, s_sent, m, R);
   __syncthreads();
-  if (valid)
+  const int e0    = (i < m) ? s_soff[i] : 0;
+  const int e1    = (i < m) ? s_soff[i + 1] : 0;
+  const int4* ix4 = (const int4*)s_idx;
+  // the loop bound is the workgroup's, so every lane meets every barrier
+  for (long c0 = (long)blockIdx.x * gpw; c0 < ncells; c0 += (long)gridDim.x * gpw)
   {
-    const double* yc = y + cell * m;
-    for (int r = 0; r < R; r++)
+    const long cell = c0 + grp;
+    const bool live = cell < ncells;
+    if (live && i < m) { ys[i] = y[cell * m + i]; }
+    __syncthreads();
+    if (live)
     {
-      int ix[4] = {idx[4 * r], idx[4 * r + 1], idx[4 * r + 2], idx[4 * r + 3]};
-      if (ix[0] != i && ix[1] != i && ix[2] != i && ix[3] != i) { continue; }
-      double kfv   = kf[(long)r * ncells + cell];
-      double kbv   = kb[(long)r * ncells + cell];
-      double dw[4] = {kfv * yc[ix[1]], kfv * yc[ix[0]], -(kbv * yc[ix[3]]), -(kbv * yc[ix[2]])};
-      for (int t = 0; t < 4; t++)
+      const double* kfc = kf + cell * R;
+      const double* kbc = kb + cell * R;
+      for (int r = i; r < R; r += G)
       {
-        if (ix[t] != i) { continue; }
+        const int4 ix   = ix4[r];
+        const double wf = (kfc[r] * ys[ix.x]) * ys[ix.y];
+        const double wb = (kbc[r] * ys[ix.z]) * ys[ix.w];
+        ws[r]           = wf - wb;
+      }
+    }
+    __syncthreads();
+    if (live && i < m)
+    {
+      double acc = 0.0;
+      for (int e = e0; e < e1; e++)
+      {
+        const int code = s_sent[e];
+        const double w = ws[code >> 2];
+        acc            = ((code & 3) < 2) ? acc - w : acc + w;
+      }
+      f[cell * m + i] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// J = df/dy, block column-major: per group, the block (m*m), y (m) and the
+// cell's rates (2R) in LDS
+__global__ void __launch_bounds__(64) k_ma_jac(const double* __restrict__ y,
+                                               double* __restrict__ J,
+                                               const int* __restrict__ idx,
+                                               const int* __restrict__ soff,
+                                               const int* __restrict__ sent,
+                                               const double* __restrict__ kf,
+                                               const double* __restrict__ kb, int m, int G,
+                                               int R, long ncells)
+{
+  extern __shared__ __align__(16) double sj[];
+  const int gpw   = 64 / G;
+  const int mm    = m * m;
+  const long per  = (long)((mm + m + 2 * R + 1) & ~1);
+  const int grp   = threadIdx.x / G;
+  const int i     = threadIdx.x % G;
+  double* Jl      = sj + grp * per;
+  double* ys      = Jl + mm;
+  double* rk      = ys + m;
+  int* s_idx      = (int*)(sj + gpw * per);
+  int* s_sent     = s_idx + 4 * R;
+  int* s_soff     = s_sent + 4 * R;
+  ma_stage_tables(idx, soff, sent, s_idx, s_soff, s_sent, m, R);
+  __syncthreads();
+  const int e0    = (i < m) ? s_soff[i] : 0;
+  const int e1    = (i < m) ? s_soff[i + 1] : 0;
+  const int4* ix4 = (const int4*)s_idx;
+  for (long c0 = (long)blockIdx.x * gpw; c0 < ncells; c0 += (long)gridDim.x * gpw)
+  {
+    const long cell = c0 + grp;
+    const bool live = cell < ncells;
+    if (live)
+    {
+      for (int e = i; e < mm; e += G) { Jl[e] = 0.0; }
+      if (i < m) { ys[i] = y[cell * m + i]; }
+      for (int r = i; r < R; r += G)
+      {
+        rk[r]     = kf[cell * R + r];
+        rk[R + r] = kb[cell * R + r];
+      }
+    }
+    __syncthreads();
+    if (live && i < m)
+    {
+      // row i: the terms in which species i is a target, each adding the
+      // reaction's partial derivatives to the columns of its four species
+      for (int e = e0; e < e1; e++)
+      {
+        const int code  = s_sent[e];
+        const int r     = code >> 2;
+        const bool loss = (code & 3) < 2;
+        const int4 ix   = ix4[r];
+        const double kfv = rk[r], kbv = rk[R + r];
+        const int col[4]   = {ix.x, ix.y, ix.z, ix.w};
+        const double dw[4] = {kfv * ys[ix.y], kfv * ys[ix.x], -(kbv * ys[ix.w]),
+                              -(kbv * ys[ix.z])};
+#pragma unroll
         for (int q = 0; q < 4; q++)
         {
-          int e = ix[q] * m + i;
-          Jl[e] = (t < 2) ? (Jl[e] - dw[q]) : (Jl[e] + dw[q]);
+          const int a = col[q] * m + i;
+          Jl[a]       = loss ? (Jl[a] - dw[q]) : (Jl[a] + dw[q]);
         }
       }
     }
-  }
-  __syncthreads();
-  if (live)
-  {
-    double* Jc = J + cell * (long)m * m;
-    for (int e = i; e < m * m; e += G) { Jc[e] = Jl[e]; }
+    __syncthreads();
+    if (live)
+    {
+      double* Jc = J + cell * (long)mm;
+      for (int e = i; e < mm; e += G) { Jc[e] = Jl[e]; }
+    }
+    __syncthreads();
   }
 }
 
@@ -154,38 +209,76 @@ static int pow2ceil_h(int m)
   return g;
 }
 
+// dynamic LDS of the RHS and Jacobian workgroups
+static size_t ma_rhs_lds(int m, int R)
+{
+  const int gpw = 64 / pow2ceil_h(m);
+  return sizeof(double) * (size_t)gpw * (size_t)((m + R + 1) & ~1) +
+         sizeof(int) * ma_table_ints(m, R);
+}
+
+static size_t ma_jac_lds(int m, int R)
+{
+  const int gpw = 64 / pow2ceil_h(m);
+  return sizeof(double) * (size_t)gpw * (size_t)((m * m + m + 2 * R + 1) & ~1) +
+         sizeof(int) * ma_table_ints(m, R);
+}
+
 extern "C" SUNAMD_MassActionDev* SUNAMD_MassActionToDevice(const SUNAMD_MassAction* p,
                                                            void* stream)
 {
-  if (!p || p->m > 64) { return NULL; }
+  if (!p || p->m < 1 || p->m > 64 || p->R < 1) { return NULL; }
+  if (ma_jac_lds(p->m, p->R) > 65536 || ma_rhs_lds(p->m, p->R) > 65536) { return NULL; }
+  for (int t = 0; t < 4 * p->R; t++)
+  {
+    if (p->idx[t] < 0 || p->idx[t] >= p->m) { return NULL; }
+  }
   SUNAMD_MassActionDev* d = (SUNAMD_MassActionDev*)calloc(1, sizeof(SUNAMD_MassActionDev));
   if (!d) { return NULL; }
   d->m      = p->m;
   d->R      = p->R;
   d->ncells = (long)p->ncells;
   d->stream = (hipStream_t)stream;
-  size_t nr = (size_t)p->R * (size_t)p->ncells;
-  double* t = (double*)malloc(sizeof(double) * nr);
-  hipError_t e = hipMalloc((void**)&d->idx, sizeof(int) * 4 * (size_t)p->R);
+  const size_t nr = (size_t)p->R * (size_t)p->ncells;
+  // species s's terms in reaction order, roles a b c d within a reaction:
+  // the order in which the serial RHS and Jacobian accumulate them
+  int* off  = (int*)calloc((size_t)p->m + 1, sizeof(int));
+  int* ent  = (int*)malloc(sizeof(int) * 4 * (size_t)p->R);
+  int* fill = (int*)malloc(sizeof(int) * (size_t)p->m);
+  hipError_t e = (off && ent && fill) ? hipSuccess : hipErrorOutOfMemory;
+  if (e == hipSuccess)
+  {
+    for (int t = 0; t < 4 * p->R; t++) { off[p->idx[t] + 1]++; }
+    for (int q = 0; q < p->m; q++) { off[q + 1] += off[q]; }
+    memcpy(fill, off, sizeof(int) * (size_t)p->m);
+    for (int t = 0; t < 4 * p->R; t++) { ent[fill[p->idx[t]]++] = t; }
+  }
+  if (e == hipSuccess) { e = hipMalloc((void**)&d->idx, sizeof(int) * 4 * (size_t)p->R); }
+  if (e == hipSuccess) { e = hipMalloc((void**)&d->soff, sizeof(int) * ((size_t)p->m + 1)); }
+  if (e == hipSuccess) { e = hipMalloc((void**)&d->sent, sizeof(int) * 4 * (size_t)p->R); }
   if (e == hipSuccess) { e = hipMalloc((void**)&d->kf, sizeof(double) * nr); }
   if (e == hipSuccess) { e = hipMalloc((void**)&d->kb, sizeof(double) * nr); }
-  if (e == hipSuccess && t)
+  if (e == hipSuccess)
   {
     e = hipMemcpy(d->idx, p->idx, sizeof(int) * 4 * (size_t)p->R, hipMemcpyHostToDevice);
-    for (int pass = 0; pass < 2 && e == hipSuccess; pass++)
-    {
-      const double* src = pass ? p->kb : p->kf;
-      for (long c = 0; c < d->ncells; c++)
-      {
-        for (int r = 0; r < p->R; r++) { t[(size_t)r * d->ncells + c] = src[c * p->R + r]; }
-      }
-      e = hipMemcpy(pass ? d->kb : d->kf, t, sizeof(double) * nr, hipMemcpyHostToDevice);
-    }
   }
-  free(t);
-  if (e != hipSuccess || !t)
+  if (e == hipSuccess)
   {
-    if (e != hipSuccess) { hipfail(e, "problem upload"); }
+    e = hipMemcpy(d->soff, off, sizeof(int) * ((size_t)p->m + 1), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess)
+  {
+    e = hipMemcpy(d->sent, ent, sizeof(int) * 4 * (size_t)p->R, hipMemcpyHostToDevice);
+  }
+  // the host problem keeps its rates cell-major already
+  if (e == hipSuccess) { e = hipMemcpy(d->kf, p->kf, sizeof(double) * nr, hipMemcpyHostToDevice); }
+  if (e == hipSuccess) { e = hipMemcpy(d->kb, p->kb, sizeof(double) * nr, hipMemcpyHostToDevice); }
+  free(off);
+  free(ent);
+  free(fill);
+  if (e != hipSuccess)
+  {
+    hipfail(e, "problem upload");
     SUNAMD_MassActionDevFree(d);
     return NULL;
   }
@@ -195,21 +288,52 @@ extern "C" SUNAMD_MassActionDev* SUNAMD_MassActionToDevice(const SUNAMD_MassActi
 extern "C" void SUNAMD_MassActionDevFree(SUNAMD_MassActionDev* d)
 {
   if (!d) { return; }
-  if (d->idx) { (void)hipFree(d->idx); }
-  if (d->kf) { (void)hipFree(d->kf); }
-  if (d->kb) { (void)hipFree(d->kb); }
+  void* ps[] = {d->idx, d->soff, d->sent, d->kf, d->kb};
+  for (void* q : ps)
+  {
+    if (q) { (void)hipFree(q); }
+  }
   free(d);
+}
+
+// workgroups for a grid-stride launch: enough to fill every CU at the
+// kernel's occupancy, no more than the cells need
+template <typename K>
+static unsigned ma_grid(K kern, size_t shm, long ncells, int gpw)
+{
+  static int cus = 0;
+  if (cus == 0)
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus < 1)
+    {
+      cus = 256;
+    }
+  }
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, shm) != hipSuccess ||
+      per_cu < 1)
+  {
+    per_cu = 1;
+  }
+  const long need = (ncells + gpw - 1) / gpw;
+  const long full = 4L * per_cu * cus;
+  return (unsigned)(need < full ? need : full);
 }
 
 extern "C" int SUNAMD_MassActionRhsDevArrays(SUNAMD_MassActionDev* d, const double* dy,
                                              double* df)
 {
-  const int cpb = rhs_cpb(d->m);
-  long g        = (d->ncells + cpb - 1) / cpb;
-  size_t shm    = sizeof(double) * 2 * cpb * (size_t)(d->m + 1);
+  const int G   = pow2ceil_h(d->m);
+  const int gpw = 64 / G;
+  const size_t shm = ma_rhs_lds(d->m, d->R);
+  if (d->ncells < 1) { return 0; }
+  const unsigned g = ma_grid(k_ma_rhs, shm, d->ncells, gpw);
   sunamd_prof_begin(SUNAMD_K_RHS, d->stream);
-  k_ma_rhs<<<(unsigned)g, cpb, shm, d->stream>>>(dy, df, d->idx, d->kf, d->kb, d->m, d->R,
-                                                 d->ncells);
+  k_ma_rhs<<<g, 64, shm, d->stream>>>(dy, df, d->idx, d->soff, d->sent, d->kf, d->kb, d->m, G,
+                                      d->R, d->ncells);
   sunamd_prof_end(SUNAMD_K_RHS, d->stream);
   hipError_t e = hipGetLastError();
   return (e == hipSuccess) ? 0 : hipfail(e, "rhs launch");
@@ -218,12 +342,13 @@ extern "C" int SUNAMD_MassActionRhsDevArrays(SUNAMD_MassActionDev* d, const doub
 extern "C" int SUNAMD_MassActionJacDevArrays(SUNAMD_MassActionDev* d, const double* dy,
                                              double* dJ)
 {
-  int G      = pow2ceil_h(d->m);
-  int cpw    = 64 / G;
-  long g     = (d->ncells + cpw - 1) / cpw;
-  size_t shm = sizeof(double) * (size_t)cpw * d->m * d->m;
-  k_ma_jac<<<(unsigned)g, 64, shm, d->stream>>>(dy, dJ, d->idx, d->kf, d->kb, d->m, G, d->R,
-                                                d->ncells);
+  const int G   = pow2ceil_h(d->m);
+  const int gpw = 64 / G;
+  const size_t shm = ma_jac_lds(d->m, d->R);
+  if (d->ncells < 1) { return 0; }
+  const unsigned g = ma_grid(k_ma_jac, shm, d->ncells, gpw);
+  k_ma_jac<<<g, 64, shm, d->stream>>>(dy, dJ, d->idx, d->soff, d->sent, d->kf, d->kb, d->m, G,
+                                      d->R, d->ncells);
   hipError_t e = hipGetLastError();
   return (e == hipSuccess) ? 0 : hipfail(e, "jac launch");
 }
@@ -343,8 +468,8 @@ static int coupled_run_device(SUNAMD_CoupledRun* r, double tout, SUNAMD_SolveSta
   job.R            = r->dev->R;
   job.ncells       = r->dev->ncells;
   job.idx          = r->dev->idx;
-  job.kf_rm        = r->dev->kf;
-  job.kb_rm        = r->dev->kb;
+  job.kf           = r->dev->kf;
+  job.kb           = r->dev->kb;
   job.y0           = N_VGetDeviceArrayPointer(r->y0);
   job.yout         = N_VGetDeviceArrayPointer(r->y);
   job.lu           = SUNMatrix_Mi355BlockDense_DeviceData(r->A);

@@ -211,3 +211,15 @@ def test_abi_compiled_layout():
         out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
     got = {ln.split()[0]: [int(x) for x in ln.split()[1:]] for ln in out.splitlines()}
     assert got == expect
+
+
+def test_massaction_upload_rejects_oversized_network():
+    """SUNAMD_MassActionToDevice refuses, before any device call, a network
+    whose per-cell Jacobian work area does not fit the workgroup's 64 KiB of
+    LDS (m = 64 with 2000 reactions) and m outside 1..64."""
+    core, hip = _lib.core(), _lib.hip()
+    for m, R in ((64, 2000), (65, 10)):
+        p = core.SUNAMD_MassActionCreate(m, R, 1, 5, 0.2)
+        assert p
+        assert not hip.SUNAMD_MassActionToDevice(p, None)
+        core.SUNAMD_MassActionFree(p)

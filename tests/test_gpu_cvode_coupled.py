@@ -42,8 +42,14 @@ def _dev_array(n):
     return torch.zeros(n, dtype=torch.float64, device="cuda")
 
 
-@pytest.mark.parametrize("m,R,ncells", [(4, 6, 33), (8, 16, 257), (32, 48, 65), (64, 96, 7)])
+@pytest.mark.parametrize("m,R,ncells", [(4, 6, 33), (8, 16, 257), (32, 48, 65), (64, 96, 7),
+                                        (3, 5, 1001), (24, 36, 130), (48, 72, 9),
+                                        (4, 6, 600000)])
 def test_rhs_and_jacobian_bitwise(libs, m, R, ncells):
+    """Device RHS and block Jacobian bit for bit against the serial evaluation:
+    power-of-two and other sizes (idle lanes in a group), cell counts that
+    are not a multiple of the groups per workgroup, and 600,000 cells (more
+    than one cell per group: the workgroups stride over the cells)."""
     import torch
 
     core, hip, orc = libs
