@@ -104,6 +104,17 @@ enum Out
   O_COUNT
 };
 
+// Global-memory view of an argument-block pointer.  The kernel reads its
+// pointers from the argument block in device memory, so without this the
+// compiler cannot tell global from LDS addresses and emits flat accesses,
+// which count against the LDS wait counter too: every LDS wait of a pass
+// would then also wait for the cell's streaming loads and stores.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gptr(T* p)
+{
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 struct CplArgs
 {
   BucketArgs ba; // ba.R: the reaction count the shared RHS code reads
@@ -234,11 +245,11 @@ __device__ __forceinline__ Red reduce(const CplArgs& A, const double (&acc)[3], 
     {
       double t = s_w[0][k];
       for (int v = 1; v < CW; v++) { t = t + s_w[v][k]; }
-      A.part[pb + (long)blockIdx.x * NRED + k] = t;
+      gptr(A.part)[pb + (long)blockIdx.x * NRED + k] = t;
     }
     double t = s_w[0][3];
     for (int v = 1; v < CW; v++) { t = (s_w[v][3] > t) ? s_w[v][3] : t; }
-    A.part[pb + (long)blockIdx.x * NRED + 3] = t;
+    gptr(A.part)[pb + (long)blockIdx.x * NRED + 3] = t;
   }
   Red r;
   r.dead = grid_barrier(A, gen, s_dead);
@@ -247,8 +258,8 @@ __device__ __forceinline__ Red reduce(const CplArgs& A, const double (&acc)[3], 
   for (int b = lane; b < A.nwg; b += 64)
   {
 #pragma unroll
-    for (int k = 0; k < 3; k++) { s[k] = s[k] + A.part[pb + (long)b * NRED + k]; }
-    const double v = A.part[pb + (long)b * NRED + 3];
+    for (int k = 0; k < 3; k++) { s[k] = s[k] + gptr(A.part)[pb + (long)b * NRED + k]; }
+    const double v = gptr(A.part)[pb + (long)b * NRED + 3];
     m              = (v > m) ? v : m;
   }
 #pragma unroll
@@ -264,7 +275,7 @@ __device__ __forceinline__ Red reduce(const CplArgs& A, const double (&acc)[3], 
       for (long b = 0; b < A.N; b += 64)
       {
         const long i  = b + lane;
-        const double v = (i < A.N) ? A.seq[sb + (long)k * A.N + i] : 0.0;
+        const double v = (i < A.N) ? gptr(A.seq)[sb + (long)k * A.N + i] : 0.0;
         const int cnt  = (A.N - b < 64) ? (int)(A.N - b) : 64;
         for (int j = 0; j < cnt; j++) { t = t + readlane_d(v, j); }
       }
@@ -351,8 +362,8 @@ __device__ __forceinline__ void stage_rates(const CplArgs& A, const Grp<M, T>& g
   gsync();
   for (int r = g.i; r < A.R; r += G)
   {
-    g.rates[r]       = A.kf[cc * A.R + r];
-    g.rates[A.R + r] = A.kb[cc * A.R + r];
+    g.rates[r]       = gptr(A.kf)[cc * A.R + r];
+    g.rates[A.R + r] = gptr(A.kb)[cc * A.R + r];
   }
   gsync();
 }
@@ -370,11 +381,11 @@ __device__ __forceinline__ void lu_store(const CplArgs& A, const Grp<M, T>& g,
   int p           = S.pos;
   if constexpr (G >= 32) { p = g.inv[g.valid ? g.i : 0]; }
   if (!c.lv) { return; }
-  T* lu = (T*)A.lu + c.cc * (long)M * M + g.i;
+  auto lu = gptr((T*)A.lu) + c.cc * (long)M * M + g.i;
 #pragma unroll
   for (int j = 0; j < M; j++) { lu[(long)j * M] = S.lu[j]; }
-  A.pos[c.e] = p;
-  if constexpr (sizeof(T) == 4) { A.rinv[c.e] = S.rinv; }
+  gptr(A.pos)[c.e] = p;
+  if constexpr (sizeof(T) == 4) { gptr(A.rinv)[c.e] = S.rinv; }
 }
 
 template <int M, typename T>
@@ -382,11 +393,11 @@ __device__ __forceinline__ void lu_load(const CplArgs& A, const Grp<M, T>& g, Sy
                                         const Cell<M, T>& c)
 {
   constexpr int G = pow2ceil(M);
-  const T* lu     = (const T*)A.lu + c.cc * (long)M * M + (g.valid ? g.i : 0);
+  const auto lu   = gptr((const T*)A.lu) + c.cc * (long)M * M + (g.valid ? g.i : 0);
 #pragma unroll
   for (int j = 0; j < M; j++) { S.lu[j] = g.valid ? lu[(long)j * M] : (T)0; }
-  const int p = g.valid ? A.pos[c.e] : g.i;
-  if constexpr (sizeof(T) == 4) { S.rinv = g.valid ? A.rinv[c.e] : 1.0f; }
+  const int p = g.valid ? gptr(A.pos)[c.e] : g.i;
+  if constexpr (sizeof(T) == 4) { S.rinv = g.valid ? gptr(A.rinv)[c.e] : 1.0f; }
   gsync();
   if constexpr (G >= 32)
   {
@@ -408,7 +419,7 @@ __device__ __forceinline__ void add_term(const CplArgs& A, double (&acc)[3], int
   const double t = live ? p * p : 0.0;
   if (A.sequential)
   {
-    if (live) { A.seq[(buf_of(gen) * 3 + k) * A.N + e] = t; }
+    if (live) { gptr(A.seq)[(buf_of(gen) * 3 + k) * A.N + e] = t; }
   }
   else { acc[k] = acc[k] + t; }
 }
@@ -538,14 +549,14 @@ __device__ __forceinline__ void pass_newton(const CplArgs& A, const Grp<M, T>& g
     for (int t = 0; t < RPL; t++)
     {
       const int r = g.i + t * G;
-      kr[t]       = (regs && r < A.R) ? A.kf[c.cc * A.R + r] : 0.0;
-      br[t]       = (regs && r < A.R) ? A.kb[c.cc * A.R + r] : 0.0;
+      kr[t]       = (regs && r < A.R) ? gptr(A.kf)[c.cc * A.R + r] : 0.0;
+      br[t]       = (regs && r < A.R) ? gptr(A.kb)[c.cc * A.R + r] : 0.0;
     }
     double z[QMAX + 1];
 #pragma unroll
-    for (int j = 0; j <= QMAX; j++) { z[j] = (c.lv && (j <= 1 || zn_ops)) ? A.zn[(long)j * N + c.e] : 0.0; }
-    double acor = (d.acor_zero && !d.save_acor) ? 0.0 : (c.lv ? A.acor[c.e] : 0.0);
-    const double w = c.lv ? A.ewt[c.e] : 0.0;
+    for (int j = 0; j <= QMAX; j++) { z[j] = (c.lv && (j <= 1 || zn_ops)) ? gptr(A.zn)[(long)j * N + c.e] : 0.0; }
+    double acor = (d.acor_zero && !d.save_acor) ? 0.0 : (c.lv ? gptr(A.acor)[c.e] : 0.0);
+    const double w = c.lv ? gptr(A.ewt)[c.e] : 0.0;
     if (!d.setup) { lu_load(A, g, S, c); }
     // the Newton matrix and the order-1 restart's RHS read any reaction's
     // rates: the cell's rates go to the group's LDS slot
@@ -634,7 +645,7 @@ __device__ __forceinline__ void pass_newton(const CplArgs& A, const Grp<M, T>& g
       if (c.lv)
       {
 #pragma unroll
-        for (int j = 0; j <= QMAX; j++) { A.zn[(long)j * N + c.e] = z[j]; }
+        for (int j = 0; j <= QMAX; j++) { gptr(A.zn)[(long)j * N + c.e] = z[j]; }
       }
     }
     if (d.acor_zero) { acor = 0.0; }
@@ -645,9 +656,9 @@ __device__ __forceinline__ void pass_newton(const CplArgs& A, const Grp<M, T>& g
       if (d.setup == 2)
       {
         ys = y;
-        if (c.lv) { A.ysav[c.e] = ys; }
+        if (c.lv) { gptr(A.ysav)[c.e] = ys; }
       }
-      else { ys = c.lv ? A.ysav[c.e] : 0.0; }
+      else { ys = c.lv ? gptr(A.ysav)[c.e] : 0.0; }
       if constexpr (cpl_shared_rows<M, T>()) { newton_matrix_turns(g, ys, d.gamma, S.lu); }
       else { g_newton_matrix(g, ys, d.gamma, S.lu); }
       const int fail = g_getrf(g, S);
@@ -661,7 +672,7 @@ __device__ __forceinline__ void pass_newton(const CplArgs& A, const Grp<M, T>& g
     delta            = g_getrs(g, S, delta);
     if (d.gscale != 1.0) { delta = d.gscale * delta; }
     acor = lsum(1.0, acor, 1.0, delta);
-    if (c.lv) { A.acor[c.e] = acor; }
+    if (c.lv) { gptr(A.acor)[c.e] = acor; }
     add_term(A, acc, 0, c.e, c.lv, delta, w, gen);
     add_term(A, acc, 1, c.e, c.lv, acor, w, gen);
   }
@@ -700,10 +711,10 @@ __device__ __forceinline__ void pass_done(const CplArgs& A, const Grp<M, T>& g, 
 #pragma unroll
     for (int j = 0; j <= QMAX; j++)
     {
-      z[j] = (j <= d.q || (j == QMAX && qmax_used)) ? A.zn[(long)j * N + c.e] : 0.0;
+      z[j] = (j <= d.q || (j == QMAX && qmax_used)) ? gptr(A.zn)[(long)j * N + c.e] : 0.0;
     }
-    const double acor = A.acor[c.e];
-    const double w    = A.ewt[c.e];
+    const double acor = gptr(A.acor)[c.e];
+    const double w    = gptr(A.ewt)[c.e];
 #pragma unroll
     for (int j = 0; j <= QMAX; j++)
     {
@@ -713,7 +724,7 @@ __device__ __forceinline__ void pass_done(const CplArgs& A, const Grp<M, T>& g, 
 #pragma unroll
     for (int j = 0; j <= QMAX; j++)
     {
-      if (j <= d.q || (j == QMAX && d.save)) { A.zn[(long)j * N + c.e] = z[j]; }
+      if (j <= d.q || (j == QMAX && d.save)) { gptr(A.zn)[(long)j * N + c.e] = z[j]; }
     }
     if (d.qm1)
     {
@@ -730,7 +741,7 @@ __device__ __forceinline__ void pass_done(const CplArgs& A, const Grp<M, T>& g, 
     // the next step's error weights (cvEwtSet, scalar tolerances) and the
     // norm of zn0 in them (CVode's tolerance-scale check)
     const double wn = 1.0 / ((A.rtol * fabs(z[0])) + A.atol);
-    A.ewt[c.e]      = wn;
+    gptr(A.ewt)[c.e]      = wn;
     add_term(A, acc, 2, c.e, true, z[0], wn, gen);
   }
 }
@@ -766,10 +777,10 @@ __global__ void __launch_bounds__(64 * cpl_waves<M>(), 2) k_coupled_bdf(const Cp
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int t = threadIdx.x; t < 4 * A.R; t += CT)
   {
-    s_idx[t]  = A.idx[t];
-    s_sent[t] = A.sent[t];
+    s_idx[t]  = gptr(A.idx)[t];
+    s_sent[t] = gptr(A.sent)[t];
   }
-  for (int t = threadIdx.x; t <= M; t += CT) { s_soff[t] = A.soff[t]; }
+  for (int t = threadIdx.x; t <= M; t += CT) { s_soff[t] = gptr(A.soff)[t]; }
   __syncthreads();
 
   const int grp = lane / G;
@@ -852,14 +863,14 @@ __global__ void __launch_bounds__(64 * cpl_waves<M>(), 2) k_coupled_bdf(const Cp
   {
     const Cell<M, T> c = cell_at(A, g, ch);
     stage_rates(A, g, c.cc);
-    const double y  = c.lv ? A.y0[c.e] : 0.0;
+    const double y  = c.lv ? gptr(A.y0)[c.e] : 0.0;
     const double w  = 1.0 / ((A.rtol * fabs(y)) + A.atol);
     const double f0 = g_rhs(g, y);
     if (c.lv)
     {
-      A.zn[c.e]     = y;
-      A.zn[N + c.e] = f0;
-      A.ewt[c.e]    = w;
+      gptr(A.zn)[c.e]     = y;
+      gptr(A.zn)[N + c.e] = f0;
+      gptr(A.ewt)[c.e]    = w;
       // cvUpperBoundH0: |f0| / (HUB |y0| + 1/w)
       const double inv = 1.0 / w;
       const double den = (HUB_FACTOR * fabs(y)) + (1.0 * inv);
@@ -902,12 +913,12 @@ __global__ void __launch_bounds__(64 * cpl_waves<M>(), 2) k_coupled_bdf(const Cp
           {
             const Cell<M, T> c = cell_at(A, g, ch);
             stage_rates(A, g, c.cc);
-            const double z0 = c.lv ? A.zn[c.e] : 0.0;
-            const double z1 = c.lv ? A.zn[N + c.e] : 0.0;
+            const double z0 = c.lv ? gptr(A.zn)[c.e] : 0.0;
+            const double z1 = c.lv ? gptr(A.zn)[N + c.e] : 0.0;
             const double y  = lsum(hgs, z1, 1.0, z0);
             const double f  = g_rhs(g, y);
             const double t  = lsum(ra, f, rb, z1);
-            add_term(A, a2, 0, c.e, c.lv, t, c.lv ? A.ewt[c.e] : 0.0, gen);
+            add_term(A, a2, 0, c.e, c.lv, t, c.lv ? gptr(A.ewt)[c.e] : 0.0, gen);
           }
           Red r2 = reduce<CW>(A, a2, 0.0, gen, s_w, &s_dead);
           s.nfe++;
@@ -1469,7 +1480,7 @@ __global__ void __launch_bounds__(64 * cpl_waves<M>(), 2) k_coupled_bdf(const Cp
       if (!cl.lv) { continue; }
       double z[QMAX + 1];
 #pragma unroll
-      for (int j = 0; j <= QMAX; j++) { z[j] = A.zn[(long)j * N + cl.e]; }
+      for (int j = 0; j <= QMAX; j++) { z[j] = gptr(A.zn)[(long)j * N + cl.e]; }
       if (!out_dky && pd.restore_q)
       {
 #pragma unroll
@@ -1500,12 +1511,12 @@ __global__ void __launch_bounds__(64 * cpl_waves<M>(), 2) k_coupled_bdf(const Cp
         }
       }
       else { yv = z[0]; }
-      A.yout[cl.e] = yv;
+      gptr(A.yout)[cl.e] = yv;
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0)
   {
-    double* o    = A.out;
+    auto o       = gptr(A.out);
     o[O_NST]     = (double)s.nst;
     o[O_NFE]     = (double)s.nfe;
     o[O_NSETUPS] = (double)s.nsetups;

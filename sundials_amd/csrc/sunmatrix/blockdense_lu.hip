@@ -302,25 +302,37 @@ __global__ void __launch_bounds__(64) k_getrs(const T* __restrict__ F, const int
   const bool valid  = (blk < nblocks) && (r < M);
   const int base    = grp * G;
 
-  double lu[M];
-  const T* Fb = F + blk * (long)M * M;
+  // every lane loads from inside block 0 or its own block (rows past M read
+  // row M-1), so the loads are unconditional and all in flight at once; the
+  // values of idle lanes are then replaced by zeros (a load under the
+  // validity test would be a branch per element, and for FP32 factors the
+  // conversion made each one wait for its load)
+  const long blk_s = (blk < nblocks) ? blk : 0;
+  const int r_s    = (r < M) ? r : M - 1;
+  const T* Fb      = F + blk_s * (long)M * M;
+  T raw[M];
 #pragma unroll
-  for (int j = 0; j < M; j++) { lu[j] = valid ? (double)Fb[j * M + r] : 0.0; }
+  for (int j = 0; j < M; j++) { raw[j] = Fb[j * M + r_s]; }
+  const int* p = piv + blk_s * M;
+  int pv[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) { pv[k] = p[k]; }
+  double lu[M];
+#pragma unroll
+  for (int j = 0; j < M; j++) { lu[j] = valid ? (double)raw[j] : 0.0; }
 
   // source position of b after the sequential swaps b[k] <-> b[p[k]]
-  int idx = r;
-  if (valid)
-  {
-    const int* p = piv + blk * M;
+  int idx = r_s;
 #pragma unroll
-    for (int k = M - 1; k >= 0; k--)
-    {
-      int pk = p[k];
-      if (idx == k) { idx = pk; }
-      else if (idx == pk) { idx = k; }
-    }
+  for (int k = M - 1; k >= 0; k--)
+  {
+    const int pk = pv[k];
+    if (idx == k) { idx = pk; }
+    else if (idx == pk) { idx = k; }
   }
-  double bi = valid ? b[blk * M + idx] : 0.0;
+  // (pivots of an unfactored block are not trusted to stay in range)
+  const double b_s = b[blk_s * M + (((unsigned)idx < (unsigned)M) ? idx : 0)];
+  double bi        = valid ? b_s : 0.0;
 
   // L y = P b (unit lower)
 #pragma unroll
