@@ -188,26 +188,49 @@ __global__ void __launch_bounds__(64 * LU_WAVES) k_getrf(const double* A, T* F /
   const long blk   = (long)blockIdx.x * (LU_WAVES * BPW) + slot;
   const bool valid = (blk < nblocks) && (r < M);
 
+  // every lane loads from inside block 0 or its own block (rows past M read
+  // row M-1), so the loads are unconditional and in flight together; idle
+  // lanes' values are replaced afterwards (a load under the validity test
+  // compiles to a branch per element, which waits for its load wherever a
+  // conversion or the scaling follows it).  FP32 factors, and the scaled
+  // forms at m > 32, take the FP64 source in chunks of 16 columns (a
+  // scheduling barrier between chunks), so the loads in flight do not cost the
+  // kernel its occupancy.
+  const long blk_s = (blk < nblocks) ? blk : 0;
+  const int r_s    = (r < M) ? r : M - 1;
+  const double* Ab = A + blk_s * (long)M * M;
+  const long o0    = blk * (long)M * M + r;
+  constexpr int CH = ((sizeof(T) == 4 && M > 16) || (SRC != SRC_PLAIN && M > 32)) ? 16 : M;
   T a[M];
-  const double* Ab = A + blk * (long)M * M;
-  if (SRC == SRC_PLAIN)
-  {
 #pragma unroll
-    for (int j = 0; j < M; j++) { a[j] = valid ? to_t<T>(Ab[j * M + r]) : (T)0; }
-  }
-  else
+  for (int j0 = 0; j0 < M; j0 += CH)
   {
-    const long o0 = blk * (long)M * M + r;
+    double raw[CH];
 #pragma unroll
-    for (int j = 0; j < M; j++)
+    for (int q = 0; q < CH; q++)
     {
-      double v = valid ? Ab[j * M + r] : 0.0;
-      if (SRC == SRC_SAVE && valid) { save[o0 + (long)j * M] = v; }
-      v = v * c;
-      if (j == r) { v = v + 1.0; }
-      if (!std::is_same<T, double>::value && valid) { scaled[o0 + (long)j * M] = v; }
-      a[j] = valid ? to_t<T>(v) : (T)0;
+      if (j0 + q < M) { raw[q] = Ab[(j0 + q) * M + r_s]; }
     }
+#pragma unroll
+    for (int q = 0; q < CH; q++)
+    {
+      const int j = j0 + q;
+      if (j >= M) { break; }
+      if (SRC == SRC_PLAIN) { a[j] = valid ? to_t<T>(raw[q]) : (T)0; }
+      else
+      {
+        double v = valid ? raw[q] : 0.0;
+        if (SRC == SRC_SAVE && valid) { save[o0 + (long)j * M] = v; }
+        v = v * c;
+        if (j == r) { v = v + 1.0; }
+        if (!std::is_same<T, double>::value && valid) { scaled[o0 + (long)j * M] = v; }
+        a[j] = valid ? to_t<T>(v) : (T)0;
+      }
+      // the single-precision value is formed here, not where it is first
+      // used (the compiler would otherwise keep the double until then)
+      if constexpr (sizeof(T) == 4) { asm volatile("" : "+v"(a[j])); }
+    }
+    if constexpr (CH < M) { __builtin_amdgcn_sched_barrier(0); }
   }
 
   int pos  = r; // physical row currently held by this lane
