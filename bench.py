@@ -587,7 +587,7 @@ def stream_traffic(workload, kernel, n):
 
 # the kernel source the MEASURED_TRAFFIC figures were taken at (first 16 hex digits of the
 # SHA-256 of bdf_device.h followed by bdf_args.h); a changed kernel reports traffic null
-MEASURED_TRAFFIC_SOURCE = "ecca45b33cc880d6"
+MEASURED_TRAFFIC_SOURCE = "9c515d30832c11dc"
 
 
 def batched_kernel_source_hash():
@@ -600,18 +600,26 @@ def batched_kernel_source_hash():
     return h.hexdigest()[:16]
 
 
-# HBM bytes per launch of the FP32-LU kernels at three wavefronts per SIMD (the budget the
-# autotuner picks), rocprofv3 --pmc FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (the factors a
-# plain 256 MiB copy gives) on the 65,536-system batch; almost all of it is register-spill
-# stores (the algorithmic traffic, states and rate constants, is ~20 MB per launch)
+# HBM bytes per launch of the FP32-LU kernels at three wavefronts per SIMD (8 workgroups
+# per CU) and at two (32 workgroups per CU), rocprofv3 --pmc FETCH_SIZE x 2.000 + WRITE_SIZE
+# x 1.000 (the factors a plain 256 MiB copy gives) on the 65,536-system batch; at three
+# wavefronts almost all of it is register-spill stores, at two it is close to the
+# algorithmic traffic (states, rate constants, counters: ~10-20 MB per launch)
 MEASURED_TRAFFIC = {
-    "k_batched_bdf<4, float, 3, false>": 19.32e6,
-    "k_batched_bdf<8, float, 3, false>": 34.18e6,
-    "k_batched_bdf<16, float, 3, false>": 343.8e6,
-    "k_batched_bdf<24, float, 3, false>": 146.8e6,
-    "k_batched_bdf<32, float, 3, false>": 1136.0e6,
-    "k_batched_bdf<48, float, 3, false>": 3347.6e6,
-    "k_batched_bdf<64, float, 3, false>": 6282.4e6,
+"k_batched_bdf<4, float, 3, false>": 19.3e6,
+    "k_batched_bdf<8, float, 3, false>": 34.2e6,
+    "k_batched_bdf<16, float, 3, false>": 341.8e6,
+    "k_batched_bdf<24, float, 3, false>": 147.1e6,
+    "k_batched_bdf<32, float, 3, false>": 1137.9e6,
+    "k_batched_bdf<48, float, 3, false>": 3347.8e6,
+    "k_batched_bdf<64, float, 3, false>": 6265.3e6,
+    "k_batched_bdf<4, float, 2, false>": 8.5e6,
+    "k_batched_bdf<8, float, 2, false>": 11.9e6,
+    "k_batched_bdf<16, float, 2, false>": 12.6e6,
+    "k_batched_bdf<24, float, 2, false>": 11.1e6,
+    "k_batched_bdf<32, float, 2, false>": 15.7e6,
+    "k_batched_bdf<48, float, 2, false>": 15.1e6,
+    "k_batched_bdf<64, float, 2, false>": 16.2e6,
 }
 
 

@@ -1,20 +1,13 @@
 // -----------------------------------------------------------------------------
 // Batched per-system BDF on the MI355X (host part): uploads a batch of size
-// buckets, launches one persistent kernel per bucket and gathers results back
-// into system order.  The buckets share at most SUNAMD_BATCH_LANES streams --
-// the hardware queues HIP gives a process by default; a stream beyond them
-// shares a queue with an earlier one, and its kernel then waits for that
-// queue's previous kernel to finish.  Buckets go to the streams largest work
-// first, each to the stream with the least work so far (the work of a bucket
-// is the wavefront time its last run measured, or ncells * pow2ceil(m)^0.62
-// before any run).  The
+// buckets, launches one persistent kernel per bucket on its own stream
+// (largest systems first) and gathers results back into system order.  The
 // integrator itself is in bdf_device.h; each (m, precision) kernel is compiled
 // in one of the bdf_inst_*.hip units.
 // -----------------------------------------------------------------------------
 #include <hip/hip_runtime.h>
 
 #include <chrono>
-#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -43,14 +36,10 @@ struct BucketDev
   double *kf, *kb, *y0, *yout;
   long* stats;
   unsigned long long* next;
-  hipStream_t stream;  // the stream the last run placed the bucket on (not owned)
+  hipStream_t stream;
   hipEvent_t ev0, ev1; // bracket the bucket's launch on its stream
   double last_ms;      // kernel time of the last run
-  double work;         // wavefront time of the last run (100 MHz ticks), 0 before
 };
-
-// streams the buckets are spread over (HIP's default hardware queues per process)
-#define SUNAMD_BATCH_LANES 4
 
 struct SUNAMD_BatchedBDF_
 {
@@ -76,9 +65,6 @@ struct SUNAMD_BatchedBDF_
   int log_stride;
   unsigned long long* dcount; // [0] log rows, [1 + 2k], [2 + 2k]: bucket k tried / kept
   unsigned long long* dphase; // phase clock build: SUNAMD_BDF_PHASE_VALUES per bucket
-  unsigned long long* dwtime; // per bucket: wavefront time of the run (100 MHz ticks)
-  int nlanes;
-  hipStream_t lane[SUNAMD_BATCH_LANES];
 };
 
 static int hfail(hipError_t e, const char* what)
@@ -115,7 +101,6 @@ static bool launch_bucket(const SUNAMD_BatchedBDF* B, int k, double tout, int gr
   }
   A.lstats = B->dcount + 1 + 2 * k;
   A.phase  = B->dphase ? B->dphase + (size_t)k * SUNAMD_BDF_PHASE_VALUES : nullptr;
-  A.wtime  = B->dwtime ? B->dwtime + k : nullptr;
   {
     const int gpw = 64 / pow2ceil(d.m);
     if (sunamd_bdf::bucket_lds_bytes(d.m, d.R, gpw, A.mlp_w != nullptr, A.log_x != nullptr,
@@ -193,12 +178,7 @@ extern "C" SUNAMD_BatchedBDF* SUNAMD_BatchedBDFCreate(const SUNAMD_Batch* batch,
   {
     B->num_cus = prop.multiProcessorCount;
   }
-  hipError_t e = hipMalloc((void**)&B->dwtime, sizeof(unsigned long long) * SUNAMD_BATCH_MAXB);
-  B->nlanes    = B->nb < SUNAMD_BATCH_LANES ? B->nb : SUNAMD_BATCH_LANES;
-  for (int l = 0; l < B->nlanes && e == hipSuccess; l++)
-  {
-    e = hipStreamCreateWithFlags(&B->lane[l], hipStreamNonBlocking);
-  }
+  hipError_t e = hipSuccess;
   for (int k = 0; k < B->nb && e == hipSuccess; k++)
   {
     const SUNAMD_MassAction* p = batch->bucket[k];
@@ -208,7 +188,8 @@ extern "C" SUNAMD_BatchedBDF* SUNAMD_BatchedBDFCreate(const SUNAMD_Batch* batch,
     d.ncells                   = (long)p->ncells;
     if (d.m > 64) { e = hipErrorInvalidValue; break; }
     long nc = d.ncells > 0 ? d.ncells : 1;
-    e       = hipEventCreate(&d.ev0);
+    e       = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) { e = hipEventCreate(&d.ev0); }
     if (e == hipSuccess) { e = hipEventCreate(&d.ev1); }
     if (e == hipSuccess) { e = hipMalloc((void**)&d.idx, sizeof(int) * 4 * d.R); }
     if (e == hipSuccess) { e = hipMalloc((void**)&d.kf, sizeof(double) * nc * d.R); }
@@ -311,26 +292,14 @@ extern "C" int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* w
 {
   if (hipDeviceSynchronize() != hipSuccess) { return -1; }
   auto t0 = std::chrono::steady_clock::now();
-  // the buckets' work: the last run's wavefront time when every bucket has
-  // one, else the model ncells * pow2ceil(m)^0.62 (a step's cost per system
-  // grows about so with the group width: more lanes per system, fewer
-  // systems per wavefront)
-  double work[SUNAMD_BATCH_MAXB];
-  bool measured = true;
-  for (int k = 0; k < B->nb; k++) { measured = measured && (B->b[k].ncells <= 0 || B->b[k].work > 0.0); }
-  for (int k = 0; k < B->nb; k++)
-  {
-    const BucketDev& d = B->b[k];
-    work[k] = measured ? d.work : (double)d.ncells * pow((double)pow2ceil(d.m), 0.62);
-  }
-  // largest work first, each bucket to the stream with the least work so far
+  // largest systems first: they have the longest per-system latency
   int order[SUNAMD_BATCH_MAXB];
   for (int k = 0; k < B->nb; k++) { order[k] = k; }
   for (int a = 0; a < B->nb; a++)
   {
     for (int c = a + 1; c < B->nb; c++)
     {
-      if (work[order[c]] > work[order[a]])
+      if (B->b[order[c]].m > B->b[order[a]].m)
       {
         int t    = order[a];
         order[a] = order[c];
@@ -338,21 +307,9 @@ extern "C" int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* w
       }
     }
   }
-  double lane_work[SUNAMD_BATCH_LANES] = {0.0};
-  for (int n = 0; n < B->nb; n++)
-  {
-    int best = 0;
-    for (int l = 1; l < B->nlanes; l++)
-    {
-      if (lane_work[l] < lane_work[best]) { best = l; }
-    }
-    lane_work[best] += work[order[n]];
-    B->b[order[n]].stream = B->lane[best];
-  }
   // learned-guess counters restart with every run (the log keeps filling)
   {
     hipError_t e = hipMemset(B->dcount + 1, 0, sizeof(unsigned long long) * 2 * SUNAMD_BATCH_MAXB);
-    if (e == hipSuccess) { e = hipMemset(B->dwtime, 0, sizeof(unsigned long long) * SUNAMD_BATCH_MAXB); }
     if (e != hipSuccess) { return hfail(e, "counter reset"); }
     if (B->dphase)
     {
@@ -397,12 +354,6 @@ extern "C" int SUNAMD_BatchedBDFRun(SUNAMD_BatchedBDF* B, double tout, double* w
     e        = hipEventElapsedTime(&ms, d.ev0, d.ev1);
     if (e != hipSuccess) { return hfail(e, "event timing"); }
     d.last_ms = ms;
-  }
-  {
-    unsigned long long wt[SUNAMD_BATCH_MAXB];
-    e = hipMemcpy(wt, B->dwtime, sizeof(wt), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) { return hfail(e, "work readback"); }
-    for (int k = 0; k < B->nb; k++) { B->b[k].work = (double)wt[k]; }
   }
   if (wall_s) { *wall_s = std::chrono::duration<double>(t1 - t0).count(); }
   return 0;
@@ -572,13 +523,9 @@ extern "C" void SUNAMD_BatchedBDFFree(SUNAMD_BatchedBDF* B)
     if (d.next) { (void)hipFree(d.next); }
     if (d.ev0) { (void)hipEventDestroy(d.ev0); }
     if (d.ev1) { (void)hipEventDestroy(d.ev1); }
-  }
-  for (int l = 0; l < B->nlanes; l++)
-  {
-    if (B->lane[l]) { (void)hipStreamDestroy(B->lane[l]); }
+    if (d.stream) { (void)hipStreamDestroy(d.stream); }
   }
   if (B->dcount) { (void)hipFree(B->dcount); }
-  if (B->dwtime) { (void)hipFree(B->dwtime); }
   if (B->dphase) { (void)hipFree(B->dphase); }
   free(B->sys_bucket);
   free(B->sys_cell);

@@ -48,10 +48,6 @@ struct BucketArgs
   const int* bad;
   // instrumented build only: clock ticks per phase, summed over the groups
   unsigned long long* phase;
-  // non-null: each wavefront adds the real-time clock ticks (100 MHz) it was
-  // resident -- the bucket's work, which the host uses to place the buckets
-  // on its streams in the next run
-  unsigned long long* wtime;
 };
 
 // LDS floats per lane group staging one training-log row (the inputs)

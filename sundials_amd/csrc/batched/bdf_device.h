@@ -1768,7 +1768,6 @@ __device__ __forceinline__ bool g_advance_tail(const Grp<M, T>& g, Sys<M, T>& S,
 template <int M, typename T, int W, bool LEARN>
 __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
 {
-  const unsigned long long t_resident = __builtin_amdgcn_s_memrealtime();
   constexpr int G   = pow2ceil(M);
   constexpr int GPW = 64 / G;
   __shared__ double s_red[64];
@@ -1925,10 +1924,6 @@ __global__ void __launch_bounds__(64, W) k_batched_bdf(BucketArgs A)
     atomicAdd(A.phase + 2 * SUNAMD_BDF_NPHASES + 1, (unsigned long long)CAL_MARKS);
   }
 #endif
-  if (lane == 0 && A.wtime != nullptr)
-  {
-    atomicAdd(A.wtime, __builtin_amdgcn_s_memrealtime() - t_resident);
-  }
 }
 
 
