@@ -40,6 +40,7 @@ struct SUNAMD_MassActionDev_
   double* kf; // ncells*R, cell-major
   double* kb;
   hipStream_t stream;
+  unsigned rhs_grid, jac_grid; // launch sizes, chosen at the first call
 };
 
 static int hipfail(hipError_t e, const char* what)
@@ -330,7 +331,8 @@ extern "C" int SUNAMD_MassActionRhsDevArrays(SUNAMD_MassActionDev* d, const doub
   const int gpw = 64 / G;
   const size_t shm = ma_rhs_lds(d->m, d->R);
   if (d->ncells < 1) { return 0; }
-  const unsigned g = ma_grid(k_ma_rhs, shm, d->ncells, gpw);
+  if (d->rhs_grid == 0) { d->rhs_grid = ma_grid(k_ma_rhs, shm, d->ncells, gpw); }
+  const unsigned g = d->rhs_grid;
   sunamd_prof_begin(SUNAMD_K_RHS, d->stream);
   k_ma_rhs<<<g, 64, shm, d->stream>>>(dy, df, d->idx, d->soff, d->sent, d->kf, d->kb, d->m, G,
                                       d->R, d->ncells);
@@ -346,7 +348,8 @@ extern "C" int SUNAMD_MassActionJacDevArrays(SUNAMD_MassActionDev* d, const doub
   const int gpw = 64 / G;
   const size_t shm = ma_jac_lds(d->m, d->R);
   if (d->ncells < 1) { return 0; }
-  const unsigned g = ma_grid(k_ma_jac, shm, d->ncells, gpw);
+  if (d->jac_grid == 0) { d->jac_grid = ma_grid(k_ma_jac, shm, d->ncells, gpw); }
+  const unsigned g = d->jac_grid;
   k_ma_jac<<<g, 64, shm, d->stream>>>(dy, dJ, d->idx, d->soff, d->sent, d->kf, d->kb, d->m, G,
                                       d->R, d->ncells);
   hipError_t e = hipGetLastError();
