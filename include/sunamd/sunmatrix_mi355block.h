@@ -72,6 +72,17 @@ SUNErrCode SUNLinSol_Mi355BlockLU_GetPivots(SUNLinearSolver S, int* host_out);
    matrix and solver ops are separate calls. */
 int SUNLinSol_Mi355BlockLUSetupScaled(SUNLinearSolver S, SUNMatrix A, SUNMatrix J, sunrealtype c,
                                       int save_to_J);
+/* The direct solve in one pass, in place of the calls of cvode_ls.c:1744-1784
+   (cvLsSolve): N_VConst(0, x); SUNLinSolSolve(S, A, x, b); N_VScale(1, x, b);
+   and, when the solution is scaled, N_VScale(scale, b, b).  b is solved in
+   place and each entry is written as scale * x (N_VScale's product; scale = 1
+   leaves x unchanged), so b ends bit-identical to the calls' result (x itself
+   is not written).  Returns SUN_SUCCESS, or SUN_ERR_ARG_INCOMPATIBLE without
+   doing anything when the calls must run instead: S not a block LU of A's
+   shape, b not an MI355X vector on the solver's stream, or m outside 1-10,
+   12, 16, 24, 32, 48, 64.  No counterpart in the reference. */
+int SUNLinSol_Mi355BlockLUSolveScaled(SUNLinearSolver S, SUNMatrix A, N_Vector b,
+                                      sunrealtype scale);
 
 #ifdef __cplusplus
 }

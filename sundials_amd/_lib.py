@@ -386,6 +386,8 @@ def hip():
     _sig(lib, "SUNAMD_CoupledDeviceLastShape", C.c_int, [P(C.c_int), P(C.c_int), P(C.c_int)])
     _sig(lib, "SUNLinSol_Mi355BlockLUSetupScaled", C.c_int,
          [c_void_p, c_void_p, c_void_p, c_real, C.c_int])
+    _sig(lib, "SUNLinSol_Mi355BlockLUSolveScaled", C.c_int,
+         [c_void_p, c_void_p, c_void_p, c_real])
     _sig(lib, "SUNAMD_ClassicalGS_Mi355", C.c_int, [P(c_void_p), P(P(c_real)), C.c_int, C.c_int,
                                                     P(c_real), P(c_real), P(c_void_p)])
     _sig(lib, "SUNAMD_TridiagATimes_Mi355", C.c_int, [c_void_p, c_void_p, c_void_p])

@@ -221,6 +221,7 @@ typedef struct CVodeMemRec
   /* copy + ScaleAddI + factorisation of the Newton matrix in one pass (NULL:
      the library has none); declines with SUN_ERR_ARG_INCOMPATIBLE */
   int (*fused_lsetup)(SUNLinearSolver, SUNMatrix, SUNMatrix, sunrealtype, int);
+  int (*fused_lsolve)(SUNLinearSolver, SUNMatrix, N_Vector, sunrealtype);
 
   sunbooleantype MallocDone;
 } CVodeMemRec;
@@ -575,6 +576,8 @@ int CVodeSetUseIntegratorFusedKernels(void* cvode_mem, sunbooleantype onoff)
                               N_Vector))rs;
   cv->fused_lsetup = (int (*)(SUNLinearSolver, SUNMatrix, SUNMatrix, sunrealtype,
                               int))dlsym(RTLD_DEFAULT, "SUNLinSol_Mi355BlockLUSetupScaled");
+  cv->fused_lsolve = (int (*)(SUNLinearSolver, SUNMatrix, N_Vector,
+                              sunrealtype))dlsym(RTLD_DEFAULT, "SUNLinSol_Mi355BlockLUSolveScaled");
   cv->usefused     = SUNTRUE;
   return CV_SUCCESS;
 }
@@ -785,15 +788,28 @@ static int cvLsSolve(CVodeMem cv, N_Vector b, N_Vector weight, N_Vector ynow,
   int curiter = 0;
   (void)SUNNonlinSolGetCurIter(cv->NLS, &curiter);
 
-  /* direct solver: zero tolerance, zero initial guess */
-  N_VConst(ZERO, ls->x);
-  retval = SUNLinSolSetZeroGuess(ls->LS, SUNTRUE);
-  if (retval != SUN_SUCCESS) { return -1; }
-  retval = SUNLinSolSolve(ls->LS, ls->A, ls->x, b, ZERO);
-  N_VScale(ONE, ls->x, b);
-  if (ls->scalesol && cv->gamrat != ONE)
+  /* the block LU's one-pass solve (fused kernels on): b solved in place and
+     scaled as it is written, the same bits as the calls below; declined
+     (SUN_ERR_ARG_INCOMPATIBLE) where they must run */
+  retval = SUN_ERR_ARG_INCOMPATIBLE;
+  if (cv->usefused && cv->fused_lsolve)
   {
-    N_VScale(TWO / (ONE + cv->gamrat), b, b);
+    const sunrealtype scale = (ls->scalesol && cv->gamrat != ONE) ? TWO / (ONE + cv->gamrat)
+                                                                   : ONE;
+    retval = cv->fused_lsolve(ls->LS, ls->A, b, scale);
+  }
+  if (retval == SUN_ERR_ARG_INCOMPATIBLE)
+  {
+    /* direct solver: zero tolerance, zero initial guess */
+    N_VConst(ZERO, ls->x);
+    retval = SUNLinSolSetZeroGuess(ls->LS, SUNTRUE);
+    if (retval != SUN_SUCCESS) { return -1; }
+    retval = SUNLinSolSolve(ls->LS, ls->A, ls->x, b, ZERO);
+    N_VScale(ONE, ls->x, b);
+    if (ls->scalesol && cv->gamrat != ONE)
+    {
+      N_VScale(TWO / (ONE + cv->gamrat), b, b);
+    }
   }
   if (retval != SUN_SUCCESS) { ls->ncfl++; }
   ls->last_flag = retval;

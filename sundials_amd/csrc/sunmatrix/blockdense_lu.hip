@@ -314,7 +314,8 @@ __global__ void __launch_bounds__(64 * LU_WAVES) k_getrf(const double* A, T* F /
 // x = LU^{-1} P b per block (x may alias b)
 template <int M, typename T>
 __global__ void __launch_bounds__(64) k_getrs(const T* __restrict__ F, const int* __restrict__ piv,
-                                              const double* b, double* x, long nblocks)
+                                              const double* b, double* x, long nblocks,
+                                              double scale = 1.0)
 {
   constexpr int G   = pow2ceil(M);
   constexpr int BPW = 64 / G;
@@ -373,7 +374,8 @@ __global__ void __launch_bounds__(64) k_getrs(const T* __restrict__ F, const int
     if (r < k) { bi = bi - lu[k] * bk; }
   }
   if (r == 0) { bi = bi / lu[0]; }
-  if (valid) { x[blk * M + r] = bi; }
+  // scale * x is N_VScale's per-element product (exact for scale = 1)
+  if (valid) { x[blk * M + r] = scale * bi; }
 }
 
 // generic-size fallback (any m <= 64): block staged in LDS, same arithmetic
@@ -656,7 +658,7 @@ static bool launch_getrf_scaled(int m, bool save_src, const double* src, double 
 
 template <typename T>
 static bool launch_getrs(int m, const T* F, const int* piv, const double* b, double* x,
-                         long nblocks, hipStream_t s)
+                         long nblocks, hipStream_t s, double scale = 1.0)
 {
   switch (m)
   {
@@ -665,14 +667,14 @@ static bool launch_getrs(int m, const T* F, const int* piv, const double* b, dou
   {                                                                                        \
     constexpr int BPW = 64 / pow2ceil(MM);                                                 \
     long g            = (nblocks + BPW - 1) / BPW;                                         \
-    k_getrs<MM, T><<<(unsigned)g, 64, 0, s>>>(F, piv, b, x, nblocks);                      \
+    k_getrs<MM, T><<<(unsigned)g, 64, 0, s>>>(F, piv, b, x, nblocks, scale);               \
     return true;                                                                           \
   }
     LU_SIZES(GETRS_CASE)
 #undef GETRS_CASE
   default: break;
   }
-  if (m > 64) { return false; }
+  if (m > 64 || scale != 1.0) { return false; }
   k_getrs_lds<T><<<(unsigned)nblocks, 64, 0, s>>>(F, piv, b, x, nblocks, m);
   return true;
 }
@@ -1124,6 +1126,53 @@ extern "C" int SUNLinSol_Mi355BlockLUSetupScaled(SUNLinearSolver S, SUNMatrix A,
   }
   sunamd_prof_end(SUNAMD_K_GETRF, lc->stream);
   return lu_finish(S);
+}
+
+// One pass instead of four: cvode_ls.c:1744-1784 (cvLsSolve) zeroes x, solves
+// A x = b, copies x into b and scales b by 2/(1 + gamrat); here b is solved in
+// place and each entry leaves the kernel as scale * x, the product N_VScale
+// forms (scale = 1 leaves x unchanged).  Declines with
+// SUN_ERR_ARG_INCOMPATIBLE, before touching anything, where the separate calls
+// must run instead.
+extern "C" int SUNLinSol_Mi355BlockLUSolveScaled(SUNLinearSolver S, SUNMatrix A, N_Vector b,
+                                                 sunrealtype scale)
+{
+  if (!S || !S->content || !A || !b || S->ops->solve != lu_solve || A->ops->getid != bm_getid)
+  {
+    return SUN_ERR_ARG_INCOMPATIBLE;
+  }
+  BlockLU* c = LU(S);
+  const int m = (int)c->m;
+  if (!SUNAMD_NVectorIsMi355(b) || (hipStream_t)N_VGetStream_Mi355(b) != c->stream ||
+      BM(A)->nblocks != c->nblocks || BM(A)->m != c->m || BM(A)->stream != c->stream ||
+      (m > 10 && m != 12 && m != 16 && m != 24 && m != 32 && m != 48 && m != 64))
+  {
+    return SUN_ERR_ARG_INCOMPATIBLE;
+  }
+  double* bd = N_VGetDeviceArrayPointer(b);
+  if (!bd) { return SUN_ERR_ARG_INCOMPATIBLE; }
+  bool ok;
+  sunamd_prof_begin(SUNAMD_K_GETRS, c->stream);
+  if (c->precision == SUNAMD_LU_FP32)
+  {
+    ok = launch_getrs<float>(m, c->f32, c->piv, bd, bd, (long)c->nblocks, c->stream, scale);
+  }
+  else
+  {
+    ok = launch_getrs<double>(m, BM(A)->dev, c->piv, bd, bd, (long)c->nblocks, c->stream,
+                              scale);
+  }
+  sunamd_prof_end(SUNAMD_K_GETRS, c->stream);
+  if (!ok) { return SUN_ERR_ARG_INCOMPATIBLE; }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+  {
+    hip_fail(S->sunctx, e, "k_getrs", __LINE__);
+    c->last_flag = SUN_ERR_EXT_FAIL;
+    return SUN_ERR_EXT_FAIL;
+  }
+  c->last_flag = SUN_SUCCESS;
+  return SUN_SUCCESS;
 }
 
 // ----------------------------------------------------------------------------

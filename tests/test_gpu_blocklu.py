@@ -203,6 +203,71 @@ def test_fused_setup_declines(libs, ctx):
     assert flag == SUNLS_LUFACT_FAIL
 
 
+@pytest.mark.parametrize("m", [1, 3, 8, 12, 24, 32, 48, 64])
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("scale", [1.0, 0.8125, 1.3719])
+def test_fused_scaled_solve(libs, ctx, m, prec, scale):
+    """SUNLinSol_Mi355BlockLUSolveScaled (b solved in place, written as
+    scale * x) leaves b as CVODE's calls do -- N_VConst(0, x), SUNLinSolSolve,
+    N_VScale(1, x, b), N_VScale(scale, b, b) -- bit for bit; in FP64 that is
+    the oracle's GETRS of each block times scale."""
+    core, hip, orc = libs
+    nb = 71
+    n = nb * m
+    rng = np.random.default_rng(31 * m + prec)
+    J = rng.uniform(-1, 1, (nb, m, m)) * 4.0 / m + np.eye(m)
+    y = gpu_vec(ctx, np.zeros(n))
+    MA = _gpu_matrix(hip, ctx, y, J)
+    LS = hip.SUNLinSol_Mi355BlockLU(y, MA, prec, ctx)
+    assert core.SUNLinSolInitialize(LS) == 0
+    assert core.SUNLinSolSetup(LS, MA) == 0
+    b = rng.uniform(-1, 1, n)
+    gb, gx, gr = gpu_vec(ctx, b), gpu_vec(ctx, np.ones(n)), gpu_vec(ctx, b)
+    core.N_VConst(0.0, gx)
+    assert core.SUNLinSolSolve(LS, MA, gx, gr, 0.0) == 0
+    core.N_VScale(1.0, gx, gr)
+    if scale != 1.0:
+        core.N_VScale(scale, gr, gr)
+    assert hip.SUNLinSol_Mi355BlockLUSolveScaled(LS, MA, gb, scale) == 0
+    got, ref = gpu_get(gb), gpu_get(gr)
+    assert np.array_equal(got, ref)
+    if prec == 0:
+        F = _gpu_matrix_values(hip, MA, nb, m)
+        piv = np.zeros(n, np.int32)
+        assert hip.SUNLinSol_Mi355BlockLU_GetPivots(LS, piv.ctypes.data_as(C.POINTER(C.c_int))) == 0
+        for k in range(0, nb, 7):
+            bb = b[k * m:(k + 1) * m].copy()
+            p = piv[k * m:(k + 1) * m].astype(np.int64)
+            orc.oracle_block_getrs(dp(F[k * m * m:(k + 1) * m * m].copy()), m,
+                                   p.ctypes.data_as(C.POINTER(C.c_int64)), dp(bb))
+            assert np.array_equal(got[k * m:(k + 1) * m], scale * bb), k
+    core.SUNLinSolFree(LS)
+    core.SUNMatDestroy(MA)
+    for v in (y, gb, gx, gr):
+        core.N_VDestroy(v)
+
+
+def test_fused_scaled_solve_declines(libs, ctx):
+    """Sizes without a register kernel decline before doing anything."""
+    core, hip, orc = libs
+    SUN_ERR_ARG_INCOMPATIBLE = -9998
+    m, nb = 11, 5
+    J = np.tile(np.eye(m) * 2.0, (nb, 1, 1))
+    y = gpu_vec(ctx, np.zeros(nb * m))
+    MA = _gpu_matrix(hip, ctx, y, J)
+    LS = hip.SUNLinSol_Mi355BlockLU(y, MA, 0, ctx)
+    assert core.SUNLinSolInitialize(LS) == 0
+    assert core.SUNLinSolSetup(LS, MA) == 0
+    b = np.arange(nb * m, dtype=np.float64)
+    gb = gpu_vec(ctx, b)
+    assert hip.SUNLinSol_Mi355BlockLUSolveScaled(LS, MA, gb, 1.0) == SUN_ERR_ARG_INCOMPATIBLE
+    assert np.array_equal(gpu_get(gb), b)
+    core.SUNLinSolFree(LS)
+    core.SUNMatDestroy(MA)
+    for v in (y, gb):
+        core.N_VDestroy(v)
+
+
 @pytest.mark.parametrize("m", [3, 8, 16, 32, 48, 64])
 def test_fp32_solve_residual(libs, ctx, m):
     core, hip, orc = libs
