@@ -1119,33 +1119,52 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 // out = act(in W^T + b) for the wavefront's 16 rows.  in: [16 x K] bf16 in
-// LDS (row stride ls); W: [N x kw] bf16 row-major in global memory (k >= kw
+// LDS (row stride ls); W: [N x KW] bf16 row-major in global memory (k >= KW
 // reads as zero); b: [N] FP32.  ACT: SiLU, rounded to bf16 into out_h (row
 // stride lh); otherwise the FP32 sums into out_f ([16 x 64]).  One
 // 16 x 16 x 32 bf16 MFMA per (16 output columns, 32-deep slice): lane l
 // supplies row l%16 of `in` and row n0 + l%16 of W for the 8 consecutive k
 // of its quarter l/16 (one 16-byte load each) and receives rows
 // 4(l/16) .. 4(l/16)+3 of column n0 + l%16; FP32 accumulation.
-template <int K, int N, bool ACT>
+template <int K, int N, int KW, bool ACT>
 __device__ __forceinline__ void w_layer(const unsigned short* in, int ls, const unsigned short* W,
-                                        int kw, const float* b, unsigned short* out_h, int lh,
+                                        const float* b, unsigned short* out_h, int lh,
                                         float* out_f)
 {
   const int l = __lane_id();
   const int r = l & 15, q = l >> 4;
+  // the lane's weight row and k offset; each slice's address is this plus a
+  // constant (the zero-padded slice reads the row's start and is then
+  // zeroed), so no per-slice address has to stay live across the layer
+  const unsigned short* wl = W + r * KW + 8 * q;
 #pragma unroll
   for (int n0 = 0; n0 < N; n0 += 16)
   {
+    // all slices' operands are requested before the first product (the
+    // scheduling barrier keeps the compiler from interleaving load, wait and
+    // product slice by slice)
+    constexpr int NS = K / 32;
+    bf16x8_t av[NS], wv[NS];
+#pragma unroll
+    for (int t = 0; t < NS; t++)
+    {
+      const int k0 = 32 * t;
+      const int k  = k0 + 8 * q;
+      av[t]        = *reinterpret_cast<const bf16x8_t*>(in + r * ls + k);
+      if (k0 + 32 <= KW)
+      {
+        wv[t] = *reinterpret_cast<const bf16x8_t*>(wl + n0 * KW + k0);
+      }
+      else
+      {
+        wv[t] = *reinterpret_cast<const bf16x8_t*>(wl + n0 * KW + (k < KW ? k0 : -8 * q));
+        if (k >= KW) { wv[t] = bf16x8_t{}; }
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
     f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int k0 = 0; k0 < K; k0 += 32)
-    {
-      const int k      = k0 + 8 * q;
-      const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(in + r * ls + k);
-      bf16x8_t wv      = {};
-      if (k < kw) { wv = *reinterpret_cast<const bf16x8_t*>(W + (long)(n0 + r) * kw + k); }
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wv, acc, 0, 0, 0);
-    }
+    for (int t = 0; t < NS; t++) { acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], wv[t], acc, 0, 0, 0); }
     const float bias = b[n0 + r];
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -1172,11 +1191,11 @@ __device__ __forceinline__ const float* w_forward16(const unsigned short* W, con
   unsigned short* H2       = H1 + 16 * MLP_HS;
   float* Y                 = reinterpret_cast<float*>(H1); // layer 1's outputs are dead by then
   gsync();
-  w_layer<160, 128, true>(X, MLP_XS, W, 144, b, H1, MLP_HS, nullptr);
+  w_layer<160, 128, 144, true>(X, MLP_XS, W, b, H1, MLP_HS, nullptr);
   gsync();
-  w_layer<128, 128, true>(H1, MLP_HS, W2, 128, b + 128, H2, MLP_HS, nullptr);
+  w_layer<128, 128, 128, true>(H1, MLP_HS, W2, b + 128, H2, MLP_HS, nullptr);
   gsync();
-  w_layer<128, 64, false>(H2, MLP_HS, W3, 128, b + 256, nullptr, 0, Y);
+  w_layer<128, 64, 128, false>(H2, MLP_HS, W3, b + 256, nullptr, 0, Y);
   gsync();
   return Y;
 }
