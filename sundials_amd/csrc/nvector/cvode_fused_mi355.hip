@@ -81,27 +81,69 @@ struct ZnPtrs
 typedef double dbl2_t __attribute__((ext_vector_type(2)));
 
 template <int Q, int SIGN>
+__device__ __forceinline__ void pascal_pair(dbl2_t (&z)[Q + 1])
+{
+#pragma unroll
+  for (int k = 1; k <= Q; k++)
+  {
+#pragma unroll
+    for (int j = Q; j >= k; j--) { z[j - 1] = (SIGN > 0) ? z[j - 1] + z[j] : z[j - 1] - z[j]; }
+  }
+}
+
+// Element pairs in chunks of PASCAL_U * 256 consecutive pairs per workgroup,
+// chunks strided over the grid; in a full chunk a lane requests all of its
+// pairs' history entries before the first sum (PASCAL_U * (Q + 1) 16-byte
+// loads in flight), the last, partial chunk goes pair by pair.
+#define PASCAL_U 4
+template <int Q, int SIGN>
 __global__ void __launch_bounds__(256) k_pascal(ZnPtrs Z, long n)
 {
-  const long np = n >> 1;
-  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < np; i += (long)gridDim.x * 256)
+  const long np    = n >> 1;
+  const long chunk = (long)PASCAL_U * 256;
+  for (long c = (long)blockIdx.x * chunk; c < np; c += (long)gridDim.x * chunk)
   {
-    dbl2_t z[Q + 1];
-#pragma unroll
-    for (int j = 0; j <= Q; j++)
+    const long i0 = c + threadIdx.x;
+    if (c + chunk <= np)
     {
-      z[j] = __builtin_nontemporal_load(reinterpret_cast<const dbl2_t*>(Z.p[j]) + i);
+      dbl2_t z[PASCAL_U][Q + 1];
+#pragma unroll
+      for (int u = 0; u < PASCAL_U; u++)
+      {
+#pragma unroll
+        for (int j = 0; j <= Q; j++)
+        {
+          z[u][j] = __builtin_nontemporal_load(reinterpret_cast<const dbl2_t*>(Z.p[j]) + i0 + u * 256);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < PASCAL_U; u++)
+      {
+        pascal_pair<Q, SIGN>(z[u]);
+#pragma unroll
+        for (int j = 0; j < Q; j++)
+        {
+          __builtin_nontemporal_store(z[u][j], reinterpret_cast<dbl2_t*>(Z.p[j]) + i0 + u * 256);
+        }
+      }
     }
-#pragma unroll
-    for (int k = 1; k <= Q; k++)
+    else
     {
+      for (long i = i0; i < np; i += 256)
+      {
+        dbl2_t z[Q + 1];
 #pragma unroll
-      for (int j = Q; j >= k; j--) { z[j - 1] = (SIGN > 0) ? z[j - 1] + z[j] : z[j - 1] - z[j]; }
-    }
+        for (int j = 0; j <= Q; j++)
+        {
+          z[j] = __builtin_nontemporal_load(reinterpret_cast<const dbl2_t*>(Z.p[j]) + i);
+        }
+        pascal_pair<Q, SIGN>(z);
 #pragma unroll
-    for (int j = 0; j < Q; j++)
-    {
-      __builtin_nontemporal_store(z[j], reinterpret_cast<dbl2_t*>(Z.p[j]) + i);
+        for (int j = 0; j < Q; j++)
+        {
+          __builtin_nontemporal_store(z[j], reinterpret_cast<dbl2_t*>(Z.p[j]) + i);
+        }
+      }
     }
   }
   if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0)
@@ -118,6 +160,27 @@ __global__ void __launch_bounds__(256) k_pascal(ZnPtrs Z, long n)
 #pragma unroll
     for (int j = 0; j < Q; j++) { Z.p[j][n - 1] = z[j]; }
   }
+}
+
+// chunked launch: one workgroup per chunk, at most 32 workgroups per CU
+unsigned pascal_grid(long npairs)
+{
+  static int cus = 0;
+  if (cus == 0)
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus < 1)
+    {
+      cus = 256;
+    }
+  }
+  const long chunk = (long)PASCAL_U * 256;
+  long g           = (npairs + chunk - 1) / chunk;
+  const long cap   = 32L * cus;
+  if (g > cap) { g = cap; }
+  return (unsigned)(g > 0 ? g : 1);
 }
 
 unsigned grid_for(long n)
@@ -200,7 +263,7 @@ static int pascal(int q, N_Vector* zn, int sign)
     Z.p[j] = N_VGetDeviceArrayPointer(zn[j]);
   }
   hipStream_t s     = (hipStream_t)N_VGetStream_Mi355(zn[0]);
-  const unsigned g  = grid_for((n + 1) / 2);
+  const unsigned g  = pascal_grid(n >> 1);
 #define PASCAL_CASE(QQ)                                                              \
   case QQ:                                                                           \
     if (sign > 0) { k_pascal<QQ, 1><<<g, 256, 0, s>>>(Z, n); }                       \
