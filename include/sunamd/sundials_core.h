@@ -164,6 +164,11 @@ SUNErrCode SUNNonlinSolGetNumConvFails(SUNNonlinearSolver NLS, long int* nconvfa
 
 /* Newton nonlinear solver (src/sunnonlinsol/newton/sunnonlinsol_newton.c) */
 SUNNonlinearSolver SUNNonlinSol_Newton(N_Vector y, SUNContext sunctx);
+/* No counterpart in the reference: the solve loop hands LSolve the residual
+   itself instead of its negation (for an LSolve that solves J x = -b); used by
+   the integrator's fused mode with the block LU's one-pass solve.
+   SUN_ERR_ARG_INCOMPATIBLE for another solver type. */
+SUNErrCode SUNNonlinSol_NewtonSetNegatedLSolve(SUNNonlinearSolver NLS, sunbooleantype on);
 
 /* small math helpers with the reference's semantics (sundials_math.c:37-52,
    sundials_math.h:83,107,235) */

@@ -222,6 +222,7 @@ typedef struct CVodeMemRec
      the library has none); declines with SUN_ERR_ARG_INCOMPATIBLE */
   int (*fused_lsetup)(SUNLinearSolver, SUNMatrix, SUNMatrix, sunrealtype, int);
   int (*fused_lsolve)(SUNLinearSolver, SUNMatrix, N_Vector, sunrealtype);
+  sunbooleantype nls_negated; /* the Newton solver hands LSolve r, not -r */
 
   sunbooleantype MallocDone;
 } CVodeMemRec;
@@ -356,6 +357,7 @@ int CVodeSetNonlinearSolver(void* cvode_mem, SUNNonlinearSolver NLS)
   if (cv->NLS && cv->ownNLS) { SUNNonlinSolFree(cv->NLS); }
   cv->NLS    = NLS;
   cv->ownNLS = SUNFALSE;
+  cv->nls_negated = SUNFALSE; /* the new solver has not been told */
   if (SUNNonlinSolGetType(NLS) != SUNNONLINEARSOLVER_ROOTFIND) { return CV_ILL_INPUT; }
   if (SUNNonlinSolSetSysFn(NLS, cvNlsResidual) != SUN_SUCCESS) { return CV_ILL_INPUT; }
   if (SUNNonlinSolSetConvTestFn(NLS, cvNlsConvTest, cv) != SUN_SUCCESS)
@@ -796,10 +798,12 @@ static int cvLsSolve(CVodeMem cv, N_Vector b, N_Vector weight, N_Vector ynow,
   {
     const sunrealtype scale = (ls->scalesol && cv->gamrat != ONE) ? TWO / (ONE + cv->gamrat)
                                                                    : ONE;
-    retval = cv->fused_lsolve(ls->LS, ls->A, b, scale);
+    retval = cv->fused_lsolve(ls->LS, ls->A, b, cv->nls_negated ? -scale : scale);
   }
   if (retval == SUN_ERR_ARG_INCOMPATIBLE)
   {
+    /* the Newton solver left the residual un-negated for the one-pass solve */
+    if (cv->nls_negated) { N_VScale(-ONE, b, b); }
     /* direct solver: zero tolerance, zero initial guess */
     N_VConst(ZERO, ls->x);
     retval = SUNLinSolSetZeroGuess(ls->LS, SUNTRUE);
@@ -1273,6 +1277,18 @@ static int cvNls(CVodeMem cv, int nflag)
     if (flag > 0) { return SUN_NLS_CONV_RECVR; }
   }
 
+  /* fused mode with the one-pass solve: the residual's negation moves into
+     the solve (the solution of -r is the negated solution of r, bit for bit) */
+  {
+    const sunbooleantype neg = cv->usefused && cv->fused_lsolve && cv->ownNLS;
+    if (neg != cv->nls_negated)
+    {
+      /* a solver of another type negates the residual itself */
+      cv->nls_negated = (SUNNonlinSol_NewtonSetNegatedLSolve(cv->NLS, neg) == SUN_SUCCESS)
+                          ? neg
+                          : SUNFALSE;
+    }
+  }
   flag = SUNNonlinSolSolve(cv->NLS, cv->zn[0], cv->acor, cv->ewt, cv->tq[4],
                            callSetup, cv);
 

@@ -30,6 +30,7 @@ typedef struct
   long int niters;
   long int nconvfails;
   void* ctest_data;
+  sunbooleantype negated_lsolve; /* LSolve returns the solution for -b */
 } NewtonContent;
 
 #define NC(NLS) ((NewtonContent*)((NLS)->content))
@@ -83,7 +84,7 @@ static int newton_solve(SUNNonlinearSolver NLS, N_Vector y0, N_Vector ycor,
     for (;;)
     {
       c->niters++;
-      N_VScale(SUN_RCONST(-1.0), delta, delta);
+      if (!c->negated_lsolve) { N_VScale(SUN_RCONST(-1.0), delta, delta); }
 
       retval = c->LSolve(delta, mem);
       if (retval != SUN_SUCCESS) { break; }
@@ -122,6 +123,18 @@ static int newton_solve(SUNNonlinearSolver NLS, N_Vector y0, N_Vector ycor,
 
   c->nconvfails++;
   return retval;
+}
+
+/* No counterpart in the reference.  With on, the solve loop hands LSolve the
+   residual itself instead of its negation (the N_VScale(-1) of
+   sunnonlinsol_newton.c:239 is skipped): for an integrator whose LSolve solves
+   J x = -b, the negation folded into the solve.  The solution of -b is the
+   negated solution of b, bit for bit, so the iterates are unchanged. */
+SUNErrCode SUNNonlinSol_NewtonSetNegatedLSolve(SUNNonlinearSolver NLS, sunbooleantype on)
+{
+  if (!NLS || !NLS->content || NLS->ops->solve != newton_solve) { return SUN_ERR_ARG_INCOMPATIBLE; }
+  NC(NLS)->negated_lsolve = on ? SUNTRUE : SUNFALSE;
+  return SUN_SUCCESS;
 }
 
 static SUNErrCode newton_free(SUNNonlinearSolver NLS)
